@@ -1,0 +1,161 @@
+/*
+ * simulato_hip.h — the C ABI of the MI355X (gfx950) multislice STEM library.
+ *
+ * A `.hip` backend of SimulatoKit sits beside `.cpu` and `.metal` in `Backend` (Sources/SimulatoMultislice/
+ * Multislice.swift:43-46) and reaches the GPU only through these entry points, which replace the Metal engine's
+ * internal interface (Sources/SimulatoMultislice/MetalEngine.swift, MetalMultislice.swift). Plain C types only.
+ *
+ * Conventions (as the reference's Grid/Waves, Sources/SimulatoCore/Grid.swift:7-91, Fields.swift:8-29):
+ *   - arrays are row-major [ix * ny + iy], x the first index;
+ *   - host complex arrays are split re/im float64 (the layout of `ComplexField`);
+ *   - forward FFT unscaled e^{-2 pi i}, inverse scaled 1/(nx ny) (FFT2D.swift:198-215);
+ *   - every entry point returns SK_OK (0) or a negative status; the message is in sk_hip_last_error()
+ *     (per thread), as SimulatoError.metalUnavailable / .metalFailure carry theirs (SimulatoError.swift:22-23).
+ *   - host arrays are caller-owned and read or written only during the call; device memory is owned by handles.
+ *   - calls are serialised by a process-wide lock (the reference's `runLock`, MetalEngine.swift:51).
+ */
+#ifndef SIMULATO_HIP_H
+#define SIMULATO_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum sk_status {
+  SK_OK = 0,
+  SK_ERR_UNAVAILABLE = -1, /* no HIP device (SimulatoError.metalUnavailable) */
+  SK_ERR_INVALID = -2,     /* a refused input (the SimulatoError cases thrown before work) */
+  SK_ERR_DEVICE = -3,      /* a HIP runtime failure (SimulatoError.metalFailure) */
+  SK_ERR_MEMORY = -4       /* a device allocation failed */
+};
+
+/* An nx x ny lateral grid over extent_x x extent_y Angstrom (Grid.swift:8-27). */
+typedef struct sk_grid {
+  int nx, ny;
+  double extent_x, extent_y;
+} sk_grid;
+
+/* A device-resident prepared potential: per slice the bandlimited transmission function (complex64) and the
+ * propagators of its distinct slice thicknesses, for one beam energy and tilt. Immutable after creation; may be
+ * shared between calls (MetalPreparedPotential, MetalEngine.swift:277-332). */
+typedef struct sk_potential sk_potential;
+
+/* The probe (Sources/SimulatoProbe/Probe.swift:12-69, Aberrations.swift:10-60). aberrations[] in the order
+ * C10 C12 phi12 C21 phi21 C23 phi23 C30 C32 phi32 C34 phi34 C41 phi41 C43 phi43 C45 phi45 C50 C52 phi52 C54 phi54
+ * C56 phi56 (Angstrom, rad). semiangle_mrad may be +infinity (no aperture). */
+typedef struct sk_probe {
+  double energy;          /* eV */
+  double semiangle_mrad;  /* aperture cutoff */
+  int soft;               /* abTEM's soft edge (default 1) */
+  double aberrations[25];
+  double tilt[2];         /* mrad, enters the propagators only */
+} sk_probe;
+
+/* Detector kinds (Sources/SimulatoDetector/Detectors.swift:91-206). */
+enum sk_detector_kind { SK_DET_ANNULAR = 0, SK_DET_PIXELATED = 1, SK_DET_COM = 2 };
+enum sk_max_angle { SK_ANGLE_CUTOFF = 0, SK_ANGLE_VALID = 1, SK_ANGLE_FULL = 2 };
+
+typedef struct sk_detector {
+  int kind;              /* sk_detector_kind */
+  double inner;          /* annular: inner angle, mrad */
+  double outer;          /* annular: outer angle, mrad; < 0 = abTEM's default (floor of the cutoff angle) */
+  int max_angle;         /* pixelated / COM region: sk_max_angle */
+  int units;             /* COM: 0 = mrad, 1 = 1/Angstrom */
+  int crop_nx, crop_ny;  /* pixelated: a numeric centred crop of the fftshifted pattern (0, 0 = by max_angle) */
+  int out_f32;           /* 1: this detector's output array is float32 instead of float64 */
+} sk_detector;
+
+/* Execution knobs (ExecutionOptions.swift:21-49). None changes which numbers are computed, only their rounding. */
+typedef struct sk_options {
+  int batch;        /* waves resident per launch; 0 = the library's choice for the grid */
+  int use_graph;    /* capture each batch's slice loop in a hipGraph and replay it */
+  int fast_path;    /* 1 (default): register-resident FFT passes where the length has one; 0: generic LDS passes */
+  int reserved[5];
+} sk_options;
+
+/* --- availability (MetalEngine.require / Multislice.metalIsAvailable, MetalEngine.swift:63-77,
+ *     MetalMultislice.swift:13-21) ------------------------------------------------------------------------- */
+int sk_hip_available(void);
+const char* sk_hip_last_error(void);
+
+/* --- potentials --------------------------------------------------------------------------------------------- */
+
+/* From projected potentials V [nslices x nx x ny] in V*Angstrom and thicknesses dz [nslices]
+ * (MetalMultislice.prepare(_: ProjectedPotential, ...), MetalEngine.swift:310-320; Multislice.transmissionFunctions
+ * and .propagator, Multislice.swift:83-130). The transmission phase sigma*V and the propagator phase
+ * -pi*lambda*k^2*dz are evaluated in float64 on the device and rounded to complex64. */
+int sk_hip_potential_from_values(const sk_grid* grid, int nslices, const double* V, const double* dz,
+                                 double energy, const double tilt[2], sk_potential** out);
+
+/* From atoms already wrapped and sliced by the caller (Potential.init, Sources/SimulatoMultislice/Potential.swift:
+ * 251-265): numbers Z [natoms], positions xy [natoms x 2] in Angstrom, slice_of_atom [natoms]. The slices are built
+ * on the device (MetalMultislice.prepare(_: Potential, ...), MetalEngine.swift:254-308; SliceBuilder,
+ * ProjectedPotential.swift:109-202): bilinear deltas merged per pixel in float64 on the host, FFT, times the Lobato
+ * f(k^2)/S(k) of each element, inverse FFT, real part. pi_mode 0 = exact pi, 1 = abTEM's float32 pi. */
+int sk_hip_potential_from_atoms(const sk_grid* grid, int nslices, const double* dz, int natoms, const int* numbers,
+                                const double* xy, const int* slice_of_atom, int pi_mode, double energy,
+                                const double tilt[2], sk_potential** out);
+
+void sk_hip_potential_release(sk_potential* p);
+int sk_hip_potential_slice_count(const sk_potential* p);
+
+/* Test hooks (Multislice.gpuPotentialSlices, MetalMultislice.swift:104-107): V of every slice as built on the
+ * device [nslices x nx x ny] (atoms form only), and the transmission functions as stored (complex64 widened). */
+int sk_hip_potential_values(const sk_potential* p, double* V);
+int sk_hip_potential_transmissions(const sk_potential* p, double* t_re, double* t_im);
+
+/* --- multislice ------------------------------------------------------------------------------------------------ */
+
+/* One wave through the potential (MetalMultislice.run, MetalEngine.swift:458-473; Multislice.run,
+ * Multislice.swift:52-80). planes: ascending zero-based slice indices, -1 = the incident wave (ExitPlanes,
+ * Multislice.swift:13-40). out_re/out_im: nplanes x nx x ny. */
+int sk_hip_run(const sk_potential* p, const double* in_re, const double* in_im, const int* planes, int nplanes,
+               double* out_re, double* out_im, const sk_options* opt);
+
+/* Exit waves after the last slice of `count` incident waves (MetalMultislice.runBatch, MetalEngine.swift:476-500;
+ * Multislice.runBatch, MetalMultislice.swift:52-83). in/out: count x nx x ny, input order kept. */
+int sk_hip_run_batch(const sk_potential* p, int count, const double* in_re, const double* in_im, double* out_re,
+                     double* out_im, const sk_options* opt);
+
+/* --- probes ------------------------------------------------------------------------------------------------- */
+
+/* The probes at `npos` positions [npos x 2] Angstrom as the scan builds them (Probe.build(at:),
+ * Probe.swift:72-109; Multislice.gpuProbes, MetalMultislice.swift:110-113): out npos x nx x ny, real space,
+ * sum_k |Psi(k)|^2 = 1. */
+int sk_hip_probes(const sk_grid* grid, const sk_probe* probe, int npos, const double* positions, double* out_re,
+                  double* out_im);
+
+/* --- STEM scan ----------------------------------------------------------------------------------------------- */
+
+/* STEM.scan through `npots` potentials (one per frozen-phonon configuration; slice counts may differ)
+ * (STEM.swift:49-81, 120-150; Multislice.metalScanIntensities, MetalEngine.swift:583-676): every position's probe
+ * through every potential, each detector reduced on the device from |FFT(psi_exit)|^2 (float64 accumulation),
+ * summed in potential order and divided by npots. outputs[d] receives npos x count_d values (count: 1 annular,
+ * 2 COM (x, y), mx*my pixelated), float64 or float32 per sk_detector.out_f32; outputs_on_device = 1 when the
+ * output pointers are device memory (the results then never leave HBM). */
+int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* probe, int npos, const double* positions,
+                const sk_detector* dets, int ndets, void* const* outputs, int outputs_on_device,
+                const sk_options* opt);
+
+/* The values each detector returns per position (Detector.shape(for:beam:), Detectors.swift:84-88): writes the
+ * shape (up to 2 ints) and returns the count, or a negative status for a refused detector. */
+int sk_hip_detector_shape(const sk_grid* grid, double energy, const sk_detector* det, int* shape);
+
+/* --- measurement --------------------------------------------------------------------------------------------- */
+
+/* Device time of the dominant kernels of the last sk_hip_scan on the calling thread, measured with HIP events on
+ * the library's stream: mean milliseconds per launch of the row pass and of the column pass, their launch counts,
+ * and the bytes each launch moved by the algorithmic definition (complex64 wave read + written once). */
+typedef struct sk_timing {
+  double row_ms, col_ms;
+  long long row_launches, col_launches;
+  double row_bytes, col_bytes;
+  double total_ms;
+} sk_timing;
+int sk_hip_last_timing(sk_timing* t);
+void sk_hip_set_timing(int enabled);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SIMULATO_HIP_H */

@@ -1,0 +1,1169 @@
+// sk_api.hip — the C ABI (include/simulato_hip.h): potentials, multislice runs and the STEM scan on one MI355X.
+//
+// The host side here replaces the reference's Metal engine (MetalEngine.swift, MetalMultislice.swift): it prepares
+// a potential once (transmission functions and compact propagators resident in HBM), then streams batches of waves
+// through the row/column passes of every slice, reducing the detectors inside the final pass.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "sk_internal.h"
+
+struct sk_potential {
+  sk_grid grid{};
+  int nslices = 0;
+  double energy = 0, lambda = 0, sigma = 0;
+  double tilt[2] = {0, 0};
+  std::vector<double> dz;
+  int nkept = 0, nc = 0;  // kept columns, compact width
+  std::vector<int> keep, keepinv;
+  sk::DevBuf<int> keep_d, keepinv_d;
+  sk::DevBuf<float2> t;                // nslices x N
+  std::vector<int> prop_of_slice;      // index into the compact propagators
+  std::vector<double> prop_dz;
+  sk::DevBuf<float2> props;            // nprops x nx x nc, scaled by 1/N
+  sk::DevBuf<double> aperture;         // N
+  sk::DevBuf<double> V;                // nslices x N (atoms form only)
+  bool has_values = false;
+};
+
+namespace sk {
+namespace {
+
+// ---- process context ------------------------------------------------------------------------------------------
+struct Context {
+  bool init = false;
+  int status = SK_ERR_UNAVAILABLE;
+  std::string why;
+  hipStream_t stream = nullptr;
+  DevBuf<double> lobato;  // 103 x 10
+  // timing
+  bool timing = false;
+  sk_timing last{};
+  // identity keep maps per length (full-width transforms)
+  std::map<int, std::unique_ptr<DevBuf<int>>> identity;
+};
+
+std::recursive_mutex& lock() {
+  static std::recursive_mutex m;
+  return m;
+}
+
+Context& ctx() {
+  // process lifetime on purpose: never destroyed, so no HIP call runs after the runtime has shut down
+  static Context* c = new Context;
+  return *c;
+}
+
+thread_local std::string g_error;
+
+struct LobatoRow {
+  int present;
+  double a[5];
+  double b[5];
+};
+// Lobato & Van Dyck (2014) parameters (a1..a5, b1..b5) per Z = 1 ... 103
+const LobatoRow kLobato[103] = {
+#include "lobato_table.inc"
+};
+
+Context& require() {
+  Context& c = ctx();
+  if (!c.init) {
+    c.init = true;
+    const char* hide = std::getenv("SK_HIP_HIDE_DEVICE");
+    int n = 0;
+    if (hide && hide[0] && hide[0] != '0') {
+      c.status = SK_ERR_UNAVAILABLE;
+      c.why = "the HIP device is hidden by SK_HIP_HIDE_DEVICE";
+    } else if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+      c.status = SK_ERR_UNAVAILABLE;
+      c.why = "no HIP device";
+    } else {
+      hipError_t e = hipSetDevice(0);
+      if (e == hipSuccess) e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
+      if (e != hipSuccess) {
+        c.status = SK_ERR_UNAVAILABLE;
+        c.why = std::string("HIP initialisation failed: ") + hipGetErrorString(e);
+      } else {
+        c.status = SK_OK;
+        std::vector<double> tab(103 * 10);
+        for (int z = 0; z < 103; ++z)
+          for (int k = 0; k < 5; ++k) {
+            tab[z * 10 + k] = kLobato[z].a[k];
+            tab[z * 10 + 5 + k] = kLobato[z].b[k];
+          }
+        c.lobato.upload(tab, c.stream);
+        SK_HIP(hipStreamSynchronize(c.stream));
+      }
+    }
+  }
+  if (c.status != SK_OK) fail(SK_ERR_UNAVAILABLE, "the HIP backend is unavailable: " + c.why);
+  return c;
+}
+
+template <class F>
+int guarded(F&& f) {
+  std::lock_guard<std::recursive_mutex> g(lock());
+  try {
+    f();
+    return SK_OK;
+  } catch (const Error& e) {
+    g_error = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return SK_ERR_DEVICE;
+  }
+}
+
+// ---- host geometry (Grid.swift, Antialias.swift, Detectors.swift) -----------------------------------------------
+std::vector<double> fftfreq(int n, double d) {
+  const double val = 1.0 / (double(n) * d);
+  const int positive = (n - 1) / 2 + 1;
+  std::vector<double> o(n);
+  for (int i = 0; i < n; ++i) o[i] = double(i < positive ? i : i - n) * val;
+  return o;
+}
+
+double wavelength(double energy) {  // ElectronBeam.swift:129-134
+  const double h = 6.62607004e-34, me = 9.10938356e-31, c = 299792458.0, e = 1.6021766208e-19;
+  const double c2 = c * c;
+  return h * c / std::sqrt(energy * (2 * me * c2 / e + energy)) / e * 1.0e10;
+}
+
+double sigma_of(double energy) {  // ElectronBeam.swift:115-121
+  const double h = 6.62607004e-34, me = 9.10938356e-31, c = 299792458.0, e = 1.6021766208e-19;
+  const double kg = 6.0221408585491615e+26, coulomb = 6.241509125883258e+18, second = 98226947884640.62,
+               joule = 6.241509125883258e+18;
+  const double mass = (1 + e * energy / (me * (c * c))) * me;
+  const double hsj = h * second * joule;
+  return 2 * M_PI * mass * kg * e * coulomb * wavelength(energy) / (hsj * hsj);
+}
+
+void check_grid(const sk_grid* g) {
+  if (!g) fail(SK_ERR_INVALID, "no grid");
+  if (g->nx < 2 || g->ny < 2) fail(SK_ERR_INVALID, "grid lengths must be at least 2");
+  if (!(std::isfinite(g->extent_x) && std::isfinite(g->extent_y) && g->extent_x > 0 && g->extent_y > 0))
+    fail(SK_ERR_INVALID, "grid extent must be finite and positive");
+}
+
+bool same_grid(const sk_grid& a, const sk_grid& b) {  // Grid.matches (Grid.swift:73-79)
+  auto close = [](double x, double y) {
+    return x == y || std::fabs(x - y) <= 1e-12 * std::max(std::fabs(x), std::fabs(y));
+  };
+  return a.nx == b.nx && a.ny == b.ny && close(a.extent_x, b.extent_x) && close(a.extent_y, b.extent_y);
+}
+
+// The columns whose ky lies inside the antialias disk (P is zero on every other column).
+void kept_columns(const sk_grid& g, std::vector<int>& keep, std::vector<int>& keepinv, int& nkept, int& nc) {
+  const double sx = g.extent_x / g.nx, sy = g.extent_y / g.ny;
+  const double kc = 0.6666666 / std::max(sx, sy) / 2;
+  auto ky = fftfreq(g.ny, sy);
+  keep.clear();
+  keepinv.assign(g.ny, -1);
+  for (int iy = 0; iy < g.ny; ++iy)
+    if (std::fabs(ky[iy]) <= kc) {
+      keepinv[iy] = (int)keep.size();
+      keep.push_back(iy);
+    }
+  nkept = (int)keep.size();
+  nc = std::min(g.ny, (nkept + 15) / 16 * 16);
+  if (nc < nkept) nc = nkept;
+  while ((int)keep.size() < nc) keep.push_back(-1);
+}
+
+double round_decimals(double x, int d) {
+  const double f = std::pow(10.0, double(d));
+  return std::nearbyint(x * f) / f;
+}
+
+void cutoff_gpts(const sk_grid& g, int& mx, int& my) {  // Detectors.swift:54-62
+  const double sx = g.extent_x / g.nx, sy = g.extent_y / g.ny;
+  const double kcut = 2.0 / 3.0 / std::max(sx, sy);
+  auto size = [&](int n, double extent) {
+    int m = (int)std::floor(round_decimals(kcut * extent, 7));
+    if (m % 2 != n % 2) m += 1;
+    return m;
+  };
+  mx = size(g.nx, double(g.nx) * sx);
+  my = size(g.ny, double(g.ny) * sy);
+}
+
+void pixelated_size(const sk_grid& g, const sk_detector& d, int& mx, int& my) {  // Detectors.swift:146-159
+  if (d.crop_nx > 0 || d.crop_ny > 0) {
+    mx = d.crop_nx;
+    my = d.crop_ny;
+    return;
+  }
+  if (d.max_angle == SK_ANGLE_FULL) {
+    mx = g.nx;
+    my = g.ny;
+    return;
+  }
+  int cx, cy;
+  cutoff_gpts(g, cx, cy);
+  if (d.max_angle == SK_ANGLE_CUTOFF) {
+    mx = cx;
+    my = cy;
+    return;
+  }
+  auto v = [](int m, int n) {
+    int k = (int)std::floor(round_decimals(double(m) / std::sqrt(2.0), 7));
+    if (k % 2 != n % 2) k += 1;
+    return k;
+  };
+  mx = v(cx, g.nx);
+  my = v(cy, g.ny);
+}
+
+std::vector<int> crop_indices(int n, int m) {  // Waves.swift:90-96
+  std::vector<int> o;
+  if (m >= n) {
+    for (int i = 0; i < n; ++i) o.push_back(i);
+    return o;
+  }
+  const int low = m % 2 == 1 ? m / 2 + 1 : m / 2, high = m / 2;
+  for (int i = 0; i < low; ++i) o.push_back(i);
+  for (int i = n - high; i < n; ++i) o.push_back(i);
+  return o;
+}
+
+// source index -> fftshifted crop index, or -1 (Detectors.swift:70-80)
+std::vector<int> crop_map(int n, int m) {
+  auto s = crop_indices(n, m);
+  std::vector<int> map(n, -1);
+  const int cnt = (int)s.size();
+  for (int c = 0; c < cnt; ++c) map[s[c]] = (c + cnt / 2) % cnt;
+  return map;
+}
+
+double annular_outer(const sk_grid& g, double lambda, const sk_detector& d) {  // Detectors.swift:108-117
+  const double rx = 1 / (double(g.nx) * (g.extent_x / g.nx)), ry = 1 / (double(g.ny) * (g.extent_y / g.ny));
+  const double ax = rx * lambda * 1e3, ay = ry * lambda * 1e3;
+  const double maximum = std::min(double(g.nx / 2) * ax, double(g.ny / 2) * ay);
+  if (d.outer < 0) {
+    int cx, cy;
+    cutoff_gpts(g, cx, cy);
+    return std::floor(std::min(double(cx / 2) * ax, double(cy / 2) * ay));
+  }
+  const double o = d.outer;
+  if (o > maximum && std::fabs(o - maximum) > 1e-5 + 1e-5 * std::fabs(o))
+    fail(SK_ERR_INVALID, "invalid detector: outer " + std::to_string(o) + " mrad exceeds the maximum simulated angle " +
+                             std::to_string(maximum) + " mrad");
+  return o;
+}
+
+int detector_count(const sk_grid& g, double lambda, const sk_detector& d, int* shape) {
+  if (d.kind == SK_DET_ANNULAR) {
+    if (!(std::isfinite(d.inner) && d.inner >= 0)) fail(SK_ERR_INVALID, "invalid detector: inner must be finite and >= 0");
+    const double o = annular_outer(g, lambda, d);
+    if (d.inner > o) fail(SK_ERR_INVALID, "invalid detector: inner exceeds the outer angle");
+    return 1;
+  }
+  if (d.kind == SK_DET_COM) {
+    if (shape) shape[0] = 2;
+    return 2;
+  }
+  if (d.kind == SK_DET_PIXELATED) {
+    int mx, my;
+    pixelated_size(g, d, mx, my);
+    if (mx < 1 || my < 1 || mx > g.nx + 1 || my > g.ny + 1) fail(SK_ERR_INVALID, "invalid detector: crop size");
+    if (shape) {
+      shape[0] = mx;
+      shape[1] = my;
+    }
+    return mx * my;
+  }
+  fail(SK_ERR_INVALID, "invalid detector kind");
+}
+
+// ---- FFT pass helpers (full width: identity maps) ----------------------------------------------------------------
+const int* identity_map(int n) {
+  Context& c = ctx();
+  auto it = c.identity.find(n);
+  if (it != c.identity.end()) return it->second->p;
+  auto b = std::make_unique<DevBuf<int>>();
+  std::vector<int> v(n);
+  for (int i = 0; i < n; ++i) v[i] = i;
+  b->upload(v, c.stream);
+  const int* p = b->p;
+  c.identity[n] = std::move(b);
+  return p;
+}
+
+void row_pass(const RowArgs& a, int batch, bool fast, hipStream_t s) {
+  if (fast && fast_row_supported(a.ny, a.nc, a.flags)) launch_row_fast(a, batch, s);
+  else launch_row_generic(a, batch, s);
+}
+
+void col_pass(const ColArgs& a, int batch, bool fast, hipStream_t s) {
+  if (fast && fast_col_supported(a.nx, a.nc, a.flags)) launch_col_fast(a, batch, s);
+  else launch_col_generic(a, batch, s);
+}
+
+// Full 2-D transform of `batch` waves in place (buf: batch x nx x ny): forward unscaled (sign -1) or inverse with
+// `inv_scale`, optionally multiplying the spectrum by the full-width P between the column transforms.
+void fft2_batch(float2* buf, int nx, int ny, int batch, int sign, const float2* P, float scale, hipStream_t s) {
+  RowArgs r{};
+  r.nx = nx;
+  r.ny = ny;
+  r.nc = ny;
+  r.fy = line_plan(ny, s).dev;
+  r.keep = identity_map(ny);
+  r.keepinv = r.keep;
+  r.in_full = buf;
+  r.out_full = buf;
+  r.in_full_stride = r.out_full_stride = (long long)nx * ny;
+  r.out_scale = 1.0f;
+  ColArgs c{};
+  c.nx = nx;
+  c.ny = ny;
+  c.nc = ny;
+  c.fx = line_plan(nx, s).dev;
+  c.keep = r.keep;
+  c.G = buf;
+  c.P = P;
+  if (sign < 0) {
+    r.flags = ROW_IN_FULL | ROW_FWD | ROW_OUT_FULL;
+    row_pass(r, batch, false, s);
+    c.flags = COL_IN_G | COL_FWD | COL_OUT | (P ? (COL_MUL_P | COL_INV) : 0);
+    col_pass(c, batch, false, s);
+    if (P) {
+      r.flags = ROW_IN_FULL | ROW_INV | ROW_OUT_FULL;
+      r.out_scale = scale;
+      row_pass(r, batch, false, s);
+    }
+  } else {
+    c.flags = COL_IN_G | COL_INV | COL_OUT;
+    col_pass(c, batch, false, s);
+    r.flags = ROW_IN_FULL | ROW_INV | ROW_OUT_FULL;
+    r.out_scale = scale;
+    row_pass(r, batch, false, s);
+  }
+}
+
+// ---- potential preparation ---------------------------------------------------------------------------------------
+std::unique_ptr<sk_potential> new_potential(const sk_grid* grid, int nslices, const double* dz, double energy,
+                                            const double tilt[2]) {
+  check_grid(grid);
+  if (nslices < 1) fail(SK_ERR_INVALID, "a potential needs at least one slice");
+  if (!(std::isfinite(energy) && energy > 0)) fail(SK_ERR_INVALID, "electron energy must be finite and positive");
+  for (int j = 0; j < nslices; ++j)
+    if (!(std::isfinite(dz[j]) && dz[j] > 0)) fail(SK_ERR_INVALID, "slice thickness must be finite and positive");
+  if (tilt && !(std::isfinite(tilt[0]) && std::isfinite(tilt[1]))) fail(SK_ERR_INVALID, "invalid tilt");
+  auto p = std::make_unique<sk_potential>();
+  p->grid = *grid;
+  p->nslices = nslices;
+  p->energy = energy;
+  p->lambda = wavelength(energy);
+  p->sigma = sigma_of(energy);
+  if (tilt) {
+    p->tilt[0] = tilt[0];
+    p->tilt[1] = tilt[1];
+  }
+  p->dz.assign(dz, dz + nslices);
+  return p;
+}
+
+// transmissions and propagators from V (device, float64, nslices x N)
+void finish_potential(sk_potential& p, const double* V_dev) {
+  Context& c = ctx();
+  hipStream_t s = c.stream;
+  const int nx = p.grid.nx, ny = p.grid.ny;
+  const size_t N = (size_t)nx * ny;
+  kept_columns(p.grid, p.keep, p.keepinv, p.nkept, p.nc);
+  p.keep_d.upload(p.keep, s);
+  p.keepinv_d.upload(p.keepinv, s);
+  p.aperture.alloc(N);
+  launch_antialias(nx, ny, p.grid.extent_x, p.grid.extent_y, p.aperture.p, s);
+  // t = IFFT2(FFT2(e^{i sigma V}) * A) (Multislice.swift:83-103), in chunks of slices
+  p.t.alloc((size_t)p.nslices * N);
+  DevBuf<float2> A;
+  A.alloc(N);
+  launch_aperture_as_complex(nx, ny, p.aperture.p, 1.0, A.p, s);
+  for (int j = 0; j < p.nslices; ++j) launch_phase_to_complex(V_dev + j * N, p.sigma, p.t.p + j * N, N, s);
+  const int chunk = std::max<int>(1, (int)std::min<size_t>(p.nslices, (size_t(512) << 20) / (N * sizeof(float2))));
+  for (int j0 = 0; j0 < p.nslices; j0 += chunk) {
+    const int cnt = std::min(chunk, p.nslices - j0);
+    fft2_batch(p.t.p + (size_t)j0 * N, nx, ny, cnt, -1, A.p, float(1.0 / double(N)), s);
+  }
+  // propagators per distinct thickness (Multislice.swift:136, 145-148), compact, scaled by 1/N
+  std::map<double, int> index;
+  p.prop_of_slice.resize(p.nslices);
+  for (int j = 0; j < p.nslices; ++j) {
+    auto it = index.find(p.dz[j]);
+    if (it == index.end()) {
+      const int k = (int)p.prop_dz.size();
+      index[p.dz[j]] = k;
+      p.prop_dz.push_back(p.dz[j]);
+      p.prop_of_slice[j] = k;
+    } else {
+      p.prop_of_slice[j] = it->second;
+    }
+  }
+  p.props.alloc(p.prop_dz.size() * (size_t)nx * p.nc);
+  for (size_t k = 0; k < p.prop_dz.size(); ++k)
+    launch_propagator(nx, ny, p.grid.extent_x, p.grid.extent_y, p.lambda, p.prop_dz[k], p.tilt[0], p.tilt[1],
+                      p.aperture.p, p.keep_d.p, p.nc, 1.0 / double(N), p.props.p + k * (size_t)nx * p.nc, s);
+  SK_HIP(hipStreamSynchronize(s));
+}
+
+// ---- multislice driver -------------------------------------------------------------------------------------------
+struct Workspace {
+  DevBuf<float2> G, full, planes, rx, ry, spec;
+  DevBuf<double> partial, acc, positions, tmp_re, tmp_im, norm, norm_partial;
+  DevBuf<int> flag;
+  std::vector<DevBuf<double>> pix;
+  DevBuf<unsigned> mask;
+  std::vector<DevBuf<int>> maps;
+  std::vector<DevBuf<double>> weights;
+};
+
+Workspace& ws() {
+  static Workspace* w = new Workspace;  // process lifetime (see ctx())
+  return *w;
+}
+
+int default_batch(const sk_potential& p, int count, const sk_options* opt) {
+  int b = opt && opt->batch > 0 ? opt->batch : 0;
+  if (b <= 0) {
+    const size_t per = (size_t)p.grid.nx * p.nc * sizeof(float2);
+    const size_t budget = size_t(4) << 30;
+    b = (int)std::max<size_t>(1, std::min<size_t>(1024, budget / per));
+  }
+  return std::max(1, std::min(b, count));
+}
+
+RowArgs row_args(const sk_potential& p, float2* G) {
+  RowArgs r{};
+  r.nx = p.grid.nx;
+  r.ny = p.grid.ny;
+  r.nc = p.nc;
+  r.fy = line_plan(p.grid.ny, ctx().stream).dev;
+  r.keep = p.keep_d.p;
+  r.keepinv = p.keepinv_d.p;
+  r.G = G;
+  r.out_scale = 1.0f;
+  r.in_full_stride = r.out_full_stride = r.plane_stride = (long long)p.grid.nx * p.grid.ny;
+  return r;
+}
+
+ColArgs col_args(const sk_potential& p, float2* G) {
+  ColArgs c{};
+  c.nx = p.grid.nx;
+  c.ny = p.grid.ny;
+  c.nc = p.nc;
+  c.fx = line_plan(p.grid.nx, ctx().stream).dev;
+  c.keep = p.keep_d.p;
+  c.G = G;
+  return c;
+}
+
+// Runs `batch` waves whose first row pass input is ready (full real-space waves in `full`, or compact G after a
+// generated INIT); writes requested planes. planes_mask[j] = output index for plane j (or -1).
+struct Timer {
+  bool on = false;
+  std::vector<hipEvent_t> ev;
+  std::vector<int> kind;  // 0 row, 1 col
+  size_t used = 0;
+  void mark_start(hipStream_t s, int k) {
+    if (!on) return;
+    if (used + 2 > ev.size()) {
+      for (int i = 0; i < 64; ++i) {
+        hipEvent_t e;
+        SK_HIP(hipEventCreate(&e));
+        ev.push_back(e);
+      }
+      kind.resize(ev.size() / 2 + 1);
+    }
+    kind[used / 2] = k;
+    SK_HIP(hipEventRecord(ev[used], s));
+  }
+  void mark_stop(hipStream_t s) {
+    if (!on) return;
+    SK_HIP(hipEventRecord(ev[used + 1], s));
+    used += 2;
+  }
+  void harvest(sk_timing& t) {
+    if (!on || !used) return;
+    SK_HIP(hipEventSynchronize(ev[used - 1]));
+    for (size_t i = 0; i < used; i += 2) {
+      float ms = 0;
+      SK_HIP(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+      if (kind[i / 2] == 0) {
+        t.row_ms += ms;
+        t.row_launches++;
+      } else {
+        t.col_ms += ms;
+        t.col_launches++;
+      }
+    }
+    used = 0;
+  }
+  ~Timer() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+  }
+};
+
+void slice_loop(const sk_potential& p, int batch, float2* G, bool entry_full, float2* full, int first_slice_from_full,
+                const std::vector<int>& plane_out, float2* planes, const ColArgs* final_col, bool fast, Timer& tm) {
+  hipStream_t s = ctx().stream;
+  const size_t N = (size_t)p.grid.nx * p.grid.ny;
+  RowArgs r = row_args(p, G);
+  ColArgs c = col_args(p, G);
+  for (int j = 0; j < p.nslices; ++j) {
+    // row pass of slice j: (inverse of the previous state) * t_j * forward
+    r.t = p.t.p + (size_t)j * N;
+    if (j == 0 && entry_full) {
+      r.flags = ROW_IN_FULL | ROW_MUL_T | ROW_FWD | ROW_OUT_COMPACT;
+      r.in_full = full;
+    } else if (j == 0 && first_slice_from_full) {
+      r.flags = ROW_IN_FULL | ROW_INV | ROW_MUL_T | ROW_FWD | ROW_OUT_COMPACT;
+      r.in_full = full;
+    } else {
+      r.flags = ROW_IN_COMPACT | ROW_INV | ROW_MUL_T | ROW_FWD | ROW_OUT_COMPACT;
+      if (j > 0 && plane_out[j - 1] >= 0) {
+        r.flags |= ROW_WRITE_PLANE;
+        r.plane = planes + (size_t)plane_out[j - 1] * batch * N;
+      }
+    }
+    tm.mark_start(s, 0);
+    row_pass(r, batch, fast, s);
+    tm.mark_stop(s);
+    // column pass of slice j
+    c.P = p.props.p + (size_t)p.prop_of_slice[j] * p.grid.nx * p.nc;
+    if (j == p.nslices - 1 && final_col) {
+      ColArgs f = *final_col;
+      f.P = c.P;
+      col_pass(f, batch, fast, s);
+    } else {
+      c.flags = COL_IN_G | COL_FWD | COL_MUL_P | COL_INV | COL_OUT;
+      tm.mark_start(s, 1);
+      col_pass(c, batch, fast, s);
+      tm.mark_stop(s);
+    }
+    r.flags = 0;
+    r.plane = nullptr;
+  }
+  if (!final_col) {
+    // exit: the real-space wave after the last slice (and any plane still pending)
+    const int q = plane_out[p.nslices - 1];
+    if (q >= 0) {
+      r.flags = ROW_IN_COMPACT | ROW_INV | ROW_WRITE_PLANE;
+      r.plane = planes + (size_t)q * batch * N;
+      row_pass(r, batch, fast, s);
+    }
+  }
+}
+
+}  // namespace
+}  // namespace sk
+
+using namespace sk;
+
+// =================================================================================================================
+// C ABI
+// =================================================================================================================
+extern "C" {
+
+int sk_hip_available(void) {
+  std::lock_guard<std::recursive_mutex> g(lock());
+  try {
+    require();
+    return 1;
+  } catch (const Error& e) {
+    g_error = e.what();
+    return 0;
+  }
+}
+
+const char* sk_hip_last_error(void) { return g_error.c_str(); }
+
+void sk_hip_set_timing(int enabled) {
+  std::lock_guard<std::recursive_mutex> g(lock());
+  ctx().timing = enabled != 0;
+}
+
+int sk_hip_last_timing(sk_timing* t) {
+  if (!t) return SK_ERR_INVALID;
+  std::lock_guard<std::recursive_mutex> g(lock());
+  *t = ctx().last;
+  return SK_OK;
+}
+
+int sk_hip_potential_from_values(const sk_grid* grid, int nslices, const double* V, const double* dz, double energy,
+                                 const double tilt[2], sk_potential** out) {
+  return guarded([&] {
+    Context& c = require();
+    if (!out || !V || !dz) fail(SK_ERR_INVALID, "null argument");
+    auto p = new_potential(grid, nslices, dz, energy, tilt);
+    const size_t N = (size_t)grid->nx * grid->ny;
+    DevBuf<double> Vd;
+    Vd.upload(V, N * nslices, c.stream);
+    finish_potential(*p, Vd.p);
+    *out = p.release();
+  });
+}
+
+int sk_hip_potential_from_atoms(const sk_grid* grid, int nslices, const double* dz, int natoms, const int* numbers,
+                                const double* xy, const int* slice_of_atom, int pi_mode, double energy,
+                                const double tilt[2], sk_potential** out) {
+  return guarded([&] {
+    Context& c = require();
+    hipStream_t s = c.stream;
+    if (!out || !dz || (natoms > 0 && (!numbers || !xy || !slice_of_atom))) fail(SK_ERR_INVALID, "null argument");
+    auto p = new_potential(grid, nslices, dz, energy, tilt);
+    const int nx = grid->nx, ny = grid->ny;
+    const size_t N = (size_t)nx * ny;
+    const double sx = grid->extent_x / nx, sy = grid->extent_y / ny;
+    // elements present, ascending (ProjectedPotential.swift:158)
+    std::vector<int> zs;
+    for (int a = 0; a < natoms; ++a) {
+      if (numbers[a] < 1 || numbers[a] > 103 || !kLobato[numbers[a] - 1].present)
+        fail(SK_ERR_INVALID, "element Z = " + std::to_string(numbers[a]) + " has no Lobato parameters");
+      if (slice_of_atom[a] < 0 || slice_of_atom[a] >= nslices) fail(SK_ERR_INVALID, "slice of atom out of range");
+      if (!(std::isfinite(xy[2 * a]) && std::isfinite(xy[2 * a + 1]))) fail(SK_ERR_INVALID, "non-finite position");
+      zs.push_back(numbers[a]);
+    }
+    std::sort(zs.begin(), zs.end());
+    zs.erase(std::unique(zs.begin(), zs.end()), zs.end());
+    const int nz = std::max<int>(1, (int)zs.size());
+    std::vector<int> zindex(104, -1);
+    for (size_t e = 0; e < zs.size(); ++e) zindex[zs[e]] = (int)e;
+    DevBuf<double> Vd;
+    Vd.alloc(N * nslices);
+    if (zs.empty()) {
+      SK_HIP(hipMemsetAsync(Vd.p, 0, sizeof(double) * N * nslices, s));
+    } else {
+      DevBuf<int> zs_d;
+      zs_d.upload(zs, s);
+      DevBuf<float> K;
+      K.alloc(N * zs.size());
+      launch_lobato_kernels(nx, ny, grid->extent_x, grid->extent_y, zs_d.p, (int)zs.size(), pi_mode, c.lobato.p, K.p,
+                            s);
+      // bilinear deltas per (slice, element), merged per pixel in float64 in NumPy add.at order
+      // (ProjectedPotential.swift:178-200): weight-major, then atom order
+      std::vector<std::vector<int>> atoms_of(nslices * (size_t)nz);
+      for (int a = 0; a < natoms; ++a) atoms_of[(size_t)slice_of_atom[a] * nz + zindex[numbers[a]]].push_back(a);
+      const int chunk = std::max<int>(1, (int)std::min<size_t>(nslices, (size_t(1) << 30) / (N * nz * sizeof(float2))));
+      std::vector<double> dense(N, 0.0);
+      std::vector<char> touched(N, 0);
+      DevBuf<float2> D, Sp;
+      D.alloc((size_t)chunk * nz * N);
+      Sp.alloc((size_t)chunk * N);
+      for (int j0 = 0; j0 < nslices; j0 += chunk) {
+        const int cnt = std::min(chunk, nslices - j0);
+        std::vector<long long> idx;
+        std::vector<int> idx32;
+        std::vector<double> w;
+        for (int j = j0; j < j0 + cnt; ++j)
+          for (int e = 0; e < nz; ++e) {
+            const auto& list = atoms_of[(size_t)j * nz + e];
+            if (list.empty()) continue;
+            std::vector<int> order;
+            const size_t na = list.size();
+            std::vector<int> rows(na), cols(na);
+            std::vector<double> wt[4];
+            for (auto& v : wt) v.resize(na);
+            for (size_t q = 0; q < na; ++q) {
+              const int a = list[q];
+              const double px = xy[2 * a] / sx, py = xy[2 * a + 1] / sy;
+              const int r = (int)std::floor(px), cc = (int)std::floor(py);
+              const double x = px - r, y = py - cc, xyv = x * y;
+              rows[q] = r;
+              cols[q] = cc;
+              wt[0][q] = 1 + xyv - y - x;
+              wt[1][q] = x - xyv;
+              wt[2][q] = y - xyv;
+              wt[3][q] = xyv;
+            }
+            auto mod = [](int a, int m) {
+              int r = a % m;
+              return r < 0 ? r + m : r;
+            };
+            for (int k = 0; k < 4; ++k)
+              for (size_t q = 0; q < na; ++q) {
+                const int i = mod(rows[q] + ((k == 1 || k == 3) ? 1 : 0), nx);
+                const int jj = mod(cols[q] + (k >= 2 ? 1 : 0), ny);
+                const size_t pix = (size_t)i * ny + jj;
+                if (!touched[pix]) {
+                  touched[pix] = 1;
+                  order.push_back((int)pix);
+                }
+                dense[pix] += wt[k][q];
+              }
+            const long long base = ((long long)(j - j0) * nz + e) * (long long)N;
+            for (int pix : order) {
+              idx.push_back(base + pix);
+              w.push_back(dense[pix]);
+              dense[pix] = 0.0;
+              touched[pix] = 0;
+            }
+          }
+        SK_HIP(hipMemsetAsync(D.p, 0, sizeof(float2) * (size_t)cnt * nz * N, s));
+        if (!idx.empty()) {
+          if ((size_t)cnt * nz * N > (size_t)INT32_MAX) fail(SK_ERR_INVALID, "potential chunk too large");
+          idx32.assign(idx.begin(), idx.end());
+          DevBuf<int> idx_d;
+          DevBuf<double> w_d;
+          idx_d.upload(idx32, s);
+          w_d.upload(w, s);
+          launch_scatter_deltas(idx_d.p, w_d.p, idx32.size(), D.p, s);
+          SK_HIP(hipStreamSynchronize(s));
+        }
+        fft2_batch(D.p, nx, ny, cnt * nz, -1, nullptr, 1.0f, s);
+        launch_combine_elements(D.p, K.p, cnt, nz, N, Sp.p, s);
+        fft2_batch(Sp.p, nx, ny, cnt, +1, nullptr, float(1.0 / double(N)), s);
+        launch_real_part(Sp.p, Vd.p + (size_t)j0 * N, (size_t)cnt * N, s);
+        SK_HIP(hipStreamSynchronize(s));
+      }
+    }
+    finish_potential(*p, Vd.p);
+    p->V = std::move(Vd);
+    p->has_values = true;
+    *out = p.release();
+  });
+}
+
+void sk_hip_potential_release(sk_potential* p) {
+  std::lock_guard<std::recursive_mutex> g(lock());
+  delete p;
+}
+
+int sk_hip_potential_slice_count(const sk_potential* p) { return p ? p->nslices : SK_ERR_INVALID; }
+
+int sk_hip_potential_values(const sk_potential* p, double* V) {
+  return guarded([&] {
+    Context& c = require();
+    if (!p || !V) fail(SK_ERR_INVALID, "null argument");
+    if (!p->has_values) fail(SK_ERR_INVALID, "this potential was made from values, not built on the device");
+    const size_t n = (size_t)p->grid.nx * p->grid.ny * p->nslices;
+    SK_HIP(hipMemcpyAsync(V, p->V.p, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    SK_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int sk_hip_potential_transmissions(const sk_potential* p, double* t_re, double* t_im) {
+  return guarded([&] {
+    Context& c = require();
+    if (!p || !t_re || !t_im) fail(SK_ERR_INVALID, "null argument");
+    const size_t n = (size_t)p->grid.nx * p->grid.ny * p->nslices;
+    DevBuf<double> re(n), im(n);
+    launch_complex_to_split(p->t.p, re.p, im.p, n, 1.0, c.stream);
+    SK_HIP(hipMemcpyAsync(t_re, re.p, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    SK_HIP(hipMemcpyAsync(t_im, im.p, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    SK_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+static void run_waves(const sk_potential* p, int count, const double* in_re, const double* in_im,
+                      const std::vector<int>& planes, double* out_re, double* out_im, const sk_options* opt) {
+  Context& c = require();
+  hipStream_t s = c.stream;
+  const size_t N = (size_t)p->grid.nx * p->grid.ny;
+  const int nplanes = (int)planes.size();
+  // plane j -> output slot (the incident plane -1 is copied on the host side)
+  std::vector<int> plane_out(p->nslices, -1);
+  int slot = 0;
+  for (int q = 0; q < nplanes; ++q)
+    if (planes[q] >= 0) plane_out[planes[q]] = slot++;
+  const int nslot = slot;
+  const int B = default_batch(*p, count, opt);
+  const bool fast = !opt || opt->fast_path != 0;
+  Workspace& w = ws();
+  w.G.ensure((size_t)B * p->grid.nx * p->nc);
+  w.full.ensure((size_t)B * N);
+  w.planes.ensure((size_t)std::max(1, nslot) * B * N);
+  w.tmp_re.ensure((size_t)B * N);
+  w.tmp_im.ensure((size_t)B * N);
+  Timer tm;
+  for (int b0 = 0; b0 < count; b0 += B) {
+    const int nb = std::min(B, count - b0);
+    SK_HIP(hipMemcpyAsync(w.tmp_re.p, in_re + (size_t)b0 * N, nb * N * sizeof(double), hipMemcpyHostToDevice, s));
+    SK_HIP(hipMemcpyAsync(w.tmp_im.p, in_im + (size_t)b0 * N, nb * N * sizeof(double), hipMemcpyHostToDevice, s));
+    launch_real_to_complex(w.tmp_re.p, w.tmp_im.p, w.full.p, nb * N, s);
+    slice_loop(*p, nb, w.G.p, true, w.full.p, 0, plane_out, w.planes.p, nullptr, fast, tm);
+    for (int q = 0, k = 0; q < nplanes; ++q) {
+      double* ore = out_re + ((size_t)q * count + b0) * N;
+      double* oim = out_im + ((size_t)q * count + b0) * N;
+      if (planes[q] < 0) {
+        std::memcpy(ore, in_re + (size_t)b0 * N, nb * N * sizeof(double));
+        std::memcpy(oim, in_im + (size_t)b0 * N, nb * N * sizeof(double));
+        continue;
+      }
+      launch_complex_to_split(w.planes.p + (size_t)k * nb * N, w.tmp_re.p, w.tmp_im.p, nb * N, 1.0, s);
+      SK_HIP(hipMemcpyAsync(ore, w.tmp_re.p, nb * N * sizeof(double), hipMemcpyDeviceToHost, s));
+      SK_HIP(hipMemcpyAsync(oim, w.tmp_im.p, nb * N * sizeof(double), hipMemcpyDeviceToHost, s));
+      SK_HIP(hipStreamSynchronize(s));
+      ++k;
+    }
+  }
+  SK_HIP(hipStreamSynchronize(s));
+}
+
+int sk_hip_run(const sk_potential* p, const double* in_re, const double* in_im, const int* planes, int nplanes,
+               double* out_re, double* out_im, const sk_options* opt) {
+  return guarded([&] {
+    if (!p || !in_re || !in_im || !planes || !out_re || !out_im || nplanes < 1) fail(SK_ERR_INVALID, "null argument");
+    std::vector<int> pl(planes, planes + nplanes);
+    for (int q = 0; q < nplanes; ++q) {
+      if (pl[q] < -1 || pl[q] >= p->nslices || (q > 0 && pl[q] <= pl[q - 1]))
+        fail(SK_ERR_INVALID, "invalid exit planes: afterSlices must be ascending, distinct, within -1 ..< " +
+                                 std::to_string(p->nslices));
+    }
+    // out layout: nplanes x N for one wave
+    run_waves(p, 1, in_re, in_im, pl, out_re, out_im, opt);
+  });
+}
+
+int sk_hip_run_batch(const sk_potential* p, int count, const double* in_re, const double* in_im, double* out_re,
+                     double* out_im, const sk_options* opt) {
+  return guarded([&] {
+    if (!p || count < 0) fail(SK_ERR_INVALID, "invalid argument");
+    if (count == 0) return;
+    if (!in_re || !in_im || !out_re || !out_im) fail(SK_ERR_INVALID, "null argument");
+    std::vector<int> pl{p->nslices - 1};
+    run_waves(p, count, in_re, in_im, pl, out_re, out_im, opt);
+  });
+}
+
+// probe spectrum Psi0 / ||Psi0|| * extra on the device; returns whether its support fits the kept columns
+static bool build_spectrum(const sk_grid& g, const sk_probe& pr, double extra, const std::vector<int>* keepinv,
+                           float2* out) {
+  Context& c = ctx();
+  hipStream_t s = c.stream;
+  Workspace& w = ws();
+  const size_t N = (size_t)g.nx * g.ny;
+  w.tmp_re.ensure(N);
+  w.tmp_im.ensure(N);
+  w.norm.ensure(1);
+  const int blocks = 256;
+  w.norm_partial.ensure(blocks);
+  launch_probe_spectrum(g.nx, g.ny, g.extent_x, g.extent_y, wavelength(pr.energy), pr.semiangle_mrad, pr.soft,
+                        pr.aberrations, w.tmp_re.p, w.tmp_im.p, s);
+  launch_sum_squares(w.tmp_re.p, w.tmp_im.p, N, w.norm_partial.p, blocks, w.norm.p, s);
+  launch_normalise_spectrum(w.tmp_re.p, w.tmp_im.p, w.norm.p, extra, N, out, s);
+  if (!keepinv) return true;
+  // support check: any non-zero amplitude on a pruned column?
+  std::vector<double> re(N), im(N);
+  SK_HIP(hipMemcpyAsync(re.data(), w.tmp_re.p, N * sizeof(double), hipMemcpyDeviceToHost, s));
+  SK_HIP(hipMemcpyAsync(im.data(), w.tmp_im.p, N * sizeof(double), hipMemcpyDeviceToHost, s));
+  SK_HIP(hipStreamSynchronize(s));
+  for (int ix = 0; ix < g.nx; ++ix)
+    for (int iy = 0; iy < g.ny; ++iy)
+      if ((*keepinv)[iy] < 0 && (re[(size_t)ix * g.ny + iy] != 0.0 || im[(size_t)ix * g.ny + iy] != 0.0)) return false;
+  return true;
+}
+
+static void check_probe(const sk_probe* pr) {
+  if (!pr) fail(SK_ERR_INVALID, "no probe");
+  if (!(std::isfinite(pr->energy) && pr->energy > 0)) fail(SK_ERR_INVALID, "electron energy must be finite and positive");
+  if (!(pr->semiangle_mrad > 0) || std::isnan(pr->semiangle_mrad))
+    fail(SK_ERR_INVALID, "semiangle cutoff must be finite and positive (or infinite)");
+  for (double a : pr->aberrations)
+    if (!std::isfinite(a)) fail(SK_ERR_INVALID, "invalid aberration: every coefficient must be finite");
+  if (!(std::isfinite(pr->tilt[0]) && std::isfinite(pr->tilt[1]))) fail(SK_ERR_INVALID, "invalid tilt: the tilt must be finite");
+}
+
+int sk_hip_probes(const sk_grid* grid, const sk_probe* probe, int npos, const double* positions, double* out_re,
+                  double* out_im) {
+  return guarded([&] {
+    Context& c = require();
+    hipStream_t s = c.stream;
+    check_grid(grid);
+    check_probe(probe);
+    if (npos < 0 || (npos > 0 && (!positions || !out_re || !out_im))) fail(SK_ERR_INVALID, "invalid argument");
+    for (int q = 0; q < npos; ++q)
+      if (!(std::isfinite(positions[2 * q]) && std::isfinite(positions[2 * q + 1])))
+        fail(SK_ERR_INVALID, "invalid position");
+    const int nx = grid->nx, ny = grid->ny;
+    const size_t N = (size_t)nx * ny;
+    Workspace& w = ws();
+    w.spec.ensure(N);
+    build_spectrum(*grid, *probe, 1.0 / double(N), nullptr, w.spec.p);
+    w.positions.upload(positions, 2 * (size_t)npos, s);
+    const int B = std::max(1, std::min(npos, 64));
+    w.rx.ensure((size_t)B * nx);
+    w.ry.ensure((size_t)B * ny);
+    w.full.ensure((size_t)B * N);
+    DevBuf<double> re((size_t)B * N), im((size_t)B * N);
+    for (int b0 = 0; b0 < npos; b0 += B) {
+      const int nb = std::min(B, npos - b0);
+      launch_ramps(nx, ny, grid->extent_x / nx, grid->extent_y / ny, w.positions.p, b0, nb, w.rx.p, w.ry.p, s);
+      ColArgs cg{};
+      cg.nx = nx;
+      cg.ny = ny;
+      cg.nc = ny;
+      cg.fx = line_plan(nx, s).dev;
+      cg.keep = identity_map(ny);
+      cg.G = w.full.p;
+      cg.spec = w.spec.p;
+      cg.rx = w.rx.p;
+      cg.ry = w.ry.p;
+      cg.flags = COL_GENERATE | COL_INV | COL_OUT;
+      col_pass(cg, nb, false, s);
+      RowArgs r{};
+      r.nx = nx;
+      r.ny = ny;
+      r.nc = ny;
+      r.fy = line_plan(ny, s).dev;
+      r.keep = cg.keep;
+      r.keepinv = cg.keep;
+      r.in_full = w.full.p;
+      r.out_full = w.full.p;
+      r.in_full_stride = r.out_full_stride = (long long)N;
+      r.out_scale = 1.0f;
+      r.flags = ROW_IN_FULL | ROW_INV | ROW_OUT_FULL;
+      row_pass(r, nb, false, s);
+      launch_complex_to_split(w.full.p, re.p, im.p, nb * N, 1.0, s);
+      SK_HIP(hipMemcpyAsync(out_re + (size_t)b0 * N, re.p, nb * N * sizeof(double), hipMemcpyDeviceToHost, s));
+      SK_HIP(hipMemcpyAsync(out_im + (size_t)b0 * N, im.p, nb * N * sizeof(double), hipMemcpyDeviceToHost, s));
+      SK_HIP(hipStreamSynchronize(s));
+    }
+  });
+}
+
+int sk_hip_detector_shape(const sk_grid* grid, double energy, const sk_detector* det, int* shape) {
+  int count = 0;
+  int rc = guarded([&] {
+    check_grid(grid);
+    if (!det) fail(SK_ERR_INVALID, "no detector");
+    int sh[2] = {0, 0};
+    count = detector_count(*grid, wavelength(energy), *det, sh);
+    if (shape) {
+      shape[0] = sh[0];
+      shape[1] = sh[1];
+    }
+  });
+  return rc == SK_OK ? count : rc;
+}
+
+int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* probe, int npos, const double* positions,
+                const sk_detector* dets, int ndets, void* const* outputs, int outputs_on_device,
+                const sk_options* opt) {
+  return guarded([&] {
+    Context& c = require();
+    hipStream_t s = c.stream;
+    if (!pots || npots < 1) fail(SK_ERR_INVALID, "invalid scan: no potential");
+    check_probe(probe);
+    if (npos < 1 || !positions) fail(SK_ERR_INVALID, "invalid scan: no positions");
+    if (ndets < 0 || (ndets > 0 && (!dets || !outputs))) fail(SK_ERR_INVALID, "invalid detectors");
+    const sk_potential& p0 = *pots[0];
+    const sk_grid g = p0.grid;
+    for (int k = 0; k < npots; ++k) {
+      if (!pots[k]) fail(SK_ERR_INVALID, "null potential");
+      if (!same_grid(pots[k]->grid, g)) fail(SK_ERR_INVALID, "the waves and the potential are on different grids");
+      if (pots[k]->energy != probe->energy)
+        fail(SK_ERR_INVALID, "the probe's energy differs from the potential's");
+      if (pots[k]->tilt[0] != probe->tilt[0] || pots[k]->tilt[1] != probe->tilt[1])
+        fail(SK_ERR_INVALID, "invalid tilt: the probe's tilt differs from the potential's");
+    }
+    for (int q = 0; q < npos; ++q)
+      if (!(std::isfinite(positions[2 * q]) && std::isfinite(positions[2 * q + 1])))
+        fail(SK_ERR_INVALID, "invalid position");
+    const int nx = g.nx, ny = g.ny;
+    const size_t N = (size_t)nx * ny;
+    const double lambda = wavelength(probe->energy);
+    // ---- detector plan ----
+    DetPlan dp{};
+    Workspace& w = ws();
+    std::vector<int> det_slot(ndets, -1), det_pix(ndets, -1), det_count(ndets, 0);
+    std::vector<unsigned> mask;
+    int nmask = 0, ncom = 0, npix = 0;
+    w.maps.clear();
+    w.weights.clear();
+    w.pix.clear();
+    w.maps.reserve(4 * kMaxCom + 2 * kMaxPix);
+    w.weights.reserve(2 * kMaxCom);
+    w.pix.reserve(kMaxPix);
+    const double rxs = 1 / (double(nx) * (g.extent_x / nx)), rys = 1 / (double(ny) * (g.extent_y / ny));
+    const double ax = rxs * lambda * 1e3, ay = rys * lambda * 1e3;
+    for (int d = 0; d < ndets; ++d) {
+      const sk_detector& D = dets[d];
+      det_count[d] = detector_count(g, lambda, D, nullptr);
+      if (!outputs[d]) fail(SK_ERR_INVALID, "null detector output");
+      if (D.kind == SK_DET_ANNULAR) {
+        if (dp.nslots >= kMaxSlots || nmask >= 32) fail(SK_ERR_INVALID, "too many annular detectors");
+        const double o = annular_outer(g, lambda, D);
+        if (mask.empty()) mask.assign(N, 0u);
+        auto angx = fftfreq(nx, 1 / ax / double(nx)), angy = fftfreq(ny, 1 / ay / double(ny));
+        const double i2 = D.inner * D.inner, o2 = o * o;
+        for (int ix = 0; ix < nx; ++ix)
+          for (int iy = 0; iy < ny; ++iy) {
+            const double a2x = angx[ix] * angx[ix];
+            const double a2y = angy[iy] * angy[iy];
+            const double a2 = a2x + a2y;
+            if (a2 >= i2 && a2 < o2) mask[(size_t)ix * ny + iy] |= (1u << nmask);
+          }
+        det_slot[d] = dp.nslots;
+        dp.slot_kind[dp.nslots] = 0;
+        dp.slot_arg[dp.nslots] = nmask;
+        dp.nslots++;
+        nmask++;
+      } else if (D.kind == SK_DET_COM) {
+        if (dp.nslots + 2 > kMaxSlots || ncom >= kMaxCom) fail(SK_ERR_INVALID, "too many centre-of-mass detectors");
+        int mx, my;
+        sk_detector region = D;
+        region.crop_nx = region.crop_ny = 0;
+        pixelated_size(g, region, mx, my);
+        auto mapx = crop_map(nx, mx), mapy = crop_map(ny, my);
+        const double per = D.units == 0 ? 1.0 : 1.0 / (lambda * 1e3);
+        std::vector<double> wx(nx, 0.0), wy(ny, 0.0);
+        for (int ix = 0; ix < nx; ++ix)
+          if (mapx[ix] >= 0) wx[ix] = double(mapx[ix] - mx / 2) * ax * per;
+        for (int iy = 0; iy < ny; ++iy)
+          if (mapy[iy] >= 0) wy[iy] = double(mapy[iy] - my / 2) * ay * per;
+        w.maps.emplace_back();
+        w.maps.back().upload(mapx, s);
+        dp.com_mapx[ncom] = w.maps.back().p;
+        w.maps.emplace_back();
+        w.maps.back().upload(mapy, s);
+        dp.com_mapy[ncom] = w.maps.back().p;
+        w.weights.emplace_back();
+        w.weights.back().upload(wx, s);
+        dp.com_wx[ncom] = w.weights.back().p;
+        w.weights.emplace_back();
+        w.weights.back().upload(wy, s);
+        dp.com_wy[ncom] = w.weights.back().p;
+        det_slot[d] = dp.nslots;
+        dp.slot_kind[dp.nslots] = 1;
+        dp.slot_arg[dp.nslots] = ncom;
+        dp.slot_kind[dp.nslots + 1] = 2;
+        dp.slot_arg[dp.nslots + 1] = ncom;
+        dp.nslots += 2;
+        ncom++;
+      } else {
+        if (npix >= kMaxPix) fail(SK_ERR_INVALID, "too many pixelated detectors");
+        int mx, my;
+        pixelated_size(g, D, mx, my);
+        auto mapx = crop_map(nx, mx), mapy = crop_map(ny, my);
+        w.maps.emplace_back();
+        w.maps.back().upload(mapx, s);
+        dp.pix_mapx[npix] = w.maps.back().p;
+        w.maps.emplace_back();
+        w.maps.back().upload(mapy, s);
+        dp.pix_mapy[npix] = w.maps.back().p;
+        const int cx = (int)crop_indices(nx, mx).size(), cy = (int)crop_indices(ny, my).size();
+        dp.pix_my[npix] = cy;
+        dp.pix_stride[npix] = (long long)cx * cy;
+        w.pix.emplace_back();
+        w.pix.back().alloc((size_t)npos * cx * cy);
+        SK_HIP(hipMemsetAsync(w.pix.back().p, 0, sizeof(double) * (size_t)npos * cx * cy, s));
+        dp.pix_acc[npix] = w.pix.back().p;
+        det_pix[d] = npix;
+        npix++;
+      }
+    }
+    dp.npix = npix;
+    if (!mask.empty()) {
+      w.mask.upload(mask, s);
+      dp.mask = w.mask.p;
+    } else {
+      dp.mask = nullptr;
+    }
+    dp.scale = double(N) * double(N);  // |FFT2(psi_exit)|^2 of a probe (no 1/N): P' carries 1/N
+    // ---- spectrum and positions ----
+    w.spec.ensure(N);
+    const bool fits = build_spectrum(g, *probe, 1.0 / double(N), &p0.keepinv, w.spec.p);
+    w.positions.upload(positions, 2 * (size_t)npos, s);
+    const int nslots = std::max(1, dp.nslots);
+    w.acc.ensure((size_t)npos * nslots);
+    SK_HIP(hipMemsetAsync(w.acc.p, 0, sizeof(double) * (size_t)npos * nslots, s));
+    // ---- batches ----
+    const bool fast = !opt || opt->fast_path != 0;
+    int B = default_batch(p0, npos, opt);
+    w.G.ensure((size_t)B * nx * p0.nc);
+    w.rx.ensure((size_t)B * nx);
+    w.ry.ensure((size_t)B * ny);
+    if (!fits) w.full.ensure((size_t)B * N);
+    int bpw = (p0.nc + col_generic_lines(line_plan(nx, s).dev.m) - 1) / col_generic_lines(line_plan(nx, s).dev.m);
+    if (fast && fast_col_supported(nx, p0.nc, COL_IN_G | COL_FWD | COL_MUL_P | COL_DETECT))
+      bpw = fast_col_blocks_per_wave(nx, p0.nc);
+    dp.blocks_per_wave = bpw;
+    w.partial.ensure((size_t)B * nslots * bpw);
+    dp.partial = w.partial.p;
+    Timer tm;
+    tm.on = c.timing;
+    sk_timing timing{};
+    hipEvent_t e0, e1;
+    SK_HIP(hipEventCreate(&e0));
+    SK_HIP(hipEventCreate(&e1));
+    SK_HIP(hipEventRecord(e0, s));
+    std::vector<int> no_planes;
+    for (int k = 0; k < npots; ++k) {
+      const sk_potential& P = *pots[k];
+      std::vector<int> plane_out(P.nslices, -1);
+      for (int b0 = 0; b0 < npos; b0 += B) {
+        const int nb = std::min(B, npos - b0);
+        launch_ramps(nx, ny, g.extent_x / nx, g.extent_y / ny, w.positions.p, b0, nb, w.rx.p, w.ry.p, s);
+        ColArgs init = col_args(P, fits ? w.G.p : w.full.p);
+        init.spec = w.spec.p;
+        init.rx = w.rx.p;
+        init.ry = w.ry.p;
+        init.flags = COL_GENERATE | COL_INV | COL_OUT;
+        if (!fits) {
+          init.nc = ny;
+          init.keep = identity_map(ny);
+        }
+        col_pass(init, nb, fast && fits, s);
+        ColArgs fin = col_args(P, w.G.p);
+        fin.flags = COL_IN_G | COL_FWD | COL_MUL_P | COL_DETECT;
+        fin.pos0 = b0;
+        fin.det = dp;
+        slice_loop(P, nb, w.G.p, false, w.full.p, fits ? 0 : 1, plane_out, nullptr, &fin, fast, tm);
+        launch_reduce_partials(dp, nb, b0, w.acc.p, s);
+        tm.harvest(timing);
+      }
+    }
+    SK_HIP(hipEventRecord(e1, s));
+    // ---- outputs: averaged over the potentials (STEM.swift:146-150) ----
+    const double inv = 1.0 / double(npots);
+    std::vector<DevBuf<char>> staging;
+    for (int d = 0; d < ndets; ++d) {
+      const sk_detector& D = dets[d];
+      const size_t cnt = (size_t)det_count[d] * npos;
+      const size_t esz = D.out_f32 ? sizeof(float) : sizeof(double);
+      void* dst = outputs[d];
+      DevBuf<char> tmp;
+      if (!outputs_on_device) {
+        tmp.alloc(cnt * esz);
+        dst = tmp.p;
+      }
+      if (D.kind == SK_DET_ANNULAR) {
+        launch_gather_scalar(w.acc.p, nslots, det_slot[d], npos, inv, dst, 1, D.out_f32, s);
+      } else if (D.kind == SK_DET_COM) {
+        launch_gather_scalar(w.acc.p, nslots, det_slot[d], npos, inv, dst, 2, D.out_f32, s);
+        char* dst1 = static_cast<char*>(dst) + esz;
+        launch_gather_scalar(w.acc.p, nslots, det_slot[d] + 1, npos, inv, dst1, 2, D.out_f32, s);
+      } else {
+        launch_finalise_outputs(w.pix[det_pix[d]].p, cnt, inv, dst, D.out_f32, s);
+      }
+      if (!outputs_on_device) {
+        SK_HIP(hipMemcpyAsync(outputs[d], tmp.p, cnt * esz, hipMemcpyDeviceToHost, s));
+        staging.push_back(std::move(tmp));
+      }
+    }
+    SK_HIP(hipStreamSynchronize(s));
+    float total = 0;
+    SK_HIP(hipEventElapsedTime(&total, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    // algorithmic bytes per launch: one complex64 read + write of every stored value of the batch
+    timing.total_ms = total;
+    if (timing.row_launches) timing.row_ms /= double(timing.row_launches);
+    if (timing.col_launches) timing.col_ms /= double(timing.col_launches);
+    timing.row_bytes = 2.0 * sizeof(float2) * double(B) * nx * p0.nc;
+    timing.col_bytes = timing.row_bytes;
+    c.last = timing;
+  });
+}
+
+}  // extern "C"

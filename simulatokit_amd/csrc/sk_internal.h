@@ -1,0 +1,221 @@
+// sk_internal.h — shared definitions of the MI355X multislice library (not part of the public C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/simulato_hip.h"
+
+namespace sk {
+
+// ---- errors ----------------------------------------------------------------------------------------------------
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] inline void fail(int code, const std::string& m) { throw Error(code, m); }
+
+#define SK_HIP(call)                                                                                         \
+  do {                                                                                                       \
+    hipError_t e_ = (call);                                                                                  \
+    if (e_ != hipSuccess)                                                                                    \
+      ::sk::fail(e_ == hipErrorOutOfMemory ? SK_ERR_MEMORY : SK_ERR_DEVICE,                                  \
+                 std::string(#call) + ": " + hipGetErrorString(e_));                                         \
+  } while (0)
+
+// ---- device buffers ----------------------------------------------------------------------------------------------
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { alloc(count); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    if (count == 0) return;
+    SK_HIP(hipMalloc(&p, count * sizeof(T)));
+    n = count;
+  }
+  void ensure(size_t count) {
+    if (count > n) alloc(count);
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  // copies from (pageable) host memory and waits, so the host array may be released on return
+  void upload(const T* h, size_t count, hipStream_t s) {
+    ensure(count);
+    if (count) {
+      SK_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+      SK_HIP(hipStreamSynchronize(s));
+    }
+  }
+  void upload(const std::vector<T>& v, hipStream_t s) { upload(v.data(), v.size(), s); }
+};
+
+// ---- 1-D transform plan (device view) -------------------------------------------------------------------------------
+constexpr int kMaxStages = 20;
+
+// A 1-D complex transform of logical length n, done in LDS as a Stockham transform of length m (= n, or the
+// power-of-two Bluestein length).
+struct LineFFT {
+  int n, m, nst, blue;
+  int radix[kMaxStages];
+  const float2* roots;  // m entries, e^{-2 pi i k / m}
+  const float2* chirp;  // n entries, e^{-pi i k^2 / n} (Bluestein)
+  const float2* khat;   // m entries, FFT_m of the conjugated chirp, times 1/m (Bluestein)
+};
+
+// Host-owned tables of one length.
+struct LinePlan {
+  LineFFT dev{};
+  DevBuf<float2> roots, chirp, khat;
+};
+
+// Returns the shared plan of length n (built once per process and device).
+const LinePlan& line_plan(int n, hipStream_t s);
+
+// ---- pass descriptions --------------------------------------------------------------------------------------------
+// Row pass (along y, contiguous): flags
+enum RowFlags : int {
+  ROW_IN_COMPACT = 1,   // input: compact columns of G (zero at the pruned ky)
+  ROW_IN_FULL = 2,      // input: a full-width buffer
+  ROW_INV = 4,          // apply the unscaled inverse transform along y first
+  ROW_WRITE_PLANE = 8,  // write the line after the inverse (real space) to `plane`
+  ROW_MUL_T = 16,       // multiply by the transmission t(x, y)
+  ROW_FWD = 32,         // apply the forward transform along y
+  ROW_OUT_COMPACT = 64, // store the kept columns into G
+  ROW_OUT_FULL = 128    // store the full line into `out_full`
+};
+
+struct RowArgs {
+  int nx, ny, nc, flags;
+  LineFFT fy;
+  const int* keep;      // nc: kept iy (or -1 past the kept count)
+  const int* keepinv;   // ny: compact index of iy or -1
+  float2* G;            // compact waves [batch][nx][nc]
+  const float2* in_full;  // [batch][nx][ny]
+  float2* out_full;       // [batch][nx][ny]
+  float2* plane;          // [batch][nx][ny]
+  const float2* t;        // [nx][ny]
+  float out_scale;        // applied to full / plane outputs
+  long long in_full_stride, out_full_stride, plane_stride;
+};
+
+enum ColFlags : int {
+  COL_IN_G = 1,        // input: G columns
+  COL_GENERATE = 2,    // input: probe spectrum x ramps
+  COL_FWD = 4,         // forward transform along x
+  COL_MUL_P = 8,       // multiply by the compact propagator
+  COL_DETECT = 16,     // detectors on |.|^2 (final pass)
+  COL_INV = 32,        // unscaled inverse transform along x
+  COL_OUT = 64         // store back into G
+};
+
+constexpr int kMaxSlots = 16;
+constexpr int kMaxPix = 4;
+constexpr int kMaxCom = 4;
+
+// Detector reduction plan of the final column pass.
+struct DetPlan {
+  int nslots;                   // scalar sums per wave
+  int slot_kind[kMaxSlots];     // 0: annular mask bit, 1: COM x, 2: COM y
+  int slot_arg[kMaxSlots];      // mask bit or COM index
+  const unsigned* mask;         // [nx][ny] annular membership bits
+  const int* com_mapx[kMaxCom];   // [nx] crop output index or -1
+  const int* com_mapy[kMaxCom];   // [ny]
+  const double* com_wx[kMaxCom];  // [nx] angle of the crop row (mrad or 1/A)
+  const double* com_wy[kMaxCom];  // [ny]
+  int npix;
+  const int* pix_mapx[kMaxPix];
+  const int* pix_mapy[kMaxPix];
+  int pix_my[kMaxPix];
+  double* pix_acc[kMaxPix];       // [npos][mx][my] accumulators
+  long long pix_stride[kMaxPix];  // mx * my
+  double* partial;                // [batch][nslots][blocks per wave]
+  int blocks_per_wave;
+  double scale;                   // intensity scale (N^2 for a probe)
+};
+
+struct ColArgs {
+  int nx, ny, nc, flags;
+  LineFFT fx;
+  const int* keep;        // nc: the iy of each compact column (or -1)
+  float2* G;              // [batch][nx][nc]
+  const float2* P;        // [nx][nc]
+  const float2* spec;     // [nx][ny] probe spectrum / N
+  const float2* rx;       // [batch][nx]
+  const float2* ry;       // [batch][ny]
+  int pos0;               // position index of wave 0 (detectors)
+  DetPlan det;
+};
+
+// launchers (sk_passes.hip)
+void launch_row_generic(const RowArgs& a, int batch, hipStream_t s);
+void launch_col_generic(const ColArgs& a, int batch, hipStream_t s);
+void launch_reduce_partials(const DetPlan& d, int batch, int pos0, double* acc, hipStream_t s);
+int col_generic_lines(int m);  // columns per block of the generic column pass
+
+// fast passes (sk_fast.hip): return false when the length has no fast kernel
+bool fast_row_supported(int ny, int nc, int flags);
+bool fast_col_supported(int nx, int nc, int flags);
+void launch_row_fast(const RowArgs& a, int batch, hipStream_t s);
+void launch_col_fast(const ColArgs& a, int batch, hipStream_t s);
+int fast_col_blocks_per_wave(int nx, int nc);
+
+// elementwise kernels (sk_prepare.hip)
+void launch_phase_to_complex(const double* V, double sigma, float2* out, size_t n, hipStream_t s);
+void launch_real_to_complex(const double* re, const double* im, float2* out, size_t n, hipStream_t s);
+void launch_complex_to_split(const float2* in, double* re, double* im, size_t n, double scale, hipStream_t s);
+void launch_antialias(int nx, int ny, double ex, double ey, double* out, hipStream_t s);
+void launch_propagator(int nx, int ny, double ex, double ey, double lambda, double dz, double tx, double ty,
+                       const double* aperture, const int* keep, int nc, double scale, float2* out, hipStream_t s);
+void launch_aperture_as_complex(int nx, int ny, const double* aperture, double scale, float2* out, hipStream_t s);
+void launch_probe_spectrum(int nx, int ny, double ex, double ey, double lambda, double cutoff, int soft,
+                           const double* ab, double* re, double* im, hipStream_t s);
+void launch_sum_squares(const double* re, const double* im, size_t n, double* partial, int blocks, double* out,
+                        hipStream_t s);
+void launch_normalise_spectrum(const double* re, const double* im, const double* norm2, double extra,
+                               size_t n, float2* out, hipStream_t s);
+void launch_ramps(int nx, int ny, double sx, double sy, const double* positions, int pos0, int count,
+                  float2* rx, float2* ry, hipStream_t s);
+void launch_scatter_deltas(const int* idx, const double* w, size_t n, float2* out, hipStream_t s);
+void launch_lobato_kernels(int nx, int ny, double ex, double ey, const int* zs, int nz, int pi_mode,
+                           const double* table, float* out, hipStream_t s);
+void launch_combine_elements(const float2* D, const float* K, int nslices, int nz, size_t N, float2* out,
+                             hipStream_t s);
+void launch_real_part(const float2* in, double* out, size_t n, hipStream_t s);
+void launch_finalise_outputs(const double* acc, size_t n, double inv, void* out, int f32, hipStream_t s);
+void launch_gather_scalar(const double* acc, int nslots, int slot, int npos, double inv, void* out, int stride,
+                          int f32, hipStream_t s);
+
+// ---- device complex helpers ------------------------------------------------------------------------------------
+__host__ __device__ inline float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ inline float2 cmulc(float2 a, float2 b) {  // a * conj(b)
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__host__ __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+
+}  // namespace sk

@@ -1,0 +1,506 @@
+// sk_passes.hip — the generic row and column passes of the batched multislice, for any grid length.
+//
+// One slice of the reference's loop is psi <- IFFT2(P * FFT2(t * psi)) (Multislice.swift:141-155). With the 2-D
+// transform split into a transform along y (rows, contiguous) and one along x (columns), a slice becomes two
+// passes over the batch of waves, each reading and writing every wave once:
+//   row pass:    inverse along y (previous slice) -> * t_j -> forward along y
+//   column pass: forward along x -> * P -> inverse along x
+// The waves live between passes in "compact" form G[b][x][c]: only the columns c whose ky lies inside the antialias
+// disk (Antialias.swift:18-39) are stored, since P is zero on every other column and they carry nothing into the
+// next slice. The generic passes below stage each line in LDS and transform it there (Stockham autosort, radix
+// 2/3/4/5/7/8/11/13, Bluestein's algorithm for other lengths); the fast passes (sk_fast.hip) keep lines in
+// registers for the lengths that have them.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <complex>
+#include <map>
+
+#include "sk_internal.h"
+
+namespace sk {
+
+// ---------------------------------------------------------------------------------------------------------------
+// Small DFTs (sign -1 forward, +1 inverse, unscaled)
+// ---------------------------------------------------------------------------------------------------------------
+__device__ inline float2 mul_si(float2 a, int sign) {  // a * (sign * i)
+  return sign < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+
+__device__ inline void dft2(float2& a, float2& b) {
+  float2 t = a;
+  a = cadd(t, b);
+  b = csub(t, b);
+}
+
+__device__ inline void dft4(float2& a0, float2& a1, float2& a2, float2& a3, int sign) {
+  float2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = mul_si(csub(a1, a3), sign);
+  a0 = cadd(t0, t2);
+  a2 = csub(t0, t2);
+  a1 = cadd(t1, t3);
+  a3 = csub(t1, t3);
+}
+
+template <int R>
+__device__ inline void dft_small(float2 (&u)[R], int sign, const float2* roots, int m) {
+  if constexpr (R == 2) {
+    dft2(u[0], u[1]);
+  } else if constexpr (R == 4) {
+    dft4(u[0], u[1], u[2], u[3], sign);
+  } else if constexpr (R == 8) {
+    float2 e0 = u[0], e1 = u[2], e2 = u[4], e3 = u[6];
+    float2 o0 = u[1], o1 = u[3], o2 = u[5], o3 = u[7];
+    dft4(e0, e1, e2, e3, sign);
+    dft4(o0, o1, o2, o3, sign);
+    const float h = 0.70710678118654752f;
+    const float s = float(sign);
+    // o_k * w8^k, w8 = e^{sign i pi / 4}
+    float2 w1 = make_float2((o1.x - s * o1.y) * h, (o1.y + s * o1.x) * h);
+    float2 w2 = mul_si(o2, sign);
+    float2 w3 = make_float2((-o3.x - s * o3.y) * h, (-o3.y + s * o3.x) * h);
+    u[0] = cadd(e0, o0);
+    u[4] = csub(e0, o0);
+    u[1] = cadd(e1, w1);
+    u[5] = csub(e1, w1);
+    u[2] = cadd(e2, w2);
+    u[6] = csub(e2, w2);
+    u[3] = cadd(e3, w3);
+    u[7] = csub(e3, w3);
+  } else {
+    // direct DFT of an odd prime radix with the line's roots: e^{-2 pi i j q / R} = roots[(j q mod R) m / R]
+    float2 v[R];
+    const int step = m / R;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      float2 acc = u[0];
+#pragma unroll
+      for (int j = 1; j < R; ++j) {
+        float2 w = roots[((j * q) % R) * step];
+        if (sign > 0) w.y = -w.y;
+        acc = cadd(acc, cmul(u[j], w));
+      }
+      v[q] = acc;
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) u[q] = v[q];
+  }
+}
+
+// One Stockham pass of radix R over `lines` lines of length m (line stride ls): for butterfly i in [0, m/R),
+// k = i mod L, u_j = x[i + j m/R] * w_{L R}^{j k}, DFT_R(u), y[(i - k) R + k + j L] = u_j.
+template <int R>
+__device__ void stockham_stage(const float2* __restrict__ src, float2* __restrict__ dst, int lines, int m, int ls,
+                               int L, const float2* __restrict__ roots, int sign, int tid, int T) {
+  const int stride = m / R;
+  const int step = m / (L * R);
+  const int total = lines * stride;
+  for (int w = tid; w < total; w += T) {
+    const int line = w / stride;
+    const int i = w - line * stride;
+    const float2* x = src + line * ls;
+    float2* y = dst + line * ls;
+    const int k = i % L;
+    float2 u[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      float2 v = x[i + j * stride];
+      if (j > 0 && k > 0) {
+        float2 tw = roots[(j * k * step) % m];
+        if (sign > 0) tw.y = -tw.y;
+        v = cmul(v, tw);
+      }
+      u[j] = v;
+    }
+    dft_small<R>(u, sign, roots, m);
+    const int base = (i - k) * R + k;
+#pragma unroll
+    for (int j = 0; j < R; ++j) y[base + j * L] = u[j];
+  }
+}
+
+// Unscaled Stockham transform of length p.m of every line; the data starts in `a` (synchronised); returns the
+// buffer that holds the result.
+__device__ float2* lds_stockham(float2* a, float2* b, int lines, int ls, const LineFFT& p, int sign, int tid, int T) {
+  int L = 1;
+  for (int s = 0; s < p.nst; ++s) {
+    const int r = p.radix[s];
+    switch (r) {
+      case 2: stockham_stage<2>(a, b, lines, p.m, ls, L, p.roots, sign, tid, T); break;
+      case 3: stockham_stage<3>(a, b, lines, p.m, ls, L, p.roots, sign, tid, T); break;
+      case 4: stockham_stage<4>(a, b, lines, p.m, ls, L, p.roots, sign, tid, T); break;
+      case 5: stockham_stage<5>(a, b, lines, p.m, ls, L, p.roots, sign, tid, T); break;
+      case 7: stockham_stage<7>(a, b, lines, p.m, ls, L, p.roots, sign, tid, T); break;
+      case 8: stockham_stage<8>(a, b, lines, p.m, ls, L, p.roots, sign, tid, T); break;
+      case 11: stockham_stage<11>(a, b, lines, p.m, ls, L, p.roots, sign, tid, T); break;
+      default: stockham_stage<13>(a, b, lines, p.m, ls, L, p.roots, sign, tid, T); break;
+    }
+    __syncthreads();
+    float2* t = a;
+    a = b;
+    b = t;
+    L *= r;
+  }
+  return a;
+}
+
+// Unscaled transform of logical length p.n of every line (entries [0, n) of each line of stride ls).
+// Bluestein (1970): X_k = c_k sum_j (x_j c_j) conj(c_{k-j}), c_k = e^{sign pi i k^2 / n}, the convolution done with
+// two power-of-two transforms of length m.
+__device__ float2* lds_line_transform(float2* a, float2* b, int lines, int ls, const LineFFT& p, int sign, int tid,
+                                      int T) {
+  if (!p.blue) return lds_stockham(a, b, lines, ls, p, sign, tid, T);
+  const int n = p.n, m = p.m;
+  for (int idx = tid; idx < lines * m; idx += T) {
+    const int l = idx / m, k = idx - l * m;
+    float2 v = make_float2(0.f, 0.f);
+    if (k < n) {
+      float2 c = p.chirp[k];
+      if (sign > 0) c.y = -c.y;
+      v = cmul(a[l * ls + k], c);
+    }
+    a[l * ls + k] = v;
+  }
+  __syncthreads();
+  float2* r = lds_stockham(a, b, lines, ls, p, -1, tid, T);
+  for (int idx = tid; idx < lines * m; idx += T) {
+    const int l = idx / m, k = idx - l * m;
+    float2 kh = p.khat[k];
+    if (sign > 0) kh.y = -kh.y;
+    r[l * ls + k] = cmul(r[l * ls + k], kh);
+  }
+  __syncthreads();
+  float2* o = (r == a) ? b : a;
+  r = lds_stockham(r, o, lines, ls, p, +1, tid, T);
+  for (int idx = tid; idx < lines * n; idx += T) {
+    const int l = idx / n, k = idx - l * n;
+    float2 c = p.chirp[k];
+    if (sign > 0) c.y = -c.y;
+    r[l * ls + k] = cmul(r[l * ls + k], c);
+  }
+  __syncthreads();
+  return r;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Generic row pass: block = `rows` lines along y of one wave (blockIdx.y).
+// ---------------------------------------------------------------------------------------------------------------
+constexpr int kGenericThreads = 256;
+constexpr int kGenericLdsComplex = 2048;  // complex values per LDS buffer (16 KiB); two buffers
+
+inline int row_lines(int m) { return m >= kGenericLdsComplex ? 1 : kGenericLdsComplex / m; }
+int col_generic_lines(int m) { return m >= kGenericLdsComplex ? 1 : kGenericLdsComplex / m; }
+
+__global__ void __launch_bounds__(kGenericThreads) row_generic_kernel(RowArgs a, int rows) {
+  extern __shared__ float2 lds[];
+  const int m = a.fy.m, ls = m;
+  float2* A = lds;
+  float2* B = lds + rows * ls;
+  const int tid = threadIdx.x, T = blockDim.x;
+  const int x0 = blockIdx.x * rows;
+  const long long b = blockIdx.y;
+  const int f = a.flags;
+  const long long gw = (long long)a.nx * a.nc;
+
+  for (int idx = tid; idx < rows * m; idx += T) {
+    const int l = idx / m, k = idx - l * m, x = x0 + l;
+    float2 v = make_float2(0.f, 0.f);
+    if (x < a.nx && k < a.ny) {
+      if (f & ROW_IN_COMPACT) {
+        const int c = a.keepinv[k];
+        if (c >= 0) v = a.G[b * gw + (long long)x * a.nc + c];
+      } else {
+        v = a.in_full[b * a.in_full_stride + (long long)x * a.ny + k];
+      }
+    }
+    A[l * ls + k] = v;
+  }
+  __syncthreads();
+  float2* cur = A;
+  float2* oth = B;
+  if (f & ROW_INV) {
+    cur = lds_line_transform(cur, oth, rows, ls, a.fy, +1, tid, T);
+    oth = (cur == A) ? B : A;
+  }
+  if (f & (ROW_WRITE_PLANE | ROW_MUL_T)) {
+    for (int idx = tid; idx < rows * a.ny; idx += T) {
+      const int l = idx / a.ny, k = idx - l * a.ny, x = x0 + l;
+      if (x >= a.nx) continue;
+      float2 v = cur[l * ls + k];
+      if (f & ROW_WRITE_PLANE) a.plane[b * a.plane_stride + (long long)x * a.ny + k] = cscale(v, a.out_scale);
+      if (f & ROW_MUL_T) cur[l * ls + k] = cmul(v, a.t[(long long)x * a.ny + k]);
+    }
+    __syncthreads();
+  }
+  if (f & ROW_FWD) cur = lds_line_transform(cur, oth, rows, ls, a.fy, -1, tid, T);
+  if (f & ROW_OUT_COMPACT) {
+    for (int idx = tid; idx < rows * a.nc; idx += T) {
+      const int l = idx / a.nc, c = idx - l * a.nc, x = x0 + l;
+      if (x >= a.nx) continue;
+      const int iy = a.keep[c];
+      a.G[b * gw + (long long)x * a.nc + c] = iy >= 0 ? cur[l * ls + iy] : make_float2(0.f, 0.f);
+    }
+  }
+  if (f & ROW_OUT_FULL) {
+    for (int idx = tid; idx < rows * a.ny; idx += T) {
+      const int l = idx / a.ny, k = idx - l * a.ny, x = x0 + l;
+      if (x >= a.nx) continue;
+      a.out_full[b * a.out_full_stride + (long long)x * a.ny + k] = cscale(cur[l * ls + k], a.out_scale);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Detector reduction of one block's pixels (final column pass). `pix(idx)` yields (ix, iy, intensity) or iy < 0.
+// ---------------------------------------------------------------------------------------------------------------
+__device__ inline void block_reduce_slots(double (&s)[kMaxSlots], int nslots, double* scratch, int tid, int T,
+                                          double* out, int out_stride) {
+  // deterministic tree reduction per slot; `scratch` holds T doubles
+  for (int q = 0; q < nslots; ++q) {
+    scratch[tid] = s[q];
+    __syncthreads();
+    for (int w = T / 2; w > 0; w >>= 1) {
+      if (tid < w) scratch[tid] += scratch[tid + w];
+      __syncthreads();
+    }
+    if (tid == 0) out[q * out_stride] = scratch[0];
+    __syncthreads();
+  }
+}
+
+__device__ inline void detect_pixel(const DetPlan& d, int ix, int iy, int ny, double I, long long pos,
+                                    double (&s)[kMaxSlots]) {
+  unsigned bits = d.mask ? d.mask[(long long)ix * ny + iy] : 0u;
+  for (int q = 0; q < d.nslots; ++q) {
+    const int kind = d.slot_kind[q], arg = d.slot_arg[q];
+    if (kind == 0) {
+      if ((bits >> arg) & 1u) s[q] += I;
+    } else {
+      const int mx = d.com_mapx[arg][ix], my = d.com_mapy[arg][iy];
+      if (mx >= 0 && my >= 0) s[q] += I * (kind == 1 ? d.com_wx[arg][ix] : d.com_wy[arg][iy]);
+    }
+  }
+  for (int p = 0; p < d.npix; ++p) {
+    const int ox = d.pix_mapx[p][ix], oy = d.pix_mapy[p][iy];
+    if (ox >= 0 && oy >= 0) d.pix_acc[p][pos * d.pix_stride[p] + (long long)ox * d.pix_my[p] + oy] += I;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Generic column pass: block = `cols` compact columns (lines along x) of one wave.
+// ---------------------------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kGenericThreads) col_generic_kernel(ColArgs a, int cols) {
+  extern __shared__ float2 lds[];
+  const int m = a.fx.m, ls = m;
+  float2* A = lds;
+  float2* B = lds + cols * ls;
+  const int tid = threadIdx.x, T = blockDim.x;
+  const int c0 = blockIdx.x * cols;
+  const long long b = blockIdx.y;
+  const int f = a.flags;
+  const long long gw = (long long)a.nx * a.nc;
+
+  for (int idx = tid; idx < cols * m; idx += T) {
+    const int k = idx / cols, l = idx - k * cols, c = c0 + l;
+    float2 v = make_float2(0.f, 0.f);
+    if (k < a.nx && c < a.nc) {
+      if (f & COL_IN_G) {
+        v = a.G[b * gw + (long long)k * a.nc + c];
+      } else if (f & COL_GENERATE) {
+        const int iy = a.keep[c];
+        if (iy >= 0)
+          v = cmul(cmul(a.spec[(long long)k * a.ny + iy], a.rx[b * a.nx + k]), a.ry[b * a.ny + iy]);
+      }
+    }
+    A[l * ls + k] = v;
+  }
+  __syncthreads();
+  float2* cur = A;
+  float2* oth = B;
+  if (f & COL_FWD) {
+    cur = lds_line_transform(cur, oth, cols, ls, a.fx, -1, tid, T);
+    oth = (cur == A) ? B : A;
+  }
+  if (f & COL_MUL_P) {
+    for (int idx = tid; idx < cols * a.nx; idx += T) {
+      const int k = idx / cols, l = idx - k * cols, c = c0 + l;
+      if (c < a.nc) cur[l * ls + k] = cmul(cur[l * ls + k], a.P[(long long)k * a.nc + c]);
+    }
+    __syncthreads();
+  }
+  if (f & COL_DETECT) {
+    double s[kMaxSlots];
+#pragma unroll
+    for (int q = 0; q < kMaxSlots; ++q) s[q] = 0.0;
+    const long long pos = a.pos0 + b;
+    for (int idx = tid; idx < cols * a.nx; idx += T) {
+      const int k = idx / cols, l = idx - k * cols, c = c0 + l;
+      if (c >= a.nc) continue;
+      const int iy = a.keep[c];
+      if (iy < 0) continue;
+      const float2 v = cur[l * ls + k];
+      const double I = ((double)v.x * v.x + (double)v.y * v.y) * a.det.scale;
+      detect_pixel(a.det, k, iy, a.ny, I, pos, s);
+    }
+    __syncthreads();
+    double* scratch = reinterpret_cast<double*>(oth);
+    block_reduce_slots(s, a.det.nslots, scratch, tid, T,
+                       a.det.partial + (b * a.det.nslots) * a.det.blocks_per_wave + blockIdx.x,
+                       a.det.blocks_per_wave);
+    return;
+  }
+  if (f & COL_INV) cur = lds_line_transform(cur, oth, cols, ls, a.fx, +1, tid, T);
+  if (f & COL_OUT) {
+    for (int idx = tid; idx < cols * a.nx; idx += T) {
+      const int k = idx / cols, l = idx - k * cols, c = c0 + l;
+      if (c < a.nc) a.G[b * gw + (long long)k * a.nc + c] = cur[l * ls + k];
+    }
+  }
+}
+
+// Sum the per-block partials of every wave in block order and add them to the position's accumulators.
+__global__ void reduce_partials_kernel(DetPlan d, int pos0, double* acc) {
+  const long long b = blockIdx.x;
+  for (int q = threadIdx.x; q < d.nslots; q += blockDim.x) {
+    const double* p = d.partial + (b * d.nslots + q) * d.blocks_per_wave;
+    double s = 0.0;
+    for (int i = 0; i < d.blocks_per_wave; ++i) s += p[i];
+    acc[(pos0 + b) * d.nslots + q] += s;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------------------------------------------
+void launch_row_generic(const RowArgs& a, int batch, hipStream_t s) {
+  const int rows = row_lines(a.fy.m);
+  dim3 grid((a.nx + rows - 1) / rows, batch);
+  const size_t lds = 2 * sizeof(float2) * rows * a.fy.m;
+  hipLaunchKernelGGL(row_generic_kernel, grid, dim3(kGenericThreads), lds, s, a, rows);
+  SK_HIP(hipGetLastError());
+}
+
+void launch_col_generic(const ColArgs& a, int batch, hipStream_t s) {
+  const int cols = col_generic_lines(a.fx.m);
+  dim3 grid((a.nc + cols - 1) / cols, batch);
+  size_t lds = 2 * sizeof(float2) * cols * a.fx.m;
+  if (a.flags & COL_DETECT) lds = std::max(lds, sizeof(float2) * cols * a.fx.m + sizeof(double) * kGenericThreads);
+  hipLaunchKernelGGL(col_generic_kernel, grid, dim3(kGenericThreads), lds, s, a, cols);
+  SK_HIP(hipGetLastError());
+}
+
+void launch_reduce_partials(const DetPlan& d, int batch, int pos0, double* acc, hipStream_t s) {
+  if (d.nslots == 0) return;
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(batch), dim3(64), 0, s, d, pos0, acc);
+  SK_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Plans (host): radices, roots and Bluestein tables, computed in float64 and rounded to complex64.
+// ---------------------------------------------------------------------------------------------------------------
+namespace {
+
+using cd = std::complex<double>;
+
+std::vector<int> smooth_radices(int n) {
+  std::vector<int> r;
+  while (n % 8 == 0) { r.push_back(8); n /= 8; }
+  while (n % 4 == 0) { r.push_back(4); n /= 4; }
+  while (n % 2 == 0) { r.push_back(2); n /= 2; }
+  for (int p : {3, 5, 7, 11, 13})
+    while (n % p == 0) { r.push_back(p); n /= p; }
+  if (n != 1) r.clear();
+  return r;
+}
+
+cd root(long long k, long long n) {  // e^{-2 pi i k / n}
+  k %= n;
+  double a = 2.0 * M_PI * double(k) / double(n);
+  return cd(std::cos(a), -std::sin(a));
+}
+
+void host_fft_pow2(std::vector<cd>& a) {  // forward, unscaled, iterative radix-2 (tables only)
+  const size_t n = a.size();
+  for (size_t i = 1, j = 0; i < n; ++i) {
+    size_t bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) std::swap(a[i], a[j]);
+  }
+  for (size_t len = 2; len <= n; len <<= 1)
+    for (size_t i = 0; i < n; i += len)
+      for (size_t j = 0; j < len / 2; ++j) {
+        cd w = root((long long)(j * (n / len)), (long long)n);
+        cd u = a[i + j], v = a[i + j + len / 2] * w;
+        a[i + j] = u + v;
+        a[i + j + len / 2] = u - v;
+      }
+}
+
+std::vector<float2> to_f2(const std::vector<cd>& v) {
+  std::vector<float2> o(v.size());
+  for (size_t i = 0; i < v.size(); ++i) o[i] = make_float2(float(v[i].real()), float(v[i].imag()));
+  return o;
+}
+
+std::map<int, std::unique_ptr<LinePlan>>& plans() {
+  static auto* p = new std::map<int, std::unique_ptr<LinePlan>>;  // process lifetime: never freed at exit
+  return *p;
+}
+
+}  // namespace
+
+const LinePlan& line_plan(int n, hipStream_t s) {
+  auto& ps = plans();
+  auto it = ps.find(n);
+  if (it != ps.end()) return *it->second;
+  if (n < 1) fail(SK_ERR_INVALID, "grid length must be at least 1");
+  auto lp = std::make_unique<LinePlan>();
+  LineFFT& d = lp->dev;
+  d.n = n;
+  auto r = smooth_radices(n);
+  int m = n;
+  if (r.empty() && n > 1) {
+    d.blue = 1;
+    m = 16;
+    while (m < 2 * n - 1) m <<= 1;
+    r = smooth_radices(m);
+  } else {
+    d.blue = 0;
+  }
+  if (m > 8192) fail(SK_ERR_INVALID, "grid length " + std::to_string(n) + " exceeds the device FFT's LDS tile");
+  if ((int)r.size() > kMaxStages) fail(SK_ERR_INVALID, "too many FFT stages");
+  d.m = m;
+  d.nst = (int)r.size();
+  for (size_t i = 0; i < r.size(); ++i) d.radix[i] = r[i];
+  std::vector<cd> roots(m);
+  for (int k = 0; k < m; ++k) roots[k] = root(k, m);
+  lp->roots.upload(to_f2(roots), s);
+  d.roots = lp->roots.p;
+  if (d.blue) {
+    std::vector<cd> chirp(n);
+    for (int k = 0; k < n; ++k) {
+      long long k2 = (long long)k * k % (2LL * n);
+      double a = M_PI * double(k2) / double(n);
+      chirp[k] = cd(std::cos(a), -std::sin(a));
+    }
+    std::vector<cd> bk(m, cd(0, 0));
+    bk[0] = std::conj(chirp[0]);
+    for (int k = 1; k < n; ++k) bk[k] = bk[m - k] = std::conj(chirp[k]);
+    host_fft_pow2(bk);
+    for (auto& v : bk) v /= double(m);
+    lp->chirp.upload(to_f2(chirp), s);
+    lp->khat.upload(to_f2(bk), s);
+    d.chirp = lp->chirp.p;
+    d.khat = lp->khat.p;
+  } else {
+    d.chirp = nullptr;
+    d.khat = nullptr;
+  }
+  SK_HIP(hipStreamSynchronize(s));
+  auto& ref = *lp;
+  ps[n] = std::move(lp);
+  return ref;
+}
+
+}  // namespace sk

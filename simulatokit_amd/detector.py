@@ -1,0 +1,155 @@
+"""Detectors and the STEM driver of the `.hip` backend.
+
+Mirrors Sources/SimulatoDetector/{Detectors,STEM}.swift. The detectors are reduced inside the final column pass on
+the device (float64 accumulation), so no diffraction pattern leaves HBM unless a pixelated detector asks for it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+from typing import List, Optional, Sequence, Tuple, Union
+
+import numpy as np
+
+from . import _native as N
+from .core import ExecutionOptions, SimulatoError
+from .multislice import AnyPotential, PreparedPotential, _options, _prepared, _raise, _sk_grid
+from .probe import GridScan, Probe
+
+MAX_ANGLE = {"cutoff": 0, "valid": 1, "full": 2}
+
+
+@dataclass(frozen=True)
+class AnnularDetector:
+    """Sum of P over inner^2 <= alpha^2 < outer^2 [mrad]; outer None = floor of the cutoff angle
+    (Detectors.swift:91-136)."""
+
+    inner: float = 0.0
+    outer: Optional[float] = None
+
+    def __post_init__(self):
+        if not (math.isfinite(self.inner) and self.inner >= 0):
+            raise SimulatoError("invalidDetector", "invalid detector: inner must be finite and >= 0")
+        if self.outer is not None and not (math.isfinite(self.outer) and self.outer >= self.inner):
+            raise SimulatoError("invalidDetector", f"invalid detector: outer must be finite and >= inner, got {self.outer}")
+
+    def descriptor(self) -> N.SkDetector:
+        return N.SkDetector(0, self.inner, -1.0 if self.outer is None else self.outer, 0, 0, 0, 0, 0)
+
+
+@dataclass(frozen=True)
+class PixelatedDetector:
+    """The pattern cropped and fftshifted (Detectors.swift:140-167): max_angle "cutoff", "valid" or "full"; or a
+    numeric crop (mx, my) of abTEM's fft_crop, which the reference does not offer; dtype "float64" or "float32"."""
+
+    max_angle: str = "valid"
+    crop: Optional[Tuple[int, int]] = None
+    dtype: str = "float64"
+
+    def descriptor(self) -> N.SkDetector:
+        cx, cy = self.crop if self.crop else (0, 0)
+        return N.SkDetector(1, 0.0, 0.0, MAX_ANGLE[self.max_angle], 0, int(cx), int(cy),
+                            1 if self.dtype == "float32" else 0)
+
+
+@dataclass(frozen=True)
+class CenterOfMassDetector:
+    """(sum P alpha_x, sum P alpha_y) over the pixelated region, not divided by sum P; units "mrad" or
+    "inverse_angstroms" (Detectors.swift:172-206)."""
+
+    max_angle: str = "valid"
+    units: str = "inverse_angstroms"
+
+    def descriptor(self) -> N.SkDetector:
+        return N.SkDetector(2, 0.0, 0.0, MAX_ANGLE[self.max_angle], 0 if self.units == "mrad" else 1, 0, 0, 0)
+
+
+Detector = Union[AnnularDetector, PixelatedDetector, CenterOfMassDetector]
+
+
+@dataclass
+class ScanMeasurement:
+    """One detector's values over a scan, row-major (scan x, scan y, the detector's shape) (STEM.swift:14-43)."""
+
+    scan_shape: Tuple[int, ...]
+    value_shape: Tuple[int, ...]
+    values: np.ndarray
+
+    def __getitem__(self, ix_iy):
+        ix, iy = ix_iy
+        return self.values.reshape(self.scan_shape + self.value_shape)[ix, iy]
+
+    def image(self) -> Optional[np.ndarray]:
+        if self.value_shape != () or len(self.scan_shape) != 2:
+            return None
+        return self.values.reshape(self.scan_shape)
+
+
+def detector_shape(d: Detector, grid, energy: float) -> Tuple[int, ...]:
+    desc = d.descriptor()
+    shape = np.zeros(2, dtype=np.int32)
+    rc = N.lib().sk_hip_detector_shape(C.byref(_sk_grid(grid)), energy, C.byref(desc), N.iptr(shape))
+    if rc < 0:
+        _raise(rc, "detector")
+    if isinstance(d, AnnularDetector):
+        return ()
+    if isinstance(d, CenterOfMassDetector):
+        return (2,)
+    return (int(shape[0]), int(shape[1]))
+
+
+class STEM:
+    @staticmethod
+    def scan(probe: Probe, potentials: Sequence[AnyPotential], scan: Union[GridScan, np.ndarray],
+             detectors: Sequence[Detector], options: Optional[ExecutionOptions] = None,
+             device_outputs: Optional[Sequence[int]] = None) -> List[ScanMeasurement]:
+        """Every position's probe through every potential, each detector averaged over the potentials
+        (STEM.scan, STEM.swift:49-81, 120-150). `scan` is a GridScan or an [npos, 2] array of positions (a line).
+        `device_outputs`: optional device pointers (one per detector) that receive the values instead of host
+        arrays; the returned measurements then hold None."""
+        if not potentials:
+            raise SimulatoError("invalidScan", "invalid scan: no potential")
+        if isinstance(scan, GridScan):
+            positions, scan_shape = scan.positions, scan.shape
+        else:
+            positions = np.asarray(scan, dtype=np.float64).reshape(-1, 2)
+            scan_shape = (len(positions),)
+        for p in potentials:
+            if not p.grid.matches(probe.grid):
+                raise SimulatoError("gridMismatch", "the waves and the potential are on different grids")
+        prepared = [_prepared(p, probe.beam, probe.tilt) for p in potentials]
+        handles = (C.c_void_p * len(prepared))(*[p.handle.value for p in prepared])
+        pos = np.ascontiguousarray(positions, dtype=np.float64)
+        descs = (N.SkDetector * max(1, len(detectors)))(*[d.descriptor() for d in detectors])
+        shapes = [detector_shape(d, probe.grid, probe.beam.energy) for d in detectors]
+        outs, ptrs = [], []
+        for d, s in zip(detectors, shapes):
+            dt = np.float32 if getattr(d, "dtype", "float64") == "float32" else np.float64
+            a = np.empty((len(pos),) + s, dtype=dt)
+            outs.append(a)
+            ptrs.append(a.ctypes.data)
+        on_device = device_outputs is not None
+        if on_device:
+            ptrs = list(device_outputs)
+        ptr_arr = (C.c_void_p * max(1, len(ptrs)))(*ptrs)
+        opt = _options(options)
+        _raise(N.lib().sk_hip_scan(handles, len(prepared), C.byref(probe.descriptor()), len(pos), N.dptr(pos), descs,
+                                   len(detectors), ptr_arr, 1 if on_device else 0, C.byref(opt)), "STEM.scan")
+        return [ScanMeasurement(tuple(scan_shape), s, None if on_device else a.reshape(-1))
+                for s, a in zip(shapes, outs)]
+
+
+def last_timing() -> dict:
+    """Device timing of the last scan (see sk_hip_last_timing)."""
+    t = N.SkTiming()
+    N.lib().sk_hip_last_timing(C.byref(t))
+    return {f: getattr(t, f) for f, _ in N.SkTiming._fields_}
+
+
+def set_timing(enabled: bool):
+    N.lib().sk_hip_set_timing(1 if enabled else 0)
+
+
+__all__ = ["AnnularDetector", "PixelatedDetector", "CenterOfMassDetector", "ScanMeasurement", "STEM", "last_timing",
+           "set_timing", "detector_shape"]

@@ -1,0 +1,181 @@
+"""Probe, aperture, aberrations and grid scans of the `.hip` backend.
+
+Mirrors Sources/SimulatoProbe/{Probe,Aberrations,GridScan}.swift. Probes are built on the device
+(include/simulato_hip.h: sk_hip_probes), as the reference's Metal path builds them (MetalEngine.swift:539-553).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, fields
+from typing import List, Sequence, Tuple
+
+import numpy as np
+
+from . import _native as N
+from .core import ElectronBeam, Grid, Normalization, SimulatoError, Waves, as_float_pair
+
+ABERRATION_NAMES = ("C10", "C12", "phi12", "C21", "phi21", "C23", "phi23", "C30", "C32", "phi32", "C34", "phi34",
+                    "C41", "phi41", "C43", "phi43", "C45", "phi45", "C50", "C52", "phi52", "C54", "phi54", "C56",
+                    "phi56")
+
+
+@dataclass
+class Aberrations:
+    """Polar aberration coefficients, Angstrom and rad; defocus = -C10 (Aberrations.swift:10-60)."""
+
+    C10: float = 0.0
+    C12: float = 0.0
+    phi12: float = 0.0
+    C21: float = 0.0
+    phi21: float = 0.0
+    C23: float = 0.0
+    phi23: float = 0.0
+    C30: float = 0.0
+    C32: float = 0.0
+    phi32: float = 0.0
+    C34: float = 0.0
+    phi34: float = 0.0
+    C41: float = 0.0
+    phi41: float = 0.0
+    C43: float = 0.0
+    phi43: float = 0.0
+    C45: float = 0.0
+    phi45: float = 0.0
+    C50: float = 0.0
+    C52: float = 0.0
+    phi52: float = 0.0
+    C54: float = 0.0
+    phi54: float = 0.0
+    C56: float = 0.0
+    phi56: float = 0.0
+
+    @property
+    def defocus(self) -> float:
+        return -self.C10
+
+    @defocus.setter
+    def defocus(self, v: float):
+        self.C10 = -v
+
+    @property
+    def Cs(self) -> float:
+        return self.C30
+
+    @Cs.setter
+    def Cs(self, v: float):
+        self.C30 = v
+
+    def vector(self) -> List[float]:
+        return [float(getattr(self, n)) for n in ABERRATION_NAMES]
+
+    @property
+    def is_finite(self) -> bool:
+        return all(math.isfinite(getattr(self, f.name)) for f in fields(self))
+
+
+@dataclass(frozen=True)
+class Aperture:
+    """Semiangle cutoff [mrad] (infinity: none) with abTEM's one-pixel soft edge (Probe.swift:12-49)."""
+
+    semiangle_cutoff: float
+    soft: bool = True
+
+    def __post_init__(self):
+        if not (self.semiangle_cutoff > 0) or math.isnan(self.semiangle_cutoff):
+            raise SimulatoError("invalidAperture", f"semiangle cutoff must be finite and positive (or infinite), got {self.semiangle_cutoff} mrad")
+
+
+class Probe:
+    """A convergent probe (Probe.swift:52-157)."""
+
+    def __init__(self, beam: ElectronBeam, grid: Grid, aperture: Aperture, aberrations: Aberrations = None,
+                 tilt=(0.0, 0.0)):
+        aberrations = aberrations or Aberrations()
+        if not aberrations.is_finite:
+            raise SimulatoError("invalidAberration", "invalid aberration: every coefficient must be finite")
+        tilt = as_float_pair(tilt)
+        if not (math.isfinite(tilt[0]) and math.isfinite(tilt[1])):
+            raise SimulatoError("invalidTilt", "invalid tilt: the tilt must be finite")
+        self.beam, self.grid, self.aperture, self.aberrations, self.tilt = beam, grid, aperture, aberrations, tilt
+
+    def descriptor(self) -> N.SkProbe:
+        p = N.SkProbe()
+        p.energy = self.beam.energy
+        p.semiangle_mrad = self.aperture.semiangle_cutoff
+        p.soft = 1 if self.aperture.soft else 0
+        for i, v in enumerate(self.aberrations.vector()):
+            p.aberrations[i] = v
+        p.tilt[0], p.tilt[1] = self.tilt
+        return p
+
+    def build_many(self, positions: Sequence[Sequence[float]]) -> np.ndarray:
+        """Real-space probes at the positions [npos, nx, ny] (complex128), sum_k |Psi(k)|^2 = 1."""
+        pos = np.ascontiguousarray(np.asarray(positions, dtype=np.float64).reshape(-1, 2))
+        if not np.isfinite(pos).all():
+            raise SimulatoError("invalidPosition", "the probe position must be finite")
+        g = self.grid
+        re = np.empty((len(pos), g.nx, g.ny))
+        im = np.empty_like(re)
+        from .multislice import _raise, _sk_grid
+        _raise(N.lib().sk_hip_probes(C.byref(_sk_grid(g)), C.byref(self.descriptor()), len(pos), N.dptr(pos),
+                                     N.dptr(re), N.dptr(im)), "Probe.build")
+        return re + 1j * im
+
+    def build(self, at: Sequence[float]) -> Waves:
+        """The probe centred at `at` (x, y) [Angstrom] (Probe.build(at:), Probe.swift:72-109)."""
+        return Waves(self.build_many([at])[0], self.grid, self.beam, Normalization.RECIPROCAL_SPACE, self.tilt)
+
+
+class GridScan:
+    """A rectangular grid of probe positions (GridScan.swift:10-67), index ix * ny + iy."""
+
+    def __init__(self, start: Sequence[float], end: Sequence[float], nx: int, ny: int, endpoint: bool = False):
+        if nx < 1 or ny < 1:
+            raise SimulatoError("invalidScan", f"invalid scan: gpts must be at least 1, got {nx} x {ny}")
+        start, end = as_float_pair(start), as_float_pair(end)
+        if not all(math.isfinite(v) for v in start + end):
+            raise SimulatoError("invalidScan", "invalid scan: start and end must be finite")
+        if not (end[0] - start[0] > 0 or end[1] - start[1] > 0):
+            raise SimulatoError("invalidScan", "invalid scan: scan extent must be positive on at least one axis (abTEM's rule)")
+        self.start, self.end, self.nx, self.ny, self.endpoint = start, end, int(nx), int(ny), endpoint
+
+    @staticmethod
+    def with_sampling(start, end, sampling_x: float, sampling_y: float, endpoint: bool = False) -> "GridScan":
+        """ceil(extent / sampling) points per axis (GridScan.swift:251-265)."""
+        if not (math.isfinite(sampling_x) and math.isfinite(sampling_y) and sampling_x > 0 and sampling_y > 0):
+            raise SimulatoError("invalidScan", f"invalid scan: sampling must be finite and positive, got ({sampling_x}, {sampling_y}) A")
+        start, end = as_float_pair(start), as_float_pair(end)
+        ex, ey = end[0] - start[0], end[1] - start[1]
+        if ex < 0 or ey < 0:
+            raise SimulatoError("invalidScan", "invalid scan: end must not lie before start")
+        px = math.ceil(ex / sampling_x) if ex > 0 else 0
+        py = math.ceil(ey / sampling_y) if ey > 0 else 0
+        extra = 1 if endpoint else 0
+        return GridScan(start, end, max(px + (extra if ex > 0 else 1), 1), max(py + (extra if ey > 0 else 1), 1),
+                        endpoint)
+
+    @staticmethod
+    def linspace(a: float, b: float, n: int, endpoint: bool) -> np.ndarray:
+        """NumPy's linspace: i * step + start (GridScan.swift:269-276)."""
+        div = n - 1 if endpoint else n
+        if div <= 0:
+            return np.array([a])
+        step = (b - a) / float(div)
+        out = np.arange(n, dtype=np.float64) * step + a
+        if endpoint and n > 1:
+            out[-1] = b
+        return out
+
+    @property
+    def shape(self) -> Tuple[int, int]:
+        return (self.nx, self.ny)
+
+    @property
+    def positions(self) -> np.ndarray:
+        xs = self.linspace(self.start[0], self.end[0], self.nx, self.endpoint)
+        ys = self.linspace(self.start[1], self.end[1], self.ny, self.endpoint)
+        return np.stack(np.meshgrid(xs, ys, indexing="ij"), axis=-1).reshape(-1, 2)
+
+
+__all__ = ["Aberrations", "Aperture", "Probe", "GridScan", "ABERRATION_NAMES"]
