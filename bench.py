@@ -1,0 +1,166 @@
+"""Benchmark of the multislice STEM hot path on one MI355X: the HAADF scan workload.
+
+Workload (the headline HAADF configuration): SrTiO3 tiled 8 x 8 x 20 (31.24 x 31.24 x 78.1 Angstrom, 6,400 atoms),
+512 x 512 grid, 40 slices of 1.9525 Angstrom, 200 keV probe with a 20 mrad soft aperture (C10 = C30 = 0), a 64 x 64
+GridScan over one 3.905 Angstrom cell, annular detector 50-200 mrad, complex64 waves, 1024 positions per launch.
+
+One step = one full 64 x 64 scan (4,096 positions x 40 slices = 163,840 position-slices) with the prepared
+potential resident in HBM and the HAADF image left in HBM. `value` = position-slices per second over the K timed
+steps. The roofline entry times the dominant kernel with HIP events on the library's stream; the CPU baseline runs
+the float64 oracle on a bounded sample of positions on the host's cores, and the same sample gives the HIP path's
+error against float64.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=1024)
+    ap.add_argument("--graph", action="store_true", help="replay each batch's slice loop from a hipGraph")
+    ap.add_argument("--generic", action="store_true", help="generic LDS passes instead of the fast passes")
+    ap.add_argument("--cpu-sample", type=int, default=32, help="positions of the float64 CPU baseline sample")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(8, cores), the reference's default")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="first run one device-to-device copy of BYTES (the reference kernel that scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a profile)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        sys.stderr.write("bench.py measures one GPU: this build runs the scan on a single MI355X and has no "
+                         "multi-GPU path (positions would shard with no exchange); run with --gpus 1.\n")
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import simulatokit_amd as sk
+    from simulatokit_amd.workloads import HaadfScan
+
+    if not sk.hip_is_available():
+        raise SystemExit("no HIP device: " + sk._native.last_error())
+    if args.calibrate_copy:
+        src = torch.ones(args.calibrate_copy // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+    w = HaadfScan(batch=args.batch)
+    st = w.structure
+    pot = sk.Potential(st, w.n, w.n, slice_thickness=w.slice_thickness)
+    beam = sk.ElectronBeam(w.energy)
+    probe = sk.Probe(beam, pot.grid, sk.Aperture(w.semiangle))
+    scan = sk.GridScan((0.0, 0.0), (sk.workloads.SRTIO3_A, sk.workloads.SRTIO3_A), w.scan_n, w.scan_n)
+    det = sk.AnnularDetector(w.inner, w.outer)
+    opt = sk.ExecutionOptions(batch=w.batch, use_graph=args.graph, fast_path=not args.generic)
+    S = pot.slice_count
+    npos = w.scan_n * w.scan_n
+
+    t0 = time.perf_counter()
+    prepared = pot.prepare(beam)
+    prepare_ms = (time.perf_counter() - t0) * 1e3
+    out = torch.zeros(npos, dtype=torch.float64, device="cuda:0")
+    positions = scan.positions
+
+    def step():
+        sk.STEM.scan(probe, [prepared], positions, [det], opt, device_outputs=[out.data_ptr()])
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed / args.steps * 1e3
+    value = npos * S * args.steps / elapsed
+
+    # per-kernel device time of one more scan, with HIP events around every pass launch
+    sk.set_timing(True)
+    step()
+    sk.set_timing(False)
+    tm = sk.last_timing()
+    N = w.n * w.n
+    B = min(w.batch, npos)
+    alg_bytes = 16.0 * N * B  # per pass launch: the complex64 wave of every position read and written once
+    col_ms, row_ms = tm["col_ms"], tm["row_ms"]
+    dom = "column" if col_ms >= row_ms else "row"
+    dom_ms = max(col_ms, row_ms)
+    achieved = alg_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
+    haadf = out.cpu().numpy()
+
+    result = {
+        "metric": "probe-positions*slices/sec (HAADF scan) + fraction of HBM roofline at <=1e-4 rel err",
+        "value": value,
+        "unit": "probe-positions*slices/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {"workload": w.name, "grid": [w.n, w.n], "slices": S, "positions": npos,
+                   "batch": B, "detector": f"annular {w.inner:g}-{w.outer:g} mrad", "energy_keV": w.energy / 1e3,
+                   "semiangle_mrad": w.semiangle, "fast_path": not args.generic, "graph": bool(args.graph),
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": f"{dom} pass", "kernel_ms": dom_ms, "row_ms": row_ms, "col_ms": col_ms,
+                     "algorithmic_bytes_per_launch": alg_bytes},
+        "prepare_ms": prepare_ms,
+        "scan_pos_slices": npos * S,
+    }
+
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"], result["rel_err_vs_fp64"] = cpu_baseline(args, w, st, pot, positions, haadf, S)
+
+    print(json.dumps(result), flush=True)
+
+
+def cpu_baseline(args, w, st, pot, positions, haadf, S):
+    """The float64 oracle (the reference's CPU path restated) on the first `cpu_sample` positions, with the
+    reference's threading (min(8, cores) workers, one position each); also the HIP path's error on them."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    cores = os.cpu_count() or 1
+    threads = args.cpu_threads or min(8, cores)
+    k = min(args.cpu_sample, len(positions))
+    V, _ = O.potential(w.n, w.n, st.numbers, st.positions, st.box, pot.slice_thicknesses)
+    ref = O.stem_scan([V], [pot.slice_thicknesses], (st.box[0], st.box[1]), w.energy, w.semiangle, positions[:k],
+                      [O.det_annular(w.inner, w.outer)], threads=threads)[0]
+    secs = O.last_scan_seconds()
+    got = haadf[:k]
+    rel_l2 = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    rel_max = float(np.abs(got - ref).max() / np.abs(ref).max())
+    base = {"value": k * S / secs, "unit": "probe-positions*slices/s", "cores": threads, "kind": "port",
+            "sample": f"{k} of {len(positions)} positions x {S} slices of the same scan (float64 oracle, "
+                      f"transmission functions prepared before the timed loop), {secs:.1f} s on {threads} threads "
+                      f"of {cores} host cores"}
+    return base, {"haadf_rel_l2": rel_l2, "haadf_rel_max": rel_max, "positions": k}
+
+
+if __name__ == "__main__":
+    main()

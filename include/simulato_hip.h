@@ -68,9 +68,13 @@ typedef struct sk_detector {
 /* Execution knobs (ExecutionOptions.swift:21-49). None changes which numbers are computed, only their rounding. */
 typedef struct sk_options {
   int batch;        /* waves resident per launch; 0 = the library's choice for the grid */
-  int use_graph;    /* capture each batch's slice loop in a hipGraph and replay it */
+  int use_graph;    /* capture the slice loop in hipGraphs and replay them */
   int fast_path;    /* 1 (default): register-resident FFT passes where the length has one; 0: generic LDS passes */
-  int reserved[5];
+  int pipeline;     /* scan: 1 = lanes of positions advance one slice per launch, each through its own potential
+                       and slice (potentials of different slice counts share every launch); 0 = one potential at a
+                       time; -1 = the library's choice (lanes when npots > 1 and the grid has fast passes) */
+  int lane_waves;   /* pipeline: positions per lane (0 = the library's choice) */
+  int reserved[3];
 } sk_options;
 
 /* --- availability (MetalEngine.require / Multislice.metalIsAvailable, MetalEngine.swift:63-77,

@@ -36,6 +36,7 @@ def lib():
         L.orc_wavelength.argtypes = [C.c_double]
         L.orc_sigma.restype = C.c_double
         L.orc_sigma.argtypes = [C.c_double]
+        L.orc_last_scan_seconds.restype = C.c_double
         L.orc_configuration_seed.restype = C.c_ulonglong
         L.orc_configuration_seed.argtypes = [C.c_ulonglong, C.c_longlong]
         _lib = L
@@ -300,6 +301,11 @@ def stem_scan(potentials, dz_list, extent, energy, cutoff_mrad, positions, detec
         res.append(out[off:off + c * len(pos)].reshape((len(pos),) + tuple(s)))
         off += c * len(pos)
     return res
+
+
+def last_scan_seconds():
+    """Wall time of the last stem_scan's position loop (after the transmission functions were prepared)."""
+    return lib().orc_last_scan_seconds()
 
 
 # ---- frozen phonons --------------------------------------------------------------------------------------------

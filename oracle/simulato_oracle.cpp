@@ -31,6 +31,7 @@
 #include <string>
 #include <thread>
 #include <vector>
+#include <chrono>
 
 using cd = std::complex<double>;
 
@@ -779,6 +780,7 @@ void intensity(const FFT2& fft, std::vector<cd>& psi, bool values_norm, double* 
 }
 
 thread_local std::string g_err;
+double g_scan_seconds = 0.0;  // wall time of the last scan's position loop (the CPU baseline's timed region)
 
 }  // namespace
 
@@ -1041,6 +1043,7 @@ int orc_stem_scan(int nx, int ny, double ex, double ey, double energy, double cu
   std::fill(out, out + off, 0.0);
   int workers = std::max(1, std::min({threads, (int)std::thread::hardware_concurrency(), npos}));
   std::vector<std::thread> pool;
+  auto t0 = std::chrono::steady_clock::now();
   for (int w = 0; w < workers; ++w)
     pool.emplace_back([&, w] {
       FFT2 fft(nx, ny);
@@ -1064,9 +1067,12 @@ int orc_stem_scan(int nx, int ny, double ex, double ey, double energy, double cu
       }
     });
   for (auto& t : pool) t.join();
+  g_scan_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   for (size_t i = 0; i < off; ++i) out[i] = out[i] / double(npots);
   return 0;
 }
+
+double orc_last_scan_seconds(void) { return g_scan_seconds; }
 
 // GaussianGenerator (FrozenPhonons.swift:14-58): `count` standard normals from `seed`.
 void orc_gaussian_normals(unsigned long long seed, long long count, double* out) {

@@ -200,8 +200,11 @@ class ExecutionOptions:
     computed, only their rounding."""
 
     batch: int = 0          # waves per launch; 0 = the library's choice
-    use_graph: bool = False  # replay each batch's slice loop from a captured hipGraph
+    use_graph: bool = False  # replay the slice loop from captured hipGraphs
     fast_path: bool = True   # register-resident FFT passes where the grid length has one
+    pipeline: int = -1       # scan: 1 lanes of positions each on its own potential and slice per launch, 0 one
+                             # potential at a time, -1 the library's choice (lanes when there are several potentials)
+    lane_waves: int = 0      # positions per lane (0 = the library's choice)
 
 
 @dataclass(frozen=True)

@@ -300,7 +300,7 @@ const int* identity_map(int n) {
 }
 
 void row_pass(const RowArgs& a, int batch, bool fast, hipStream_t s) {
-  if (fast && fast_row_supported(a.ny, a.nc, a.flags)) launch_row_fast(a, batch, s);
+  if (fast && fast_row_supported(a.nx, a.ny, a.flags)) launch_row_fast(a, batch, s);
   else launch_row_generic(a, batch, s);
 }
 
@@ -422,7 +422,11 @@ struct Workspace {
   DevBuf<double> partial, acc, positions, tmp_re, tmp_im, norm, norm_partial;
   DevBuf<int> flag;
   std::vector<DevBuf<double>> pix;
-  DevBuf<unsigned> mask;
+  DevBuf<double> sqx, sqy;
+  DevBuf<int> pos_dev;  // first position of a replayed batch
+  DevBuf<LaneDesc> descs;          // lane pipeline: [steps + 1][lanes]
+  DevBuf<int> step;                // lane pipeline: step counter
+  DevBuf<float2> rx_all, ry_all;   // lane pipeline: ramps of every position
   std::vector<DevBuf<int>> maps;
   std::vector<DevBuf<double>> weights;
 };
@@ -453,6 +457,15 @@ RowArgs row_args(const sk_potential& p, float2* G) {
   r.G = G;
   r.out_scale = 1.0f;
   r.in_full_stride = r.out_full_stride = r.plane_stride = (long long)p.grid.nx * p.grid.ny;
+  // the kept set {iy : |ky| <= kc} is [0, hi] and [lo, ny) (fftfreq is symmetric about 0)
+  int hi = -1;
+  while (hi + 1 < p.nkept && p.keep[hi + 1] == hi + 1) ++hi;
+  const int nneg = p.nkept - (hi + 1);
+  r.kept_hi = hi;
+  r.kept_lo = p.grid.ny - nneg;
+  r.kept_shift = p.grid.ny - p.nkept;
+  for (int c = hi + 1; c < p.nkept; ++c)
+    if (p.keep[c] != r.kept_lo + (c - hi - 1)) fail(SK_ERR_DEVICE, "kept columns are not two ranges");
   return r;
 }
 
@@ -562,6 +575,129 @@ void slice_loop(const sk_potential& p, int batch, float2* G, bool entry_full, fl
       row_pass(r, batch, fast, s);
     }
   }
+}
+
+// The lane pipeline of a scan through several potentials (frozen-phonon configurations of any slice counts).
+// Positions are cut into chunks of W; lane l takes chunks l, l + L, ... and, for each, every potential in turn (the
+// order rotated by l so that the lanes of one launch sit on different potentials). Every step advances each lane by
+// one slice: one row launch, one column launch, one reduction of the lanes that finished an item, one increment of
+// the device step counter. A lane's last slice of an item fuses the detectors with the probe of its next item.
+// All potentials of one chunk run in one lane, in a fixed order, so the float64 accumulation is deterministic.
+void run_pipeline(const sk_potential* const* pots, int npots, int npos, int W, int L, const sk_grid& g,
+                  const DetPlan& dp, bool graph, Timer& tm, sk_timing& timing) {
+  Context& c = ctx();
+  hipStream_t s = c.stream;
+  Workspace& w = ws();
+  const int nx = g.nx, ny = g.ny;
+  const size_t N = (size_t)nx * ny;
+  const sk_potential& p0 = *pots[0];
+  const int B = L * W;
+  struct Item {
+    int pos, count, pot;
+  };
+  std::vector<std::vector<Item>> lane_items(L);
+  const int chunks = (npos + W - 1) / W;
+  for (int q = 0; q < chunks; ++q) {
+    const int lane = q % L;
+    for (int k = 0; k < npots; ++k)
+      lane_items[lane].push_back({q * W, std::min(W, npos - q * W), (k + lane) % npots});
+  }
+  int nsteps = 0;
+  for (auto& li : lane_items) {
+    int t = 0;
+    for (auto& it : li) t += pots[it.pot]->nslices;
+    nsteps = std::max(nsteps, t);
+  }
+  std::vector<LaneDesc> descs((size_t)(nsteps + 1) * L, LaneDesc{});
+  for (int l = 0; l < L; ++l) {
+    const auto& li = lane_items[l];
+    if (li.empty()) continue;
+    LaneDesc& pro = descs[l];
+    pro.mode = LANE_INIT;
+    pro.next_pos = li[0].pos;
+    pro.next_count = li[0].count;
+    int st = 1;
+    for (size_t m = 0; m < li.size(); ++m) {
+      const sk_potential& P = *pots[li[m].pot];
+      for (int j = 0; j < P.nslices; ++j, ++st) {
+        LaneDesc& d = descs[(size_t)st * L + l];
+        d.t = P.t.p + (size_t)j * N;
+        d.P = P.props.p + (size_t)P.prop_of_slice[j] * nx * P.nc;
+        d.pos = li[m].pos;
+        d.count = li[m].count;
+        if (j < P.nslices - 1) {
+          d.mode = LANE_MID;
+        } else if (m + 1 < li.size()) {
+          d.mode = LANE_FINAL_INIT;
+          d.next_pos = li[m + 1].pos;
+          d.next_count = li[m + 1].count;
+        } else {
+          d.mode = LANE_FINAL;
+        }
+      }
+    }
+  }
+  w.descs.upload(descs, s);
+  w.step.ensure(1);
+  SK_HIP(hipMemsetAsync(w.step.p, 0, sizeof(int), s));
+  w.rx_all.ensure((size_t)npos * nx);
+  w.ry_all.ensure((size_t)npos * ny);
+  for (int q0 = 0; q0 < npos; q0 += 32768) {
+    const int cnt = std::min(32768, npos - q0);
+    launch_ramps(nx, ny, g.extent_x / nx, g.extent_y / ny, w.positions.p, q0, nullptr, cnt,
+                 w.rx_all.p + (size_t)q0 * nx, w.ry_all.p + (size_t)q0 * ny, s);
+  }
+  PipeArgs pa{w.descs.p, w.step.p, L, W, w.rx_all.p, w.ry_all.p};
+  RowArgs r = row_args(p0, w.G.p);
+  r.flags = ROW_IN_COMPACT | ROW_INV | ROW_MUL_T | ROW_FWD | ROW_OUT_COMPACT;
+  r.pipe = pa;
+  ColArgs cc = col_args(p0, w.G.p);
+  cc.spec = w.spec.p;
+  cc.det = dp;
+  cc.pipe = pa;
+  auto step_fn = [&](Timer& t) {
+    t.mark_start(s, 0);
+    launch_row_pipe(r, B, s);
+    t.mark_stop(s);
+    t.mark_start(s, 1);
+    launch_col_pipe(cc, B, s);
+    t.mark_stop(s);
+    launch_reduce_pipe(dp, pa, B, w.acc.p, s);
+    launch_step_increment(w.step.p, s);
+  };
+  // prologue: the probe of every lane's first item (descriptor row 0)
+  launch_col_pipe(cc, B, s);
+  launch_step_increment(w.step.p, s);
+  if (!graph) {
+    for (int st = 0; st < nsteps; ++st) {
+      step_fn(tm);
+      if ((st & 31) == 31) tm.harvest(timing);
+    }
+    tm.harvest(timing);
+    return;
+  }
+  // hipGraph: K steps captured once; every replay reads its steps from the device counter
+  const int K = std::min(nsteps, 32);
+  Timer off;
+  hipGraph_t gr;
+  hipGraphExec_t ge = nullptr;
+  SK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  try {
+    for (int k = 0; k < K; ++k) step_fn(off);
+  } catch (...) {
+    hipGraph_t dropped;
+    (void)hipStreamEndCapture(s, &dropped);
+    if (dropped) (void)hipGraphDestroy(dropped);
+    throw;
+  }
+  SK_HIP(hipStreamEndCapture(s, &gr));
+  SK_HIP(hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0));
+  SK_HIP(hipGraphDestroy(gr));
+  int st = 0;
+  for (; st + K <= nsteps; st += K) SK_HIP(hipGraphLaunch(ge, s));
+  for (; st < nsteps; ++st) step_fn(off);
+  SK_HIP(hipStreamSynchronize(s));
+  SK_HIP(hipGraphExecDestroy(ge));
 }
 
 }  // namespace
@@ -897,7 +1033,8 @@ int sk_hip_probes(const sk_grid* grid, const sk_probe* probe, int npos, const do
     DevBuf<double> re((size_t)B * N), im((size_t)B * N);
     for (int b0 = 0; b0 < npos; b0 += B) {
       const int nb = std::min(B, npos - b0);
-      launch_ramps(nx, ny, grid->extent_x / nx, grid->extent_y / ny, w.positions.p, b0, nb, w.rx.p, w.ry.p, s);
+      launch_ramps(nx, ny, grid->extent_x / nx, grid->extent_y / ny, w.positions.p, b0, nullptr, nb, w.rx.p, w.ry.p,
+                   s);
       ColArgs cg{};
       cg.nx = nx;
       cg.ny = ny;
@@ -976,7 +1113,7 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
     DetPlan dp{};
     Workspace& w = ws();
     std::vector<int> det_slot(ndets, -1), det_pix(ndets, -1), det_count(ndets, 0);
-    std::vector<unsigned> mask;
+    std::vector<double> sqx, sqy;
     int nmask = 0, ncom = 0, npix = 0;
     w.maps.clear();
     w.weights.clear();
@@ -991,21 +1128,21 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
       det_count[d] = detector_count(g, lambda, D, nullptr);
       if (!outputs[d]) fail(SK_ERR_INVALID, "null detector output");
       if (D.kind == SK_DET_ANNULAR) {
-        if (dp.nslots >= kMaxSlots || nmask >= 32) fail(SK_ERR_INVALID, "too many annular detectors");
+        if (dp.nslots >= kMaxSlots) fail(SK_ERR_INVALID, "too many detectors");
         const double o = annular_outer(g, lambda, D);
-        if (mask.empty()) mask.assign(N, 0u);
-        auto angx = fftfreq(nx, 1 / ax / double(nx)), angy = fftfreq(ny, 1 / ay / double(ny));
-        const double i2 = D.inner * D.inner, o2 = o * o;
-        for (int ix = 0; ix < nx; ++ix)
-          for (int iy = 0; iy < ny; ++iy) {
-            const double a2x = angx[ix] * angx[ix];
-            const double a2y = angy[iy] * angy[iy];
-            const double a2 = a2x + a2y;
-            if (a2 >= i2 && a2 < o2) mask[(size_t)ix * ny + iy] |= (1u << nmask);
-          }
+        if (sqx.empty()) {
+          // the pixel angles (Detectors.swift:43-44, 124-131): fftfreq(n, 1 / d / n), squared in float64
+          auto angx = fftfreq(nx, 1 / ax / double(nx)), angy = fftfreq(ny, 1 / ay / double(ny));
+          sqx.resize(nx);
+          sqy.resize(ny);
+          for (int ix = 0; ix < nx; ++ix) sqx[ix] = angx[ix] * angx[ix];
+          for (int iy = 0; iy < ny; ++iy) sqy[iy] = angy[iy] * angy[iy];
+        }
         det_slot[d] = dp.nslots;
         dp.slot_kind[dp.nslots] = 0;
         dp.slot_arg[dp.nslots] = nmask;
+        dp.slot_i2[dp.nslots] = D.inner * D.inner;
+        dp.slot_o2[dp.nslots] = o * o;
         dp.nslots++;
         nmask++;
       } else if (D.kind == SK_DET_COM) {
@@ -1063,11 +1200,13 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
       }
     }
     dp.npix = npix;
-    if (!mask.empty()) {
-      w.mask.upload(mask, s);
-      dp.mask = w.mask.p;
+    if (!sqx.empty()) {
+      w.sqx.upload(sqx, s);
+      w.sqy.upload(sqy, s);
+      dp.sqx = w.sqx.p;
+      dp.sqy = w.sqy.p;
     } else {
-      dp.mask = nullptr;
+      dp.sqx = dp.sqy = nullptr;
     }
     dp.scale = double(N) * double(N);  // |FFT2(psi_exit)|^2 of a probe (no 1/N): P' carries 1/N
     // ---- spectrum and positions ----
@@ -1080,6 +1219,25 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
     // ---- batches ----
     const bool fast = !opt || opt->fast_path != 0;
     int B = default_batch(p0, npos, opt);
+    // lane pipeline: lanes of W positions, each on its own potential and slice at every step
+    int pipe = opt ? opt->pipeline : 0;
+    if (pipe < 0) pipe = npots > 1 ? 1 : 0;
+    int W = 0, L = 0;
+    if (pipe) {
+      const int q = pipe_positions_per_block();
+      W = opt && opt->lane_waves > 0 ? opt->lane_waves
+                                     : std::max(q, std::min(128, ((npos + 7) / 8 + q - 1) / q * q));
+      const int chunks = (npos + W - 1) / W;
+      L = std::max(1, std::min(chunks, std::max(B, W) / W));
+      if (!(fast && fits && pipeline_supported(nx, ny, p0.nc, W))) {
+        if (opt && opt->pipeline > 0)
+          fail(SK_ERR_INVALID, "the lane pipeline needs the fast passes (grid lengths 256 or 512), lanes of a "
+                               "multiple of " + std::to_string(q) + " positions and a probe inside the bandlimit");
+        pipe = 0;
+      } else {
+        B = L * W;
+      }
+    }
     w.G.ensure((size_t)B * nx * p0.nc);
     w.rx.ensure((size_t)B * nx);
     w.ry.ensure((size_t)B * ny);
@@ -1097,31 +1255,86 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
     SK_HIP(hipEventCreate(&e0));
     SK_HIP(hipEventCreate(&e1));
     SK_HIP(hipEventRecord(e0, s));
-    std::vector<int> no_planes;
-    for (int k = 0; k < npots; ++k) {
-      const sk_potential& P = *pots[k];
+    // One batch of nb positions starting at b0 (+ *pos_dev when replayed from a graph) through potential P:
+    // ramps, the first column pass (the probe, generated), every slice, the detector reduction.
+    auto enqueue_batch = [&](const sk_potential& P, int b0, const int* pos_dev, int nb, Timer& t) {
       std::vector<int> plane_out(P.nslices, -1);
-      for (int b0 = 0; b0 < npos; b0 += B) {
-        const int nb = std::min(B, npos - b0);
-        launch_ramps(nx, ny, g.extent_x / nx, g.extent_y / ny, w.positions.p, b0, nb, w.rx.p, w.ry.p, s);
-        ColArgs init = col_args(P, fits ? w.G.p : w.full.p);
-        init.spec = w.spec.p;
-        init.rx = w.rx.p;
-        init.ry = w.ry.p;
-        init.flags = COL_GENERATE | COL_INV | COL_OUT;
-        if (!fits) {
-          init.nc = ny;
-          init.keep = identity_map(ny);
-        }
-        col_pass(init, nb, fast && fits, s);
-        ColArgs fin = col_args(P, w.G.p);
-        fin.flags = COL_IN_G | COL_FWD | COL_MUL_P | COL_DETECT;
-        fin.pos0 = b0;
-        fin.det = dp;
-        slice_loop(P, nb, w.G.p, false, w.full.p, fits ? 0 : 1, plane_out, nullptr, &fin, fast, tm);
-        launch_reduce_partials(dp, nb, b0, w.acc.p, s);
-        tm.harvest(timing);
+      launch_ramps(nx, ny, g.extent_x / nx, g.extent_y / ny, w.positions.p, b0, pos_dev, nb, w.rx.p, w.ry.p, s);
+      ColArgs init = col_args(P, fits ? w.G.p : w.full.p);
+      init.spec = w.spec.p;
+      init.rx = w.rx.p;
+      init.ry = w.ry.p;
+      init.flags = COL_GENERATE | COL_INV | COL_OUT;
+      if (!fits) {
+        init.nc = ny;
+        init.keep = identity_map(ny);
       }
+      col_pass(init, nb, fast && fits, s);
+      ColArgs fin = col_args(P, w.G.p);
+      fin.flags = COL_IN_G | COL_FWD | COL_MUL_P | COL_DETECT;
+      fin.pos0 = b0;
+      fin.pos_dev = pos_dev;
+      fin.det = dp;
+      slice_loop(P, nb, w.G.p, false, w.full.p, fits ? 0 : 1, plane_out, nullptr, &fin, fast, t);
+      launch_reduce_partials(dp, nb, b0, pos_dev, w.acc.p, s);
+    };
+    const bool graph = opt && opt->use_graph;
+    if (pipe) {
+      run_pipeline(pots, npots, npos, W, L, g, dp, graph, tm, timing);
+    } else if (!graph) {
+      for (int k = 0; k < npots; ++k)
+        for (int b0 = 0; b0 < npos; b0 += B) {
+          enqueue_batch(*pots[k], b0, nullptr, std::min(B, npos - b0), tm);
+          tm.harvest(timing);
+        }
+    } else {
+      // hipGraph replay: per potential, one graph per batch shape (full, remainder), captured once and replayed for
+      // every batch; the first position of a replay is read from device memory, copied in from pinned host memory
+      // before each launch.
+      line_plan(nx, s);
+      line_plan(ny, s);
+      identity_map(ny);
+      const int nbatches = (npos + B - 1) / B;
+      int* host_first = nullptr;
+      SK_HIP(hipHostMalloc(&host_first, sizeof(int) * nbatches, hipHostMallocDefault));
+      for (int i = 0; i < nbatches; ++i) host_first[i] = i * B;
+      w.pos_dev.ensure(1);
+      Timer off;
+      try {
+        for (int k = 0; k < npots; ++k) {
+          hipGraphExec_t exec[2] = {nullptr, nullptr};
+          for (int i = 0; i < nbatches; ++i) {
+            const int nb = std::min(B, npos - i * B);
+            hipGraphExec_t& ge = exec[nb == B ? 0 : 1];
+            if (!ge) {
+              hipGraph_t gr;
+              SK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+              try {
+                enqueue_batch(*pots[k], 0, w.pos_dev.p, nb, off);
+              } catch (...) {
+                hipGraph_t dropped;
+                (void)hipStreamEndCapture(s, &dropped);
+                if (dropped) (void)hipGraphDestroy(dropped);
+                throw;
+              }
+              SK_HIP(hipStreamEndCapture(s, &gr));
+              SK_HIP(hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0));
+              SK_HIP(hipGraphDestroy(gr));
+            }
+            SK_HIP(hipMemcpyAsync(w.pos_dev.p, host_first + i, sizeof(int), hipMemcpyHostToDevice, s));
+            SK_HIP(hipGraphLaunch(ge, s));
+          }
+          SK_HIP(hipStreamSynchronize(s));
+          for (auto& e : exec)
+            if (e) SK_HIP(hipGraphExecDestroy(e));
+        }
+      } catch (...) {
+        (void)hipStreamSynchronize(s);
+        (void)hipHostFree(host_first);
+        throw;
+      }
+      SK_HIP(hipStreamSynchronize(s));
+      SK_HIP(hipHostFree(host_first));
     }
     SK_HIP(hipEventRecord(e1, s));
     // ---- outputs: averaged over the potentials (STEM.swift:146-150) ----

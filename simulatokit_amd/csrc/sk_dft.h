@@ -1,0 +1,82 @@
+// sk_dft.h — small unscaled DFTs (sign -1 forward, +1 inverse) shared by the generic and the fast passes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "sk_internal.h"
+
+namespace sk {
+
+__device__ inline float2 mul_si(float2 a, int sign) {  // a * (sign * i)
+  return sign < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+
+__device__ inline void dft2(float2& a, float2& b) {
+  float2 t = a;
+  a = cadd(t, b);
+  b = csub(t, b);
+}
+
+__device__ inline void dft4(float2& a0, float2& a1, float2& a2, float2& a3, int sign) {
+  float2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = mul_si(csub(a1, a3), sign);
+  a0 = cadd(t0, t2);
+  a2 = csub(t0, t2);
+  a1 = cadd(t1, t3);
+  a3 = csub(t1, t3);
+}
+
+// DFT of length 8 as two DFT4s (even and odd samples) and the w8^k combination, w8 = e^{sign i pi / 4}
+__device__ inline void dft8(float2 (&u)[8], int sign) {
+  float2 e0 = u[0], e1 = u[2], e2 = u[4], e3 = u[6];
+  float2 o0 = u[1], o1 = u[3], o2 = u[5], o3 = u[7];
+  dft4(e0, e1, e2, e3, sign);
+  dft4(o0, o1, o2, o3, sign);
+  const float h = 0.70710678118654752f;
+  const float s = float(sign);
+  float2 w1 = make_float2((o1.x - s * o1.y) * h, (o1.y + s * o1.x) * h);
+  float2 w2 = mul_si(o2, sign);
+  float2 w3 = make_float2((-o3.x - s * o3.y) * h, (-o3.y + s * o3.x) * h);
+  u[0] = cadd(e0, o0);
+  u[4] = csub(e0, o0);
+  u[1] = cadd(e1, w1);
+  u[5] = csub(e1, w1);
+  u[2] = cadd(e2, w2);
+  u[6] = csub(e2, w2);
+  u[3] = cadd(e3, w3);
+  u[7] = csub(e3, w3);
+}
+
+// One pixel of the final pass: intensity I at (ix, iy) into the scalar slots and the pixelated accumulators
+// (AnnularDetector, CenterOfMassDetector, PixelatedDetector; Detectors.swift:119-205).
+__device__ inline void detect_pixel(const DetPlan& d, int ix, int iy, double I, long long pos,
+                                    double (&s)[kMaxSlots]) {
+  // annular membership: a2 = ax^2 + ay^2 in float64 exactly as the host rounds it (no contraction)
+  const double a2 = d.sqx ? __dadd_rn(d.sqx[ix], d.sqy[iy]) : 0.0;
+  for (int q = 0; q < d.nslots; ++q) {
+    const int kind = d.slot_kind[q], arg = d.slot_arg[q];
+    if (kind == 0) {
+      if (a2 >= d.slot_i2[q] && a2 < d.slot_o2[q]) s[q] += I;
+    } else {
+      const int mx = d.com_mapx[arg][ix], my = d.com_mapy[arg][iy];
+      if (mx >= 0 && my >= 0) s[q] += I * (kind == 1 ? d.com_wx[arg][ix] : d.com_wy[arg][iy]);
+    }
+  }
+  for (int p = 0; p < d.npix; ++p) {
+    const int ox = d.pix_mapx[p][ix], oy = d.pix_mapy[p][iy];
+    if (ox >= 0 && oy >= 0) d.pix_acc[p][pos * d.pix_stride[p] + (long long)ox * d.pix_my[p] + oy] += I;
+  }
+}
+
+template <int R>
+__device__ inline void dft_pow2(float2 (&u)[R], int sign) {
+  if constexpr (R == 2) {
+    dft2(u[0], u[1]);
+  } else if constexpr (R == 4) {
+    dft4(u[0], u[1], u[2], u[3], sign);
+  } else {
+    static_assert(R == 8, "radix 2, 4 or 8");
+    dft8(u, sign);
+  }
+}
+
+}  // namespace sk

@@ -1,14 +1,500 @@
-// sk_fast.hip — register-resident row and column passes for the lengths that have them (filled in below).
+// sk_fast.hip — register-resident row and column passes for lines of 256 and 512 points.
+//
+// Each line (a row along y, or a column along x) is held by T = N / 8 lanes, eight values per lane: lane i holds
+// the elements i + T r, r = 0..7. The line transform is a Stockham autosort FFT (radix 8, 8, 8 for 512; 8, 8, 4 for
+// 256): the first stage works straight from registers, the last stage leaves the result in registers in natural
+// order, and the two exchanges between stages go through LDS. The pointwise work of the slice (the transmission
+// multiply in the row pass, the propagator multiply in the column pass) and the detector reduction of the final
+// pass happen on the registers between the transforms, so every pass reads and writes each stored value once.
+//
+//   row pass:    one wavefront per block and row x; IFFT_y -> * t(x, y) -> FFT_y for PB positions in turn
+//   column pass: 16 compact columns per block;      FFT_x -> * P(kx, ky) -> IFFT_x | detectors, PB positions
+//
+// A block keeps its spatial tile fixed and walks PB positions of the batch, so the lane's twiddle factors and its
+// share of t (row pass) or P (column pass) are loaded into registers once and reused PB times.
+//
+// LDS layouts are chosen so that each group of 16 lanes touches 16 distinct 8-byte bank pairs: a row keeps one pad
+// slot per eight values (index j + j/8); a column tile is stored transposed, [x][16 columns], so the 16 lanes that
+// share an x read 128 contiguous bytes. Twiddles come from the line's float64-derived root table.
+
 #include <hip/hip_runtime.h>
 
+#include "sk_dft.h"
 #include "sk_internal.h"
 
 namespace sk {
+namespace {
 
-bool fast_row_supported(int, int, int) { return false; }
-bool fast_col_supported(int, int, int) { return false; }
-void launch_row_fast(const RowArgs&, int, hipStream_t) { fail(SK_ERR_DEVICE, "no fast row pass"); }
-void launch_col_fast(const ColArgs&, int, hipStream_t) { fail(SK_ERR_DEVICE, "no fast column pass"); }
-int fast_col_blocks_per_wave(int, int) { return 0; }
+// The stage plan of a fast length: radices (R1, R2, R3) with sub-lengths L = 1, R1, R1 R2.
+template <int N>
+struct FastPlan;
+template <>
+struct FastPlan<512> {
+  static constexpr int R1 = 8, R2 = 8, R3 = 8;
+};
+template <>
+struct FastPlan<256> {
+  static constexpr int R1 = 8, R2 = 8, R3 = 4;
+};
+
+// The twiddle factors a lane applies in stages 2 and 3 (stage 1 has L = 1 and none): w[q][j] = w_{L R}^{j k},
+// k = (i + T q) mod L, for butterfly q of the lane. Forward factors; the inverse uses their conjugates.
+template <int N>
+struct LaneTwiddles {
+  static constexpr int T = N / 8;
+  static constexpr int R2 = FastPlan<N>::R2, R3 = FastPlan<N>::R3;
+  static constexpr int L2 = FastPlan<N>::R1, L3 = FastPlan<N>::R1 * FastPlan<N>::R2;
+  float2 w2[8 / R2][R2];
+  float2 w3[8 / R3][R3];
+  __device__ __forceinline__ void load(const float2* __restrict__ tw, int i) {
+#pragma unroll
+    for (int q = 0; q < 8 / R2; ++q) {
+      const int k = (i + T * q) % L2;
+#pragma unroll
+      for (int j = 0; j < R2; ++j) w2[q][j] = tw[j * k * (N / (L2 * R2))];
+    }
+#pragma unroll
+    for (int q = 0; q < 8 / R3; ++q) {
+      const int k = (i + T * q) % L3;
+#pragma unroll
+      for (int j = 0; j < R3; ++j) w3[q][j] = tw[j * k * (N / (L3 * R3))];
+    }
+  }
+};
+
+// One Stockham stage of radix R at sub-length L (see the file comment); W: the lane's factors for this stage
+// (ignored when L = 1). FROM_REG: inputs are the registers; TO_REG: outputs go to the registers.
+template <int N, int R, int L, int SIGN, bool FROM_REG, bool TO_REG, class Idx>
+__device__ __forceinline__ void stage(float2 (&v)[8], float2* __restrict__ buf, int i, const float2 (*W)[R], Idx idx) {
+  constexpr int T = N / 8;
+  constexpr int NB = 8 / R;
+  constexpr int STRIDE = N / R;
+  float2 u[NB][R];
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int bi = i + T * q;
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      if constexpr (FROM_REG) u[q][j] = v[q + NB * j];
+      else u[q][j] = buf[idx(bi + STRIDE * j)];
+    }
+  }
+  if constexpr (!FROM_REG) __syncthreads();  // every lane has read the line before any lane overwrites it
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int bi = i + T * q;
+    const int k = (L == 1) ? 0 : (bi % L);
+    if constexpr (L > 1) {
+#pragma unroll
+      for (int j = 1; j < R; ++j) u[q][j] = SIGN < 0 ? cmul(u[q][j], W[q][j]) : cmulc(u[q][j], W[q][j]);
+    }
+    dft_pow2<R>(u[q], SIGN);
+    if constexpr (TO_REG) {
+#pragma unroll
+      for (int j = 0; j < R; ++j) v[q + NB * j] = u[q][j];
+    } else {
+      const int base = (bi - k) * R + k;
+#pragma unroll
+      for (int j = 0; j < R; ++j) buf[idx(base + j * L)] = u[q][j];
+    }
+  }
+  if constexpr (!TO_REG) __syncthreads();
+}
+
+// Unscaled transform of the line in registers (SIGN -1 forward, +1 inverse).
+template <int N, int SIGN, class Idx>
+__device__ __forceinline__ void fft_reg(float2 (&v)[8], float2* buf, int i, const LaneTwiddles<N>& w, Idx idx) {
+  using P = FastPlan<N>;
+  stage<N, P::R1, 1, SIGN, true, false>(v, buf, i, (const float2(*)[P::R1]) nullptr, idx);
+  stage<N, P::R2, P::R1, SIGN, false, false>(v, buf, i, w.w2, idx);
+  stage<N, P::R3, P::R1 * P::R2, SIGN, false, true>(v, buf, i, w.w3, idx);
+}
+
+constexpr int kColTile = 16;  // compact columns per column-pass block
+
+constexpr int ROW_MID = ROW_IN_COMPACT | ROW_INV | ROW_MUL_T | ROW_FWD | ROW_OUT_COMPACT;
+constexpr int ROW_MID_PLANE = ROW_MID | ROW_WRITE_PLANE;
+constexpr int ROW_ENTRY = ROW_IN_FULL | ROW_MUL_T | ROW_FWD | ROW_OUT_COMPACT;
+constexpr int ROW_FROM_FULL = ROW_IN_FULL | ROW_INV | ROW_MUL_T | ROW_FWD | ROW_OUT_COMPACT;
+constexpr int ROW_EXIT = ROW_IN_COMPACT | ROW_INV | ROW_WRITE_PLANE;
+
+constexpr int COL_MID = COL_IN_G | COL_FWD | COL_MUL_P | COL_INV | COL_OUT;
+constexpr int COL_INIT = COL_GENERATE | COL_INV | COL_OUT;
+constexpr int COL_FINAL = COL_IN_G | COL_FWD | COL_MUL_P | COL_DETECT;
+
+// compact column of iy, or -1 for a pruned ky
+__device__ __forceinline__ int compact_of(const RowArgs& a, int iy) {
+  return iy <= a.kept_hi ? iy : (iy >= a.kept_lo ? iy - a.kept_shift : -1);
+}
+
+// Row pass: block = one wavefront = 64 / T rows x of positions [b0, b0 + PB). PIPE: the positions belong to one lane
+// of the pipeline, whose descriptor for the current step gives the transmission function and the position count.
+template <int N, int FLAGS, int PB, bool PIPE = false>
+__global__ void __launch_bounds__(64) row_fast_kernel(RowArgs a, const float2* __restrict__ tw, int batch) {
+  constexpr int T = N / 8, ROWS = 64 / T, LD = N + N / 8;
+  __shared__ float2 lds[ROWS * LD];
+  const int tid = threadIdx.x, row = tid / T, i = tid - row * T;
+  const int x = blockIdx.x * ROWS + row;  // the host launches nx / ROWS blocks along x: every x is in range
+  const int b0 = blockIdx.y * PB;
+  const float2* tsrc = a.t;
+  long long limit = batch;
+  if constexpr (PIPE) {
+    const int lane = b0 / a.pipe.lane_waves;  // lane_waves is a multiple of PB: the block stays in one lane
+    const LaneDesc d = a.pipe.descs[(long long)(*a.pipe.step) * a.pipe.lanes + lane];
+    if (d.t == nullptr) return;  // idle lane (uniform over the block)
+    tsrc = d.t;
+    limit = (long long)lane * a.pipe.lane_waves + d.count;
+  }
+  float2* buf = lds + row * LD;
+  auto idx = [](int j) { return j + (j >> 3); };
+  const long long gw = (long long)a.nx * a.nc;
+  LaneTwiddles<N> w;
+  w.load(tw, i);
+  int cidx[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) cidx[r] = compact_of(a, i + T * r);
+  float2 t[8];
+  if constexpr ((FLAGS & ROW_MUL_T) != 0) {
+    const float2* tr = tsrc + (long long)x * N;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) t[r] = tr[i + T * r];
+  }
+  for (int p = 0; p < PB; ++p) {
+    const long long b = b0 + p;
+    if (b >= limit) break;  // uniform over the block
+    float2 v[8];
+    if constexpr ((FLAGS & ROW_IN_COMPACT) != 0) {
+      const float2* g = a.G + b * gw + (long long)x * a.nc;
+#pragma unroll
+      for (int r = 0; r < 8; ++r) v[r] = cidx[r] >= 0 ? g[cidx[r]] : make_float2(0.f, 0.f);
+    } else {
+      const float2* g = a.in_full + b * a.in_full_stride + (long long)x * N;
+#pragma unroll
+      for (int r = 0; r < 8; ++r) v[r] = g[i + T * r];
+    }
+    if constexpr ((FLAGS & ROW_INV) != 0) fft_reg<N, +1>(v, buf, i, w, idx);
+    if constexpr ((FLAGS & ROW_WRITE_PLANE) != 0) {
+      float2* pl = a.plane + b * a.plane_stride + (long long)x * N;
+#pragma unroll
+      for (int r = 0; r < 8; ++r) pl[i + T * r] = cscale(v[r], a.out_scale);
+    }
+    if constexpr ((FLAGS & ROW_MUL_T) != 0) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) v[r] = cmul(v[r], t[r]);
+    }
+    if constexpr ((FLAGS & ROW_FWD) != 0) fft_reg<N, -1>(v, buf, i, w, idx);
+    if constexpr ((FLAGS & ROW_OUT_COMPACT) != 0) {
+      float2* g = a.G + b * gw + (long long)x * a.nc;
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+        if (cidx[r] >= 0) g[cidx[r]] = v[r];
+    }
+  }
+}
+
+// The detectors of one wave's column tile (final pass): |Psi(k)|^2 of the lane's eight pixels (zero on a padding
+// column) into the pixelated accumulators of position `pos`, and one scalar sum per slot, reduced over the block
+// deterministically (butterflies within each wavefront, then the wavefronts in order) into the partial sums of wave
+// `b`. Ends with a barrier, so the LDS is free again.
+template <int N, int NT>
+__device__ __forceinline__ void detect_block(const float2 (&v)[8], const DetPlan& d, long long pos, long long b, int iy,
+                                             int i, int tid, float2* lds) {
+  constexpr int T = N / 8;
+  double I[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) I[r] = iy >= 0 ? ((double)v[r].x * v[r].x + (double)v[r].y * v[r].y) * d.scale : 0.0;
+  if (iy >= 0) {
+    for (int q = 0; q < d.npix; ++q) {
+      const int oy = d.pix_mapy[q][iy];
+      if (oy < 0) continue;
+      double* acc = d.pix_acc[q] + pos * d.pix_stride[q] + oy;
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const int ox = d.pix_mapx[q][i + T * r];
+        if (ox >= 0) acc[(long long)ox * d.pix_my[q]] += I[r];
+      }
+    }
+  }
+  double* red = reinterpret_cast<double*>(lds);
+  const int lane = tid & 63, wave = tid >> 6;
+  const double sy = (iy >= 0 && d.sqy) ? d.sqy[iy] : 0.0;
+  for (int q = 0; q < d.nslots; ++q) {
+    const int kind = d.slot_kind[q], arg = d.slot_arg[q];
+    double val = 0.0;
+    if (iy >= 0) {
+      if (kind == 0) {
+        const double i2 = d.slot_i2[q], o2 = d.slot_o2[q];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          const double a2 = __dadd_rn(d.sqx[i + T * r], sy);  // float64, as the host rounds it
+          if (a2 >= i2 && a2 < o2) val += I[r];
+        }
+      } else if (d.com_mapy[arg][iy] >= 0) {
+        const double wy = d.com_wy[arg][iy];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          const int x = i + T * r;
+          if (d.com_mapx[arg][x] >= 0) val += I[r] * (kind == 1 ? d.com_wx[arg][x] : wy);
+        }
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) val += __shfl_xor(val, off);
+    if (lane == 0) red[wave * kMaxSlots + q] = val;
+  }
+  __syncthreads();
+  if (tid < d.nslots) {
+    double t = 0.0;
+    for (int wv = 0; wv < NT / 64; ++wv) t += red[wv * kMaxSlots + tid];
+    d.partial[(b * d.nslots + tid) * d.blocks_per_wave + blockIdx.x] = t;
+  }
+  __syncthreads();  // `red` is read before the next transform reuses the LDS
+}
+
+// Column pass: block = 16 compact columns (lanes c = tid mod 16, i = tid / 16) of positions [b0, b0 + PB).
+template <int N, int FLAGS, int PB>
+__global__ void __launch_bounds__(kColTile * (N / 8)) col_fast_kernel(ColArgs a, const float2* __restrict__ tw,
+                                                                        int batch) {
+  constexpr int T = N / 8, NT = kColTile * T;
+  __shared__ float2 lds[N * kColTile];  // [x][column]
+  const int tid = threadIdx.x, cl = tid % kColTile, i = tid / kColTile;
+  const int c = blockIdx.x * kColTile + cl;  // the host launches nc / 16 blocks along x: every c is in range
+  const int b0 = blockIdx.y * PB;
+  const int iy = a.keep[c];
+  auto idx = [cl](int j) { return j * kColTile + cl; };
+  const long long gw = (long long)a.nx * a.nc;
+  LaneTwiddles<N> w;
+  w.load(tw, i);
+  float2 pv[8];
+  if constexpr ((FLAGS & COL_MUL_P) != 0) {
+    const float2* P = a.P + c;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) pv[r] = P[(long long)(i + T * r) * a.nc];
+  }
+  for (int p = 0; p < PB; ++p) {
+    const long long b = b0 + p;
+    if (b >= batch) break;  // uniform over the block
+    float2 v[8];
+    if constexpr ((FLAGS & COL_IN_G) != 0) {
+      const float2* g = a.G + b * gw + c;
+#pragma unroll
+      for (int r = 0; r < 8; ++r) v[r] = iy >= 0 ? g[(long long)(i + T * r) * a.nc] : make_float2(0.f, 0.f);
+    } else {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const int x = i + T * r;
+        v[r] = iy >= 0 ? cmul(cmul(a.spec[(long long)x * a.ny + iy], a.rx[b * a.nx + x]), a.ry[b * a.ny + iy])
+                       : make_float2(0.f, 0.f);
+      }
+    }
+    if constexpr ((FLAGS & COL_FWD) != 0) fft_reg<N, -1>(v, lds, i, w, idx);
+    if constexpr ((FLAGS & COL_MUL_P) != 0) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) v[r] = cmul(v[r], pv[r]);
+    }
+    if constexpr ((FLAGS & COL_DETECT) != 0) {
+      detect_block<N, NT>(v, a.det, first_position(a.pos0, a.pos_dev) + b, b, iy, i, tid, lds);
+      continue;
+    }
+    if constexpr ((FLAGS & COL_INV) != 0) fft_reg<N, +1>(v, lds, i, w, idx);
+    if constexpr ((FLAGS & COL_OUT) != 0) {
+      float2* g = a.G + b * gw + c;
+#pragma unroll
+      for (int r = 0; r < 8; ++r) g[(long long)(i + T * r) * a.nc] = v[r];
+    }
+  }
+}
+
+constexpr int kPipePB = 8;  // positions per block in pipelined launches; lane_waves is a multiple of it
+
+// Pipelined column pass: the lane of the block's positions gives the mode of this step —
+//   MID:        FFT_x, * P, IFFT_x                       (a slice inside the item)
+//   FINAL:      FFT_x, * P, detectors                    (the item's last slice; the lane then idles)
+//   FINAL_INIT: FFT_x, * P, detectors, then the probe of the lane's next item generated into the same storage
+//   INIT:       the probe of the lane's first item
+template <int N>
+__global__ void __launch_bounds__(kColTile * (N / 8)) col_pipe_kernel(ColArgs a, const float2* __restrict__ tw) {
+  constexpr int T = N / 8, NT = kColTile * T;
+  __shared__ float2 lds[N * kColTile];  // [x][column]
+  const int tid = threadIdx.x, cl = tid % kColTile, i = tid / kColTile;
+  const int c = blockIdx.x * kColTile + cl;
+  const int b0 = blockIdx.y * kPipePB;
+  const int lane = b0 / a.pipe.lane_waves, local0 = b0 - lane * a.pipe.lane_waves;
+  const LaneDesc d = a.pipe.descs[(long long)(*a.pipe.step) * a.pipe.lanes + lane];
+  if (d.mode == LANE_IDLE) return;  // uniform over the block
+  const int iy = a.keep[c];
+  auto idx = [cl](int j) { return j * kColTile + cl; };
+  const long long gw = (long long)a.nx * a.nc;
+  LaneTwiddles<N> w;
+  w.load(tw, i);
+  float2 pv[8];
+  if (d.mode != LANE_INIT) {
+    const float2* P = d.P + c;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) pv[r] = P[(long long)(i + T * r) * a.nc];
+  }
+  for (int p = 0; p < kPipePB; ++p) {
+    const int local = local0 + p;
+    const long long b = b0 + p;
+    float2* g = a.G + b * gw + c;
+    float2 v[8];
+    if (d.mode != LANE_INIT && local < d.count) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) v[r] = iy >= 0 ? g[(long long)(i + T * r) * a.nc] : make_float2(0.f, 0.f);
+      fft_reg<N, -1>(v, lds, i, w, idx);
+#pragma unroll
+      for (int r = 0; r < 8; ++r) v[r] = cmul(v[r], pv[r]);
+      if (d.mode == LANE_MID) {
+        fft_reg<N, +1>(v, lds, i, w, idx);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) g[(long long)(i + T * r) * a.nc] = v[r];
+        continue;
+      }
+      detect_block<N, NT>(v, a.det, (long long)d.pos + local, b, iy, i, tid, lds);
+    }
+    if ((d.mode == LANE_FINAL_INIT || d.mode == LANE_INIT) && local < d.next_count) {
+      const long long q = (long long)d.next_pos + local;
+      const float2* rx = a.pipe.rx_all + q * a.nx;
+      const float2 ryv = iy >= 0 ? a.pipe.ry_all[q * a.ny + iy] : make_float2(0.f, 0.f);
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const int x = i + T * r;
+        v[r] = iy >= 0 ? cmul(cmul(a.spec[(long long)x * a.ny + iy], rx[x]), ryv) : make_float2(0.f, 0.f);
+      }
+      fft_reg<N, +1>(v, lds, i, w, idx);
+#pragma unroll
+      for (int r = 0; r < 8; ++r) g[(long long)(i + T * r) * a.nc] = v[r];
+    }
+  }
+}
+
+// Sum the partials of every wave whose lane finished an item this step into its position's accumulators.
+__global__ void reduce_pipe_kernel(DetPlan d, PipeArgs pa, double* acc) {
+  const long long b = blockIdx.x;
+  const int lane = int(b / pa.lane_waves), local = int(b - (long long)lane * pa.lane_waves);
+  const LaneDesc ld = pa.descs[(long long)(*pa.step) * pa.lanes + lane];
+  if (!(ld.mode == LANE_FINAL || ld.mode == LANE_FINAL_INIT) || local >= ld.count) return;
+  const long long pos = (long long)ld.pos + local;
+  for (int q = threadIdx.x; q < d.nslots; q += blockDim.x) {
+    const double* p = d.partial + (b * d.nslots + q) * d.blocks_per_wave;
+    double s = 0.0;
+    for (int k = 0; k < d.blocks_per_wave; ++k) s += p[k];
+    acc[pos * d.nslots + q] += s;
+  }
+}
+
+__global__ void step_increment_kernel(int* step) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
+}
+
+bool is_fast_length(int n) { return n == 256 || n == 512; }
+
+constexpr int kRowPB = 8;  // positions per row-pass block
+constexpr int kColPB = 4;  // positions per column-pass block
+
+template <int N, int F>
+void row_go(const RowArgs& a, int batch, hipStream_t s, const float2* tw) {
+  constexpr int ROWS = 64 / (N / 8);
+  dim3 grid(a.nx / ROWS, (batch + kRowPB - 1) / kRowPB);
+  hipLaunchKernelGGL((row_fast_kernel<N, F, kRowPB>), grid, dim3(64), 0, s, a, tw, batch);
+}
+
+template <int N>
+void row_launch(const RowArgs& a, int batch, hipStream_t s, const float2* tw) {
+  switch (a.flags) {
+    case ROW_MID: row_go<N, ROW_MID>(a, batch, s, tw); break;
+    case ROW_MID_PLANE: row_go<N, ROW_MID_PLANE>(a, batch, s, tw); break;
+    case ROW_ENTRY: row_go<N, ROW_ENTRY>(a, batch, s, tw); break;
+    case ROW_FROM_FULL: row_go<N, ROW_FROM_FULL>(a, batch, s, tw); break;
+    default: row_go<N, ROW_EXIT>(a, batch, s, tw); break;
+  }
+  SK_HIP(hipGetLastError());
+}
+
+template <int N, int F>
+void col_go(const ColArgs& a, int batch, hipStream_t s, const float2* tw) {
+  constexpr int NT = kColTile * (N / 8);
+  dim3 grid(a.nc / kColTile, (batch + kColPB - 1) / kColPB);
+  hipLaunchKernelGGL((col_fast_kernel<N, F, kColPB>), grid, dim3(NT), 0, s, a, tw, batch);
+}
+
+template <int N>
+void col_launch(const ColArgs& a, int batch, hipStream_t s, const float2* tw) {
+  switch (a.flags) {
+    case COL_MID: col_go<N, COL_MID>(a, batch, s, tw); break;
+    case COL_INIT: col_go<N, COL_INIT>(a, batch, s, tw); break;
+    default: col_go<N, COL_FINAL>(a, batch, s, tw); break;
+  }
+  SK_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+bool fast_row_supported(int nx, int ny, int flags) {
+  if (!is_fast_length(ny)) return false;
+  const int rows = 64 / (ny / 8);
+  if (nx % rows != 0) return false;
+  return flags == ROW_MID || flags == ROW_MID_PLANE || flags == ROW_ENTRY || flags == ROW_FROM_FULL ||
+         flags == ROW_EXIT;
+}
+
+bool fast_col_supported(int nx, int nc, int flags) {
+  if (!is_fast_length(nx) || nc % kColTile != 0) return false;
+  return flags == COL_MID || flags == COL_INIT || flags == COL_FINAL;
+}
+
+int fast_col_blocks_per_wave(int, int nc) { return nc / kColTile; }
+
+void launch_row_fast(const RowArgs& a, int batch, hipStream_t s) {
+  const float2* tw = line_plan(a.ny, s).dev.roots;
+  if (a.ny == 512) row_launch<512>(a, batch, s, tw);
+  else row_launch<256>(a, batch, s, tw);
+}
+
+void launch_col_fast(const ColArgs& a, int batch, hipStream_t s) {
+  const float2* tw = line_plan(a.nx, s).dev.roots;
+  if (a.nx == 512) col_launch<512>(a, batch, s, tw);
+  else col_launch<256>(a, batch, s, tw);
+}
+
+// ---- lane pipeline launches --------------------------------------------------------------------------------------
+bool pipeline_supported(int nx, int ny, int nc, int lane_waves) {
+  return is_fast_length(nx) && is_fast_length(ny) && nc % kColTile == 0 && nx % (64 / (ny / 8)) == 0 &&
+         lane_waves > 0 && lane_waves % kPipePB == 0;
+}
+
+int pipe_positions_per_block() { return kPipePB; }
+
+void launch_row_pipe(const RowArgs& a, int batch, hipStream_t s) {
+  const float2* tw = line_plan(a.ny, s).dev.roots;
+  constexpr int F = ROW_MID;
+  if (a.ny == 512) {
+    dim3 grid(a.nx / 1, batch / kPipePB);
+    hipLaunchKernelGGL((row_fast_kernel<512, F, kPipePB, true>), grid, dim3(64), 0, s, a, tw, batch);
+  } else {
+    dim3 grid(a.nx / 2, batch / kPipePB);
+    hipLaunchKernelGGL((row_fast_kernel<256, F, kPipePB, true>), grid, dim3(64), 0, s, a, tw, batch);
+  }
+  SK_HIP(hipGetLastError());
+}
+
+void launch_col_pipe(const ColArgs& a, int batch, hipStream_t s) {
+  const float2* tw = line_plan(a.nx, s).dev.roots;
+  dim3 grid(a.nc / kColTile, batch / kPipePB);
+  if (a.nx == 512) hipLaunchKernelGGL((col_pipe_kernel<512>), grid, dim3(kColTile * 64), 0, s, a, tw);
+  else hipLaunchKernelGGL((col_pipe_kernel<256>), grid, dim3(kColTile * 32), 0, s, a, tw);
+  SK_HIP(hipGetLastError());
+}
+
+void launch_reduce_pipe(const DetPlan& d, const PipeArgs& p, int batch, double* acc, hipStream_t s) {
+  if (d.nslots == 0) return;
+  hipLaunchKernelGGL(reduce_pipe_kernel, dim3(batch), dim3(64), 0, s, d, p, acc);
+  SK_HIP(hipGetLastError());
+}
+
+void launch_step_increment(int* step, hipStream_t s) {
+  hipLaunchKernelGGL(step_increment_kernel, dim3(1), dim3(64), 0, s, step);
+  SK_HIP(hipGetLastError());
+}
 
 }  // namespace sk

@@ -93,6 +93,31 @@ struct LinePlan {
 // Returns the shared plan of length n (built once per process and device).
 const LinePlan& line_plan(int n, hipStream_t s);
 
+// ---- lane pipeline ----------------------------------------------------------------------------------------------
+// The scan's waves are split into lanes of lane_waves positions. At every step each lane advances one slice of its
+// current item (a chunk of positions through one potential), so one launch serves lanes on different potentials and
+// slices. The host writes one LaneDesc per lane per step; the kernels read row `*step` of the table.
+enum LaneMode : int { LANE_IDLE = 0, LANE_MID = 1, LANE_FINAL = 2, LANE_FINAL_INIT = 3, LANE_INIT = 4 };
+
+struct LaneDesc {
+  const float2* t;  // row pass: transmission of (potential, slice); null: the lane is idle
+  const float2* P;  // column pass: propagator of (potential, slice)
+  int mode;         // column pass: LaneMode
+  int pos;          // first position of the current item (detector accumulators)
+  int count;        // positions of the current item (<= lane_waves)
+  int next_pos;     // FINAL_INIT / INIT: first position of the lane's next item
+  int next_count;   // its positions
+  int pad;
+};
+
+struct PipeArgs {
+  const LaneDesc* descs;  // [steps][lanes]; null: not a pipelined launch
+  const int* step;        // device step counter
+  int lanes, lane_waves;
+  const float2* rx_all;   // [npos][nx] shift ramps of every position
+  const float2* ry_all;   // [npos][ny]
+};
+
 // ---- pass descriptions --------------------------------------------------------------------------------------------
 // Row pass (along y, contiguous): flags
 enum RowFlags : int {
@@ -118,6 +143,9 @@ struct RowArgs {
   const float2* t;        // [nx][ny]
   float out_scale;        // applied to full / plane outputs
   long long in_full_stride, out_full_stride, plane_stride;
+  // the kept set as two ranges: iy in [0, kept_hi] -> c = iy; iy in [kept_lo, ny) -> c = iy - kept_shift
+  int kept_hi, kept_lo, kept_shift;
+  PipeArgs pipe;          // lane pipeline (fast passes only); pipe.descs null otherwise
 };
 
 enum ColFlags : int {
@@ -137,9 +165,12 @@ constexpr int kMaxCom = 4;
 // Detector reduction plan of the final column pass.
 struct DetPlan {
   int nslots;                   // scalar sums per wave
-  int slot_kind[kMaxSlots];     // 0: annular mask bit, 1: COM x, 2: COM y
-  int slot_arg[kMaxSlots];      // mask bit or COM index
-  const unsigned* mask;         // [nx][ny] annular membership bits
+  int slot_kind[kMaxSlots];     // 0: annular, 1: COM x, 2: COM y
+  int slot_arg[kMaxSlots];      // COM index (COM slots)
+  double slot_i2[kMaxSlots];    // annular: inner^2, outer^2 [mrad^2], half-open inner^2 <= a2 < outer^2
+  double slot_o2[kMaxSlots];
+  const double* sqx;            // [nx] angle_x^2 of the pattern's pixels [mrad^2] (float64, host-rounded)
+  const double* sqy;            // [ny]
   const int* com_mapx[kMaxCom];   // [nx] crop output index or -1
   const int* com_mapy[kMaxCom];   // [ny]
   const double* com_wx[kMaxCom];  // [nx] angle of the crop row (mrad or 1/A)
@@ -165,21 +196,35 @@ struct ColArgs {
   const float2* rx;       // [batch][nx]
   const float2* ry;       // [batch][ny]
   int pos0;               // position index of wave 0 (detectors)
+  const int* pos_dev;     // optional device offset added to pos0 (a replayed graph reads its batch here)
   DetPlan det;
+  PipeArgs pipe;          // lane pipeline (fast passes only); pipe.descs null otherwise
 };
+
+__device__ __forceinline__ long long first_position(int pos0, const int* pos_dev) {
+  return (long long)pos0 + (pos_dev ? *pos_dev : 0);
+}
 
 // launchers (sk_passes.hip)
 void launch_row_generic(const RowArgs& a, int batch, hipStream_t s);
 void launch_col_generic(const ColArgs& a, int batch, hipStream_t s);
-void launch_reduce_partials(const DetPlan& d, int batch, int pos0, double* acc, hipStream_t s);
+void launch_reduce_partials(const DetPlan& d, int batch, int pos0, const int* pos_dev, double* acc, hipStream_t s);
 int col_generic_lines(int m);  // columns per block of the generic column pass
 
 // fast passes (sk_fast.hip): return false when the length has no fast kernel
-bool fast_row_supported(int ny, int nc, int flags);
+bool fast_row_supported(int nx, int ny, int flags);
 bool fast_col_supported(int nx, int nc, int flags);
 void launch_row_fast(const RowArgs& a, int batch, hipStream_t s);
 void launch_col_fast(const ColArgs& a, int batch, hipStream_t s);
 int fast_col_blocks_per_wave(int nx, int nc);
+// lane pipeline launches (sk_fast.hip): the row pass and the column pass of one step, the partial-sum reduction of
+// the lanes that finished an item, and the step counter's increment
+bool pipeline_supported(int nx, int ny, int nc, int lane_waves);
+void launch_row_pipe(const RowArgs& a, int batch, hipStream_t s);
+void launch_col_pipe(const ColArgs& a, int batch, hipStream_t s);
+void launch_reduce_pipe(const DetPlan& d, const PipeArgs& p, int batch, double* acc, hipStream_t s);
+void launch_step_increment(int* step, hipStream_t s);
+int pipe_positions_per_block();
 
 // elementwise kernels (sk_prepare.hip)
 void launch_phase_to_complex(const double* V, double sigma, float2* out, size_t n, hipStream_t s);
@@ -195,7 +240,7 @@ void launch_sum_squares(const double* re, const double* im, size_t n, double* pa
                         hipStream_t s);
 void launch_normalise_spectrum(const double* re, const double* im, const double* norm2, double extra,
                                size_t n, float2* out, hipStream_t s);
-void launch_ramps(int nx, int ny, double sx, double sy, const double* positions, int pos0, int count,
+void launch_ramps(int nx, int ny, double sx, double sy, const double* positions, int pos0, const int* pos_dev, int count,
                   float2* rx, float2* ry, hipStream_t s);
 void launch_scatter_deltas(const int* idx, const double* w, size_t n, float2* out, hipStream_t s);
 void launch_lobato_kernels(int nx, int ny, double ex, double ey, const int* zs, int nz, int pi_mode,

@@ -17,6 +17,7 @@
 #include <complex>
 #include <map>
 
+#include "sk_dft.h"
 #include "sk_internal.h"
 
 namespace sk {
@@ -24,49 +25,10 @@ namespace sk {
 // ---------------------------------------------------------------------------------------------------------------
 // Small DFTs (sign -1 forward, +1 inverse, unscaled)
 // ---------------------------------------------------------------------------------------------------------------
-__device__ inline float2 mul_si(float2 a, int sign) {  // a * (sign * i)
-  return sign < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
-}
-
-__device__ inline void dft2(float2& a, float2& b) {
-  float2 t = a;
-  a = cadd(t, b);
-  b = csub(t, b);
-}
-
-__device__ inline void dft4(float2& a0, float2& a1, float2& a2, float2& a3, int sign) {
-  float2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = mul_si(csub(a1, a3), sign);
-  a0 = cadd(t0, t2);
-  a2 = csub(t0, t2);
-  a1 = cadd(t1, t3);
-  a3 = csub(t1, t3);
-}
-
 template <int R>
 __device__ inline void dft_small(float2 (&u)[R], int sign, const float2* roots, int m) {
-  if constexpr (R == 2) {
-    dft2(u[0], u[1]);
-  } else if constexpr (R == 4) {
-    dft4(u[0], u[1], u[2], u[3], sign);
-  } else if constexpr (R == 8) {
-    float2 e0 = u[0], e1 = u[2], e2 = u[4], e3 = u[6];
-    float2 o0 = u[1], o1 = u[3], o2 = u[5], o3 = u[7];
-    dft4(e0, e1, e2, e3, sign);
-    dft4(o0, o1, o2, o3, sign);
-    const float h = 0.70710678118654752f;
-    const float s = float(sign);
-    // o_k * w8^k, w8 = e^{sign i pi / 4}
-    float2 w1 = make_float2((o1.x - s * o1.y) * h, (o1.y + s * o1.x) * h);
-    float2 w2 = mul_si(o2, sign);
-    float2 w3 = make_float2((-o3.x - s * o3.y) * h, (-o3.y + s * o3.x) * h);
-    u[0] = cadd(e0, o0);
-    u[4] = csub(e0, o0);
-    u[1] = cadd(e1, w1);
-    u[5] = csub(e1, w1);
-    u[2] = cadd(e2, w2);
-    u[6] = csub(e2, w2);
-    u[3] = cadd(e3, w3);
-    u[7] = csub(e3, w3);
+  if constexpr (R == 2 || R == 4 || R == 8) {
+    dft_pow2<R>(u, sign);
   } else {
     // direct DFT of an odd prime radix with the line's roots: e^{-2 pi i j q / R} = roots[(j q mod R) m / R]
     float2 v[R];
@@ -268,24 +230,6 @@ __device__ inline void block_reduce_slots(double (&s)[kMaxSlots], int nslots, do
   }
 }
 
-__device__ inline void detect_pixel(const DetPlan& d, int ix, int iy, int ny, double I, long long pos,
-                                    double (&s)[kMaxSlots]) {
-  unsigned bits = d.mask ? d.mask[(long long)ix * ny + iy] : 0u;
-  for (int q = 0; q < d.nslots; ++q) {
-    const int kind = d.slot_kind[q], arg = d.slot_arg[q];
-    if (kind == 0) {
-      if ((bits >> arg) & 1u) s[q] += I;
-    } else {
-      const int mx = d.com_mapx[arg][ix], my = d.com_mapy[arg][iy];
-      if (mx >= 0 && my >= 0) s[q] += I * (kind == 1 ? d.com_wx[arg][ix] : d.com_wy[arg][iy]);
-    }
-  }
-  for (int p = 0; p < d.npix; ++p) {
-    const int ox = d.pix_mapx[p][ix], oy = d.pix_mapy[p][iy];
-    if (ox >= 0 && oy >= 0) d.pix_acc[p][pos * d.pix_stride[p] + (long long)ox * d.pix_my[p] + oy] += I;
-  }
-}
-
 // ---------------------------------------------------------------------------------------------------------------
 // Generic column pass: block = `cols` compact columns (lines along x) of one wave.
 // ---------------------------------------------------------------------------------------------------------------
@@ -332,7 +276,7 @@ __global__ void __launch_bounds__(kGenericThreads) col_generic_kernel(ColArgs a,
     double s[kMaxSlots];
 #pragma unroll
     for (int q = 0; q < kMaxSlots; ++q) s[q] = 0.0;
-    const long long pos = a.pos0 + b;
+    const long long pos = first_position(a.pos0, a.pos_dev) + b;
     for (int idx = tid; idx < cols * a.nx; idx += T) {
       const int k = idx / cols, l = idx - k * cols, c = c0 + l;
       if (c >= a.nc) continue;
@@ -340,7 +284,7 @@ __global__ void __launch_bounds__(kGenericThreads) col_generic_kernel(ColArgs a,
       if (iy < 0) continue;
       const float2 v = cur[l * ls + k];
       const double I = ((double)v.x * v.x + (double)v.y * v.y) * a.det.scale;
-      detect_pixel(a.det, k, iy, a.ny, I, pos, s);
+      detect_pixel(a.det, k, iy, I, pos, s);
     }
     __syncthreads();
     double* scratch = reinterpret_cast<double*>(oth);
@@ -359,13 +303,14 @@ __global__ void __launch_bounds__(kGenericThreads) col_generic_kernel(ColArgs a,
 }
 
 // Sum the per-block partials of every wave in block order and add them to the position's accumulators.
-__global__ void reduce_partials_kernel(DetPlan d, int pos0, double* acc) {
+__global__ void reduce_partials_kernel(DetPlan d, int pos0, const int* pos_dev, double* acc) {
   const long long b = blockIdx.x;
+  const long long pos = first_position(pos0, pos_dev) + b;
   for (int q = threadIdx.x; q < d.nslots; q += blockDim.x) {
     const double* p = d.partial + (b * d.nslots + q) * d.blocks_per_wave;
     double s = 0.0;
     for (int i = 0; i < d.blocks_per_wave; ++i) s += p[i];
-    acc[(pos0 + b) * d.nslots + q] += s;
+    acc[pos * d.nslots + q] += s;
   }
 }
 
@@ -389,9 +334,9 @@ void launch_col_generic(const ColArgs& a, int batch, hipStream_t s) {
   SK_HIP(hipGetLastError());
 }
 
-void launch_reduce_partials(const DetPlan& d, int batch, int pos0, double* acc, hipStream_t s) {
+void launch_reduce_partials(const DetPlan& d, int batch, int pos0, const int* pos_dev, double* acc, hipStream_t s) {
   if (d.nslots == 0) return;
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3(batch), dim3(64), 0, s, d, pos0, acc);
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(batch), dim3(64), 0, s, d, pos0, pos_dev, acc);
   SK_HIP(hipGetLastError());
 }
 

@@ -232,11 +232,12 @@ void launch_normalise_spectrum(const double* re, const double* im, const double*
 }
 
 // Probe.shiftRamps (Probe.swift:140-156): r_x[ix] = e^{-2 pi i p_x[ix] u_x}, p = fftfreq(n, 1), u = position / sampling
-__global__ void ramps_kernel(int nx, int ny, double sx, double sy, const double* positions, int pos0, int count,
-                             float2* rx, float2* ry) {
+__global__ void ramps_kernel(int nx, int ny, double sx, double sy, const double* positions, int pos0,
+                             const int* pos_dev, int count, float2* rx, float2* ry) {
   const int b = blockIdx.y;
   if (b >= count) return;
-  const double ux = positions[2 * (size_t)(pos0 + b)] / sx, uy = positions[2 * (size_t)(pos0 + b) + 1] / sy;
+  const long long q = first_position(pos0, pos_dev) + b;
+  const double ux = positions[2 * q] / sx, uy = positions[2 * q + 1] / sy;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nx + ny; i += gridDim.x * blockDim.x) {
     double sn, cs;
     if (i < nx) {
@@ -251,10 +252,11 @@ __global__ void ramps_kernel(int nx, int ny, double sx, double sy, const double*
     }
   }
 }
-void launch_ramps(int nx, int ny, double sx, double sy, const double* positions, int pos0, int count, float2* rx,
-                  float2* ry, hipStream_t s) {
+void launch_ramps(int nx, int ny, double sx, double sy, const double* positions, int pos0, const int* pos_dev,
+                  int count, float2* rx, float2* ry, hipStream_t s) {
   dim3 grid((nx + ny + kThreads - 1) / kThreads, count);
-  hipLaunchKernelGGL(ramps_kernel, grid, dim3(kThreads), 0, s, nx, ny, sx, sy, positions, pos0, count, rx, ry);
+  hipLaunchKernelGGL(ramps_kernel, grid, dim3(kThreads), 0, s, nx, ny, sx, sy, positions, pos0, pos_dev, count, rx,
+                     ry);
   SK_HIP(hipGetLastError());
 }
 

@@ -54,6 +54,8 @@ def _options(opt: Optional[ExecutionOptions]) -> N.SkOptions:
     s.batch = int(o.batch)
     s.use_graph = 1 if o.use_graph else 0
     s.fast_path = 1 if o.fast_path else 0
+    s.pipeline = int(o.pipeline)
+    s.lane_waves = int(o.lane_waves)
     return s
 
 

@@ -1,0 +1,78 @@
+"""The measured workloads on the GPU, at their real sizes.
+
+- HRTEM (SrTiO3 4 x 4 x 10, 256 x 256, 20 slices, 200 keV plane wave): exit-wave intensity against the float64
+  oracle.
+- HAADF scan (SrTiO3 8 x 8 x 20, 512 x 512, 40 slices, 64 x 64 positions, 50-200 mrad): the full scan, checked by
+  properties that hold whatever its size (the 4mm symmetry of SrTiO3 [001] maps the image onto its transpose and its
+  mirrors; the detector never sees more than the incident intensity) and against the oracle at sample positions.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.conftest import rel_l2, rel_max
+
+import simulatokit_amd as sk
+from simulatokit_amd.workloads import HaadfScan, SRTIO3_A, srtio3
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-4
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    if not sk.hip_is_available():
+        pytest.fail("no HIP device: " + sk._native.last_error())
+
+
+def test_hrtem_exit_intensity_equals_the_oracle():
+    st = srtio3(4, 4, 10)
+    pot = sk.Potential(st, 256, 256, slice_thickness=2.0)
+    assert pot.slice_count == 20
+    beam = sk.ElectronBeam(200e3)
+    w = sk.Multislice.run(sk.Waves.plane_wave(beam, pot.grid), pot)[0]
+    V, _ = O.potential(256, 256, st.numbers, st.positions, st.box, pot.slice_thicknesses)
+    want = O.multislice(V, pot.slice_thicknesses, (st.box[0], st.box[1]), 200e3, np.ones((256, 256), complex))[0]
+    I, Io = w.intensity(), np.abs(want) ** 2
+    print("HRTEM intensity rel L2", rel_l2(I, Io), "rel max", rel_max(I, Io))
+    assert rel_l2(I, Io) <= TOL
+    assert rel_max(I, Io) <= TOL
+
+
+@pytest.fixture(scope="module")
+def haadf_scan():
+    w = HaadfScan()
+    st = w.structure
+    pot = sk.Potential(st, w.n, w.n, slice_thickness=w.slice_thickness)
+    probe = sk.Probe(sk.ElectronBeam(w.energy), pot.grid, sk.Aperture(w.semiangle))
+    scan = sk.GridScan((0.0, 0.0), (SRTIO3_A, SRTIO3_A), w.scan_n, w.scan_n)
+    prepared = pot.prepare(probe.beam)
+    dets = [sk.AnnularDetector(w.inner, w.outer), sk.AnnularDetector(0, None)]
+    haadf, total = sk.STEM.scan(probe, [prepared], scan, dets, sk.ExecutionOptions(batch=w.batch))
+    return w, st, pot, scan, haadf, total
+
+
+def test_full_haadf_scan_has_the_crystal_symmetry(haadf_scan):
+    w, _, _, _, haadf, total = haadf_scan
+    img = haadf.image()
+    assert img.shape == (64, 64)
+    n = w.scan_n
+    mirror = img[(n - np.arange(n)) % n, :]
+    scale = np.abs(img).max()
+    print("transpose", np.abs(img - img.T).max() / scale, "mirror", np.abs(img - mirror).max() / scale)
+    assert np.abs(img - img.T).max() <= TOL * scale
+    assert np.abs(img - mirror).max() <= TOL * scale
+    # the annulus never holds more than the bandlimited total, which never exceeds the incident intensity 1
+    assert (img >= 0).all() and (img <= total.image() + 1e-12).all() and (total.image() <= 1 + 1e-5).all()
+
+
+def test_full_haadf_scan_equals_the_oracle_at_sample_positions(haadf_scan):
+    w, st, pot, scan, haadf, _ = haadf_scan
+    pick = [0, 1, 33, 64 * 17 + 5, 64 * 32 + 32, 4095]
+    pos = scan.positions[pick]
+    V, _ = O.potential(w.n, w.n, st.numbers, st.positions, st.box, pot.slice_thicknesses)
+    want = O.stem_scan([V], [pot.slice_thicknesses], (st.box[0], st.box[1]), w.energy, w.semiangle, pos,
+                       [O.det_annular(w.inner, w.outer)], threads=6)[0]
+    got = haadf.values[pick]
+    print("HAADF rel err at samples", np.abs(got - want) / np.abs(want))
+    assert rel_max(got, want) <= TOL
