@@ -24,6 +24,24 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# measured HBM bytes per launch of the dominant kernel: rocprofv3 FETCH_SIZE and WRITE_SIZE passes over this
+# workload, scaled by what the same counters read for a device copy of known size (the summary holds the commands)
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "column_pass_haadf_512_traffic.md")
+
+
+def measured_traffic(kernel: str):
+    """The `traffic` block of the committed counter summary for `kernel` at this workload, or None."""
+    try:
+        text = open(TRAFFIC_SUMMARY).read()
+    except OSError:
+        return None
+    start = text.find("```json")
+    if start < 0:
+        return None
+    end = text.find("```", start + 7)
+    data = json.loads(text[start + 7:end])
+    entry = data.get(kernel)
+    return None if entry is None else float(entry["hbm_bytes_per_launch"])
 
 
 def main():
@@ -34,6 +52,7 @@ def main():
     ap.add_argument("--batch", type=int, default=1024)
     ap.add_argument("--graph", action="store_true", help="replay each batch's slice loop from a hipGraph")
     ap.add_argument("--generic", action="store_true", help="generic LDS passes instead of the fast passes")
+    ap.add_argument("--col-tile", type=int, default=0, help="columns per 512-point column-pass block (8, 16)")
     ap.add_argument("--cpu-sample", type=int, default=32, help="positions of the float64 CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(8, cores), the reference's default")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -69,7 +88,8 @@ def main():
     probe = sk.Probe(beam, pot.grid, sk.Aperture(w.semiangle))
     scan = sk.GridScan((0.0, 0.0), (sk.workloads.SRTIO3_A, sk.workloads.SRTIO3_A), w.scan_n, w.scan_n)
     det = sk.AnnularDetector(w.inner, w.outer)
-    opt = sk.ExecutionOptions(batch=w.batch, use_graph=args.graph, fast_path=not args.generic)
+    opt = sk.ExecutionOptions(batch=w.batch, use_graph=args.graph, fast_path=not args.generic,
+                              col_tile=args.col_tile)
     S = pot.slice_count
     npos = w.scan_n * w.scan_n
 
@@ -101,10 +121,11 @@ def main():
     N = w.n * w.n
     B = min(w.batch, npos)
     alg_bytes = 16.0 * N * B  # per pass launch: the complex64 wave of every position read and written once
-    col_ms, row_ms = tm["col_ms"], tm["row_ms"]
+    col_ms, row_ms = tm["col_ms"], tm["row_ms"]  # the average launch of each pass
     dom = "column" if col_ms >= row_ms else "row"
     dom_ms = max(col_ms, row_ms)
     achieved = alg_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
+    stored_bytes = tm["col_bytes"]  # per launch: the compact columns of every wave read and written once
     haadf = out.cpu().numpy()
 
     result = {
@@ -125,9 +146,9 @@ def main():
                    "semiangle_mrad": w.semiangle, "fast_path": not args.generic, "graph": bool(args.graph),
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(f"{dom} pass"),
                      "kernel": f"{dom} pass", "kernel_ms": dom_ms, "row_ms": row_ms, "col_ms": col_ms,
-                     "algorithmic_bytes_per_launch": alg_bytes},
+                     "algorithmic_bytes_per_launch": alg_bytes, "stored_bytes_per_launch": stored_bytes},
         "prepare_ms": prepare_ms,
         "scan_pos_slices": npos * S,
     }

@@ -74,7 +74,8 @@ typedef struct sk_options {
                        and slice (potentials of different slice counts share every launch); 0 = one potential at a
                        time; -1 = the library's choice (lanes when npots > 1 and the grid has fast passes) */
   int lane_waves;   /* pipeline: positions per lane (0 = the library's choice) */
-  int reserved[3];
+  int col_tile;     /* 512-point columns: columns per block of the column pass, 8 or 16 (0 = the library's choice) */
+  int reserved[2];
 } sk_options;
 
 /* --- availability (MetalEngine.require / Multislice.metalIsAvailable, MetalEngine.swift:63-77,

@@ -205,6 +205,7 @@ class ExecutionOptions:
     pipeline: int = -1       # scan: 1 lanes of positions each on its own potential and slice per launch, 0 one
                              # potential at a time, -1 the library's choice (lanes when there are several potentials)
     lane_waves: int = 0      # positions per lane (0 = the library's choice)
+    col_tile: int = 0        # 512-point columns: columns per column-pass block, 8 or 16 (0 = the library's choice)
 
 
 @dataclass(frozen=True)

@@ -47,6 +47,7 @@ struct Context {
   // timing
   bool timing = false;
   sk_timing last{};
+  int col_tile = 0;  // the running call's sk_options.col_tile
   // identity keep maps per length (full-width transforms)
   std::map<int, std::unique_ptr<DevBuf<int>>> identity;
 };
@@ -436,6 +437,12 @@ Workspace& ws() {
   return *w;
 }
 
+void set_col_tile(Context& c, const sk_options* opt) {
+  const int t = opt ? opt->col_tile : 0;
+  if (t != 0 && t != 8 && t != 16) fail(SK_ERR_INVALID, "col_tile must be 0, 8 or 16");
+  c.col_tile = t;
+}
+
 int default_batch(const sk_potential& p, int count, const sk_options* opt) {
   int b = opt && opt->batch > 0 ? opt->batch : 0;
   if (b <= 0) {
@@ -477,6 +484,7 @@ ColArgs col_args(const sk_potential& p, float2* G) {
   c.fx = line_plan(p.grid.nx, ctx().stream).dev;
   c.keep = p.keep_d.p;
   c.G = G;
+  c.col_tile = fast_col_tile(p.grid.nx, ctx().col_tile);
   return c;
 }
 
@@ -904,6 +912,7 @@ static void run_waves(const sk_potential* p, int count, const double* in_re, con
                       const std::vector<int>& planes, double* out_re, double* out_im, const sk_options* opt) {
   Context& c = require();
   hipStream_t s = c.stream;
+  set_col_tile(c, opt);
   const size_t N = (size_t)p->grid.nx * p->grid.ny;
   const int nplanes = (int)planes.size();
   // plane j -> output slot (the incident plane -1 is copied on the host side)
@@ -1089,6 +1098,7 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
   return guarded([&] {
     Context& c = require();
     hipStream_t s = c.stream;
+    set_col_tile(c, opt);
     if (!pots || npots < 1) fail(SK_ERR_INVALID, "invalid scan: no potential");
     check_probe(probe);
     if (npos < 1 || !positions) fail(SK_ERR_INVALID, "invalid scan: no positions");
@@ -1244,7 +1254,7 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
     if (!fits) w.full.ensure((size_t)B * N);
     int bpw = (p0.nc + col_generic_lines(line_plan(nx, s).dev.m) - 1) / col_generic_lines(line_plan(nx, s).dev.m);
     if (fast && fast_col_supported(nx, p0.nc, COL_IN_G | COL_FWD | COL_MUL_P | COL_DETECT))
-      bpw = fast_col_blocks_per_wave(nx, p0.nc);
+      bpw = fast_col_blocks_per_wave(nx, p0.nc, pipe ? 16 : c.col_tile);
     dp.blocks_per_wave = bpw;
     w.partial.ensure((size_t)B * nslots * bpw);
     dp.partial = w.partial.p;

@@ -60,12 +60,78 @@ struct LaneTwiddles {
       for (int j = 0; j < R3; ++j) w3[q][j] = tw[j * k * (N / (L3 * R3))];
     }
   }
+  __device__ __forceinline__ float2 s2(int q, int j) const { return w2[q][j]; }
+  __device__ __forceinline__ float2 s3(int q, int j) const { return w3[q][j]; }
 };
 
+// The same factors read from per-block LDS tables t2[j][k] (k < L2) and t3[j][k] (k < L3): lanes read consecutive
+// entries (stage 3) or share them (stage 2), so the reads are free of bank conflicts. Frees the lane's registers.
+template <int N>
+struct LdsTwiddles {
+  static constexpr int T = N / 8;
+  static constexpr int R2 = FastPlan<N>::R2, R3 = FastPlan<N>::R3;
+  static constexpr int L2 = FastPlan<N>::R1, L3 = FastPlan<N>::R1 * FastPlan<N>::R2;
+  static constexpr int kSize = R2 * L2 + R3 * L3;
+  const float2* t2;
+  const float2* t3;
+  int i;
+  // fill the tables (all threads of the block; the caller synchronises)
+  __device__ static void build(float2* table, const float2* __restrict__ tw, int tid, int nthreads) {
+    for (int e = tid; e < kSize; e += nthreads) {
+      if (e < R2 * L2) {
+        const int j = e / L2, k = e % L2;
+        table[e] = tw[j * k * (N / (L2 * R2))];
+      } else {
+        const int f = e - R2 * L2, j = f / L3, k = f % L3;
+        table[e] = tw[j * k * (N / (L3 * R3))];
+      }
+    }
+  }
+  __device__ __forceinline__ float2 s2(int q, int j) const { return t2[j * L2 + (i + T * q) % L2]; }
+  __device__ __forceinline__ float2 s3(int q, int j) const { return t3[j * L3 + (i + T * q) % L3]; }
+};
+
+// Synchronisation of an exchange that stays inside one wavefront: its lanes execute in lockstep and the LDS serves
+// one wavefront's instructions in order, so only the compiler must not move LDS accesses across this point.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <bool WAVE>
+__device__ __forceinline__ void exchange_sync() {
+  if constexpr (WAVE) wave_sync();
+  else __syncthreads();
+}
+
+// LDS slot of element j of a row line: an XOR swizzle under which every access pattern of the 256- and 512-point
+// stages (stores 8i + j, loads i + T j, stage-2 stores (i / 8) 64 + i mod 8 + 8 j) hits distinct bank pairs in every
+// group of 16 and of 32 lanes.
+__device__ __forceinline__ int row_slot(int j) { return j ^ ((j >> 3) & 31); }
+
+// LDS slot of (x, column) in a 512 x TC column tile: conflict-free both for the block's coalesced phases (lanes that
+// share an x take consecutive columns) and for each column's own exchanges (the row_slot patterns in x), in groups of
+// 16 and of 32 lanes.
+template <int TC>
+__device__ __forceinline__ int tile_slot(int x, int c) {
+  if constexpr (TC == 16) {
+    const int h = ((x >> 4) ^ (x >> 7)) & 1;
+    const int g = (x ^ (x >> 3)) & 15;
+    return (x ^ h) * 16 + (c ^ g);
+  } else {
+    static_assert(TC == 8, "column tiles of 8 or 16");
+    const int h = ((x >> 1) ^ (x >> 6)) & 3;
+    const int g = (x ^ (x >> 3)) & 7;
+    return (x ^ h) * 8 + (c ^ g);
+  }
+}
+
 // One Stockham stage of radix R at sub-length L (see the file comment); W: the lane's factors for this stage
-// (ignored when L = 1). FROM_REG: inputs are the registers; TO_REG: outputs go to the registers.
-template <int N, int R, int L, int SIGN, bool FROM_REG, bool TO_REG, class Idx>
-__device__ __forceinline__ void stage(float2 (&v)[8], float2* __restrict__ buf, int i, const float2 (*W)[R], Idx idx) {
+// (ignored when L = 1). FROM_REG: inputs are the registers; TO_REG: outputs go to the registers. WAVE: the line
+// lives in one wavefront, so the exchanges need no block barrier.
+template <int N, int R, int L, int SIGN, bool FROM_REG, bool TO_REG, bool WAVE = false, class Idx, class Tw>
+__device__ __forceinline__ void stage(float2 (&v)[8], float2* __restrict__ buf, int i, Tw W, Idx idx) {
   constexpr int T = N / 8;
   constexpr int NB = 8 / R;
   constexpr int STRIDE = N / R;
@@ -79,14 +145,17 @@ __device__ __forceinline__ void stage(float2 (&v)[8], float2* __restrict__ buf, 
       else u[q][j] = buf[idx(bi + STRIDE * j)];
     }
   }
-  if constexpr (!FROM_REG) __syncthreads();  // every lane has read the line before any lane overwrites it
+  if constexpr (!FROM_REG) exchange_sync<WAVE>();  // every lane has read the line before any lane overwrites it
 #pragma unroll
   for (int q = 0; q < NB; ++q) {
     const int bi = i + T * q;
     const int k = (L == 1) ? 0 : (bi % L);
     if constexpr (L > 1) {
 #pragma unroll
-      for (int j = 1; j < R; ++j) u[q][j] = SIGN < 0 ? cmul(u[q][j], W[q][j]) : cmulc(u[q][j], W[q][j]);
+      for (int j = 1; j < R; ++j) {
+        const float2 wf = W(q, j);
+        u[q][j] = SIGN < 0 ? cmul(u[q][j], wf) : cmulc(u[q][j], wf);
+      }
     }
     dft_pow2<R>(u[q], SIGN);
     if constexpr (TO_REG) {
@@ -98,16 +167,19 @@ __device__ __forceinline__ void stage(float2 (&v)[8], float2* __restrict__ buf, 
       for (int j = 0; j < R; ++j) buf[idx(base + j * L)] = u[q][j];
     }
   }
-  if constexpr (!TO_REG) __syncthreads();
+  if constexpr (!TO_REG) exchange_sync<WAVE>();
 }
 
 // Unscaled transform of the line in registers (SIGN -1 forward, +1 inverse).
-template <int N, int SIGN, class Idx>
-__device__ __forceinline__ void fft_reg(float2 (&v)[8], float2* buf, int i, const LaneTwiddles<N>& w, Idx idx) {
+template <int N, int SIGN, bool WAVE = false, class Idx, class TW>
+__device__ __forceinline__ void fft_reg(float2 (&v)[8], float2* buf, int i, const TW& w, Idx idx) {
   using P = FastPlan<N>;
-  stage<N, P::R1, 1, SIGN, true, false>(v, buf, i, (const float2(*)[P::R1]) nullptr, idx);
-  stage<N, P::R2, P::R1, SIGN, false, false>(v, buf, i, w.w2, idx);
-  stage<N, P::R3, P::R1 * P::R2, SIGN, false, true>(v, buf, i, w.w3, idx);
+  auto none = [](int, int) { return make_float2(1.f, 0.f); };
+  auto f2 = [&w](int q, int j) { return w.s2(q, j); };
+  auto f3 = [&w](int q, int j) { return w.s3(q, j); };
+  stage<N, P::R1, 1, SIGN, true, false, WAVE>(v, buf, i, none, idx);
+  stage<N, P::R2, P::R1, SIGN, false, false, WAVE>(v, buf, i, f2, idx);
+  stage<N, P::R3, P::R1 * P::R2, SIGN, false, true, WAVE>(v, buf, i, f3, idx);
 }
 
 constexpr int kColTile = 16;  // compact columns per column-pass block
@@ -131,7 +203,7 @@ __device__ __forceinline__ int compact_of(const RowArgs& a, int iy) {
 // of the pipeline, whose descriptor for the current step gives the transmission function and the position count.
 template <int N, int FLAGS, int PB, bool PIPE = false>
 __global__ void __launch_bounds__(64) row_fast_kernel(RowArgs a, const float2* __restrict__ tw, int batch) {
-  constexpr int T = N / 8, ROWS = 64 / T, LD = N + N / 8;
+  constexpr int T = N / 8, ROWS = 64 / T, LD = N;
   __shared__ float2 lds[ROWS * LD];
   const int tid = threadIdx.x, row = tid / T, i = tid - row * T;
   const int x = blockIdx.x * ROWS + row;  // the host launches nx / ROWS blocks along x: every x is in range
@@ -146,7 +218,7 @@ __global__ void __launch_bounds__(64) row_fast_kernel(RowArgs a, const float2* _
     limit = (long long)lane * a.pipe.lane_waves + d.count;
   }
   float2* buf = lds + row * LD;
-  auto idx = [](int j) { return j + (j >> 3); };
+  auto idx = [](int j) { return row_slot(j); };
   const long long gw = (long long)a.nx * a.nc;
   LaneTwiddles<N> w;
   w.load(tw, i);
@@ -172,7 +244,7 @@ __global__ void __launch_bounds__(64) row_fast_kernel(RowArgs a, const float2* _
 #pragma unroll
       for (int r = 0; r < 8; ++r) v[r] = g[i + T * r];
     }
-    if constexpr ((FLAGS & ROW_INV) != 0) fft_reg<N, +1>(v, buf, i, w, idx);
+    if constexpr ((FLAGS & ROW_INV) != 0) fft_reg<N, +1, true>(v, buf, i, w, idx);
     if constexpr ((FLAGS & ROW_WRITE_PLANE) != 0) {
       float2* pl = a.plane + b * a.plane_stride + (long long)x * N;
 #pragma unroll
@@ -182,7 +254,7 @@ __global__ void __launch_bounds__(64) row_fast_kernel(RowArgs a, const float2* _
 #pragma unroll
       for (int r = 0; r < 8; ++r) v[r] = cmul(v[r], t[r]);
     }
-    if constexpr ((FLAGS & ROW_FWD) != 0) fft_reg<N, -1>(v, buf, i, w, idx);
+    if constexpr ((FLAGS & ROW_FWD) != 0) fft_reg<N, -1, true>(v, buf, i, w, idx);
     if constexpr ((FLAGS & ROW_OUT_COMPACT) != 0) {
       float2* g = a.G + b * gw + (long long)x * a.nc;
 #pragma unroll
@@ -305,6 +377,81 @@ __global__ void __launch_bounds__(kColTile * (N / 8)) col_fast_kernel(ColArgs a,
   }
 }
 
+// Column pass of 512-point columns, one column per wavefront. The block's TC columns move between HBM and a
+// swizzled LDS tile in coalesced row segments of TC values (block phases); each wavefront then transforms its own
+// column with wavefront-local exchanges inside the tile (no block barrier), and the next position's loads are in
+// flight meanwhile. Three block barriers per position. With TC = 8 two blocks share a CU, so one block's barriers
+// overlap the other's work.
+template <int FLAGS, int PB, int TC>
+__global__ void __launch_bounds__(TC * 64) col512_kernel(ColArgs a, const float2* __restrict__ tw, int batch) {
+  constexpr int N = 512, T = 64, NT = TC * 64;
+  constexpr bool HAS_P = (FLAGS & COL_MUL_P) != 0;
+  __shared__ float2 tile[N * TC];
+  __shared__ float2 ptile[HAS_P ? N * TC : 1];  // the block's columns of P (swizzled as the tile)
+  __shared__ float2 twt[LdsTwiddles<N>::kSize];
+  __shared__ double red[(NT / 64) * kMaxSlots];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int cb = tid % TC, xi = tid / TC;  // block phases: column cb of the tile, rows xi + 64 r
+  const int c0 = blockIdx.x * TC;          // the host launches nc / TC blocks along x: every column is in range
+  const int iyb = a.keep[c0 + cb];
+  const int iyw = a.keep[c0 + wave];       // the wavefront's column
+  const long long b0 = (long long)blockIdx.y * PB;
+  const long long blim = b0 + PB < batch ? b0 + PB : batch;
+  const long long gw = (long long)a.nx * a.nc;
+  auto tile_slot = [](int x, int c) { return sk::tile_slot<TC>(x, c); };
+  auto widx = [wave, tile_slot](int j) { return tile_slot(j, wave); };
+  LdsTwiddles<N>::build(twt, tw, tid, NT);
+  const LdsTwiddles<N> w{twt, twt + LdsTwiddles<N>::R2 * LdsTwiddles<N>::L2, lane};
+  if constexpr (HAS_P) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) ptile[tile_slot(xi + T * r, cb)] = a.P[(long long)(xi + T * r) * a.nc + c0 + cb];
+  }
+  __syncthreads();
+  auto gload = [&](long long b, float2 (&g)[8]) {
+    if constexpr ((FLAGS & COL_IN_G) != 0) {
+      const float2* src = a.G + b * gw + c0 + cb;
+#pragma unroll
+      for (int r = 0; r < 8; ++r) g[r] = iyb >= 0 ? src[(long long)(xi + T * r) * a.nc] : make_float2(0.f, 0.f);
+    } else {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const int x = xi + T * r;
+        g[r] = iyb >= 0 ? cmul(cmul(a.spec[(long long)x * a.ny + iyb], a.rx[b * a.nx + x]), a.ry[b * a.ny + iyb])
+                        : make_float2(0.f, 0.f);
+      }
+    }
+  };
+  float2 g[8];
+  if (b0 < blim) gload(b0, g);
+  for (long long b = b0; b < blim; ++b) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) tile[tile_slot(xi + T * r, cb)] = g[r];
+    __syncthreads();
+    if (b + 1 < blim) gload(b + 1, g);  // in flight during the transforms
+    float2 v[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = tile[tile_slot(lane + T * r, wave)];
+    if constexpr ((FLAGS & COL_FWD) != 0) fft_reg<N, -1, true>(v, tile, lane, w, widx);
+    if constexpr (HAS_P) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) v[r] = cmul(v[r], ptile[tile_slot(lane + T * r, wave)]);
+    }
+    if constexpr ((FLAGS & COL_DETECT) != 0) {
+      detect_block<N, NT>(v, a.det, first_position(a.pos0, a.pos_dev) + b, b, iyw, lane, tid,
+                          reinterpret_cast<float2*>(red));
+      continue;  // detect_block ends with a barrier: the tile is free again
+    }
+    if constexpr ((FLAGS & COL_INV) != 0) fft_reg<N, +1, true>(v, tile, lane, w, widx);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) tile[tile_slot(lane + T * r, wave)] = v[r];
+    __syncthreads();
+    float2* dst = a.G + b * gw + c0 + cb;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) dst[(long long)(xi + T * r) * a.nc] = tile[tile_slot(xi + T * r, cb)];
+    __syncthreads();
+  }
+}
+
 constexpr int kPipePB = 8;  // positions per block in pipelined launches; lane_waves is a multiple of it
 
 // Pipelined column pass: the lane of the block's positions gives the mode of this step —
@@ -411,11 +558,20 @@ void row_launch(const RowArgs& a, int batch, hipStream_t s, const float2* tw) {
   SK_HIP(hipGetLastError());
 }
 
+constexpr int kCol512Tile = 16;  // default columns per block of the 512-point column pass (8 measured slower)
+
 template <int N, int F>
 void col_go(const ColArgs& a, int batch, hipStream_t s, const float2* tw) {
-  constexpr int NT = kColTile * (N / 8);
-  dim3 grid(a.nc / kColTile, (batch + kColPB - 1) / kColPB);
-  hipLaunchKernelGGL((col_fast_kernel<N, F, kColPB>), grid, dim3(NT), 0, s, a, tw, batch);
+  const int by = (batch + kColPB - 1) / kColPB;
+  if constexpr (N == 512) {
+    if (fast_col_tile(512, a.col_tile) == 8)
+      hipLaunchKernelGGL((col512_kernel<F, kColPB, 8>), dim3(a.nc / 8, by), dim3(8 * 64), 0, s, a, tw, batch);
+    else
+      hipLaunchKernelGGL((col512_kernel<F, kColPB, 16>), dim3(a.nc / 16, by), dim3(16 * 64), 0, s, a, tw, batch);
+  } else {
+    hipLaunchKernelGGL((col_fast_kernel<N, F, kColPB>), dim3(a.nc / kColTile, by), dim3(kColTile * (N / 8)), 0, s,
+                       a, tw, batch);
+  }
 }
 
 template <int N>
@@ -443,7 +599,12 @@ bool fast_col_supported(int nx, int nc, int flags) {
   return flags == COL_MID || flags == COL_INIT || flags == COL_FINAL;
 }
 
-int fast_col_blocks_per_wave(int, int nc) { return nc / kColTile; }
+int fast_col_tile(int nx, int requested) {
+  if (nx != 512) return kColTile;
+  return requested == 8 || requested == 16 ? requested : kCol512Tile;
+}
+
+int fast_col_blocks_per_wave(int nx, int nc, int col_tile) { return nc / fast_col_tile(nx, col_tile); }
 
 void launch_row_fast(const RowArgs& a, int batch, hipStream_t s) {
   const float2* tw = line_plan(a.ny, s).dev.roots;

@@ -197,6 +197,7 @@ struct ColArgs {
   const float2* ry;       // [batch][ny]
   int pos0;               // position index of wave 0 (detectors)
   const int* pos_dev;     // optional device offset added to pos0 (a replayed graph reads its batch here)
+  int col_tile;           // 512-point fast column pass: columns per block, 8 or 16 (0 = the default)
   DetPlan det;
   PipeArgs pipe;          // lane pipeline (fast passes only); pipe.descs null otherwise
 };
@@ -216,7 +217,8 @@ bool fast_row_supported(int nx, int ny, int flags);
 bool fast_col_supported(int nx, int nc, int flags);
 void launch_row_fast(const RowArgs& a, int batch, hipStream_t s);
 void launch_col_fast(const ColArgs& a, int batch, hipStream_t s);
-int fast_col_blocks_per_wave(int nx, int nc);
+int fast_col_tile(int nx, int requested);  // columns per block the fast column pass uses
+int fast_col_blocks_per_wave(int nx, int nc, int col_tile);
 // lane pipeline launches (sk_fast.hip): the row pass and the column pass of one step, the partial-sum reduction of
 // the lanes that finished an item, and the step counter's increment
 bool pipeline_supported(int nx, int ny, int nc, int lane_waves);

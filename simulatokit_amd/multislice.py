@@ -56,6 +56,7 @@ def _options(opt: Optional[ExecutionOptions]) -> N.SkOptions:
     s.fast_path = 1 if o.fast_path else 0
     s.pipeline = int(o.pipeline)
     s.lane_waves = int(o.lane_waves)
+    s.col_tile = int(o.col_tile)
     return s
 
 

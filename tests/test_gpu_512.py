@@ -1,0 +1,71 @@
+"""The 512-point fast passes against the float64 oracle, on an MI355X.
+
+The 512 x 512 grid is the HAADF workload's; its column pass has two block shapes (8 or 16 columns per block) and the
+lane pipeline has its own column kernel, so each runs here on the same inputs: exit waves of a plane wave, and a scan
+with annular, centre-of-mass and pixelated detectors. Bounds as for the other grids (complex64 vs float64).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.conftest import rel_max
+
+import simulatokit_amd as sk
+from simulatokit_amd.workloads import SRTIO3_A, srtio3
+
+pytestmark = pytest.mark.gpu
+
+ACC = sk.REGISTERED
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    if not sk.hip_is_available():
+        pytest.fail("no HIP device: " + sk._native.last_error())
+
+
+@pytest.fixture(scope="module")
+def stack():
+    st = srtio3(4, 4, 2)
+    pot = sk.Potential(st, 512, 512, slice_thickness=2.0)
+    V, _ = O.potential(512, 512, st.numbers, st.positions, st.box, pot.slice_thicknesses)
+    return st, pot, V
+
+
+@pytest.mark.parametrize("col_tile", [8, 16])
+def test_plane_wave_exit_512_equals_the_oracle(stack, col_tile):
+    st, pot, V = stack
+    beam = sk.ElectronBeam(200e3)
+    opt = sk.ExecutionOptions(col_tile=col_tile)
+    w = sk.Multislice.run(sk.Waves.plane_wave(beam, pot.grid), pot, options=opt)[0]
+    want = O.multislice(V, pot.slice_thicknesses, (st.box[0], st.box[1]), 200e3, np.ones((512, 512), complex))[0]
+    assert rel_max(w.field, want) <= ACC.wave
+
+
+def test_col_tile_choices_are_refused_outside_8_and_16(stack):
+    _, pot, _ = stack
+    with pytest.raises(sk.SimulatoError):
+        sk.Multislice.run(sk.Waves.plane_wave(sk.ElectronBeam(200e3), pot.grid), pot,
+                          options=sk.ExecutionOptions(col_tile=4))
+
+
+@pytest.fixture(scope="module")
+def scan_case(stack):
+    st, pot, V = stack
+    probe = sk.Probe(sk.ElectronBeam(200e3), pot.grid, sk.Aperture(21.4))
+    scan = sk.GridScan((0, 0), (SRTIO3_A, SRTIO3_A), 3, 4)
+    dets = [sk.AnnularDetector(50, 200), sk.CenterOfMassDetector("cutoff", "mrad"), sk.PixelatedDetector("cutoff")]
+    want = O.stem_scan([V], [pot.slice_thicknesses], (st.box[0], st.box[1]), 200e3, 21.4, scan.positions,
+                       [O.det_annular(50, 200), O.det_com("cutoff", "mrad"), O.det_pixelated("cutoff")])
+    return pot.prepare(probe.beam), probe, scan, dets, want
+
+
+@pytest.mark.parametrize("opt", [dict(col_tile=8), dict(col_tile=16), dict(pipeline=1, lane_waves=8),
+                                 dict(col_tile=8, use_graph=True, batch=5)],
+                         ids=["tile8", "tile16", "pipeline", "graph"])
+def test_scan_512_equals_the_oracle(scan_case, opt):
+    prepared, probe, scan, dets, want = scan_case
+    got = sk.STEM.scan(probe, [prepared], scan, dets, sk.ExecutionOptions(**opt))
+    for m, wv in zip(got, want):
+        wv = wv.reshape(-1)
+        assert np.abs(m.values - wv).max() <= ACC.detector_bound(np.abs(wv).max())
