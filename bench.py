@@ -53,7 +53,7 @@ def main():
     ap.add_argument("--graph", action="store_true", help="replay each batch's slice loop from a hipGraph")
     ap.add_argument("--generic", action="store_true", help="generic LDS passes instead of the fast passes")
     ap.add_argument("--col-tile", type=int, default=0, help="columns per 512-point column-pass block (8, 16)")
-    ap.add_argument("--cpu-sample", type=int, default=32, help="positions of the float64 CPU baseline sample")
+    ap.add_argument("--cpu-sample", type=int, default=128, help="positions of the float64 CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(8, cores), the reference's default")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
