@@ -1,12 +1,15 @@
 """Build the HIP library in-tree: simulatokit_amd/_lib/libsimulato_hip.so (gfx950).
 
     python -m simulatokit_amd.build
+
+Each source compiles to its own object in parallel, then one link.
 """
 from __future__ import annotations
 
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -15,23 +18,33 @@ LIB_PATH = os.path.join(LIB_DIR, "libsimulato_hip.so")
 SOURCES = ["sk_passes.hip", "sk_fast.hip", "sk_prepare.hip", "sk_api.hip"]
 ARCH = os.environ.get("SK_OFFLOAD_ARCH", "gfx950")
 
-# Packed FP32 instructions are disabled for every device code object ("Build").
-HIPCC_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
+# Packed FP32 instructions (v_pk_*_f32) are kept out of every device code object: the compiler would otherwise form
+# them from plain float code, and some of their forms have given wrong results without any error.
+HIPCC_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
 
 
-def build(verbose: bool = False) -> str:
-    os.makedirs(LIB_DIR, exist_ok=True)
-    tmp = LIB_PATH + ".tmp"
-    cmd = ["hipcc", *HIPCC_FLAGS, *[os.path.join(CSRC, s) for s in SOURCES], "-o", tmp]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed to build libsimulato_hip.so")
-    os.replace(tmp, LIB_PATH)
-    return LIB_PATH
+        raise RuntimeError("hipcc failed: " + " ".join(cmd[-3:]))
+
+
+def build(verbose: bool = False, out: str = LIB_PATH, extra_flags=()) -> str:
+    obj_dir = os.path.join(os.path.dirname(out), "obj")
+    os.makedirs(obj_dir, exist_ok=True)
+    objs = [os.path.join(obj_dir, os.path.splitext(s)[0] + ".o") for s in SOURCES]
+    cmds = [["hipcc", *HIPCC_FLAGS, *extra_flags, "-c", os.path.join(CSRC, s), "-o", o]
+            for s, o in zip(SOURCES, objs)]
+    with ThreadPoolExecutor(max_workers=len(cmds)) as pool:
+        list(pool.map(lambda c: _run(c, verbose), cmds))
+    tmp = out + ".tmp"
+    _run(["hipcc", "-shared", f"--offload-arch={ARCH}", *objs, "-o", tmp], verbose)
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

@@ -270,7 +270,7 @@ __global__ void __launch_bounds__(64) row_fast_kernel(RowArgs a, const float2* _
 // `b`. Ends with a barrier, so the LDS is free again.
 template <int N, int NT>
 __device__ __forceinline__ void detect_block(const float2 (&v)[8], const DetPlan& d, long long pos, long long b, int iy,
-                                             int i, int tid, float2* lds) {
+                                             int i, int tid, float2* lds, int tile) {
   constexpr int T = N / 8;
   double I[8];
 #pragma unroll
@@ -318,7 +318,7 @@ __device__ __forceinline__ void detect_block(const float2 (&v)[8], const DetPlan
   if (tid < d.nslots) {
     double t = 0.0;
     for (int wv = 0; wv < NT / 64; ++wv) t += red[wv * kMaxSlots + tid];
-    d.partial[(b * d.nslots + tid) * d.blocks_per_wave + blockIdx.x] = t;
+    d.partial[(b * d.nslots + tid) * d.blocks_per_wave + tile] = t;
   }
   __syncthreads();  // `red` is read before the next transform reuses the LDS
 }
@@ -365,7 +365,7 @@ __global__ void __launch_bounds__(kColTile * (N / 8)) col_fast_kernel(ColArgs a,
       for (int r = 0; r < 8; ++r) v[r] = cmul(v[r], pv[r]);
     }
     if constexpr ((FLAGS & COL_DETECT) != 0) {
-      detect_block<N, NT>(v, a.det, first_position(a.pos0, a.pos_dev) + b, b, iy, i, tid, lds);
+      detect_block<N, NT>(v, a.det, first_position(a.pos0, a.pos_dev) + b, b, iy, i, tid, lds, blockIdx.x);
       continue;
     }
     if constexpr ((FLAGS & COL_INV) != 0) fft_reg<N, +1>(v, lds, i, w, idx);
@@ -375,6 +375,19 @@ __global__ void __launch_bounds__(kColTile * (N / 8)) col_fast_kernel(ColArgs a,
       for (int r = 0; r < 8; ++r) g[(long long)(i + T * r) * a.nc] = v[r];
     }
   }
+}
+
+// Work-group order for tiles narrower than a 128-byte line: the hardware deals work-groups round-robin over the 8 XCDs
+// (each with its own L2), so consecutive block ids land on different XCDs. Renumbering XCD-major makes the blocks
+// that one XCD runs back to back take consecutive tiles, so the two 8-column tiles of a line share that XCD's L2.
+// A bijection of the grid whenever its size is a multiple of 8 (otherwise the order is left alone).
+__device__ __forceinline__ void xcd_pair_order(int& bx, int& by) {
+  const int gx = gridDim.x, total = gx * gridDim.y;
+  if (total % 8 != 0) return;
+  const int L = bx + gx * by, per = total / 8;
+  const int j = (L % 8) * per + L / 8;
+  bx = j % gx;
+  by = j / gx;
 }
 
 // Column pass of 512-point columns, one column per wavefront. The block's TC columns move between HBM and a
@@ -392,10 +405,12 @@ __global__ void __launch_bounds__(TC * 64) col512_kernel(ColArgs a, const float2
   __shared__ double red[(NT / 64) * kMaxSlots];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int cb = tid % TC, xi = tid / TC;  // block phases: column cb of the tile, rows xi + 64 r
-  const int c0 = blockIdx.x * TC;          // the host launches nc / TC blocks along x: every column is in range
+  int bx = blockIdx.x, by = blockIdx.y;
+  if constexpr (TC == 8) xcd_pair_order(bx, by);
+  const int c0 = bx * TC;                  // the host launches nc / TC blocks along x: every column is in range
   const int iyb = a.keep[c0 + cb];
   const int iyw = a.keep[c0 + wave];       // the wavefront's column
-  const long long b0 = (long long)blockIdx.y * PB;
+  const long long b0 = (long long)by * PB;
   const long long blim = b0 + PB < batch ? b0 + PB : batch;
   const long long gw = (long long)a.nx * a.nc;
   auto tile_slot = [](int x, int c) { return sk::tile_slot<TC>(x, c); };
@@ -438,7 +453,7 @@ __global__ void __launch_bounds__(TC * 64) col512_kernel(ColArgs a, const float2
     }
     if constexpr ((FLAGS & COL_DETECT) != 0) {
       detect_block<N, NT>(v, a.det, first_position(a.pos0, a.pos_dev) + b, b, iyw, lane, tid,
-                          reinterpret_cast<float2*>(red));
+                          reinterpret_cast<float2*>(red), bx);
       continue;  // detect_block ends with a barrier: the tile is free again
     }
     if constexpr ((FLAGS & COL_INV) != 0) fft_reg<N, +1, true>(v, tile, lane, w, widx);
@@ -497,7 +512,7 @@ __global__ void __launch_bounds__(kColTile * (N / 8)) col_pipe_kernel(ColArgs a,
         for (int r = 0; r < 8; ++r) g[(long long)(i + T * r) * a.nc] = v[r];
         continue;
       }
-      detect_block<N, NT>(v, a.det, (long long)d.pos + local, b, iy, i, tid, lds);
+      detect_block<N, NT>(v, a.det, (long long)d.pos + local, b, iy, i, tid, lds, blockIdx.x);
     }
     if ((d.mode == LANE_FINAL_INIT || d.mode == LANE_INIT) && local < d.next_count) {
       const long long q = (long long)d.next_pos + local;

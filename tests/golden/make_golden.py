@@ -6,7 +6,7 @@ The reference ships its parity data as ``*-pins.json`` (scalars, cases, an index
 This script copies the arrays the multislice/STEM parity tests need into compressed ``.npz`` files and the
 case descriptions into ``.json`` files beside it. It only reads data files; it runs no reference code.
 
-Run once, in the build container (``/root/reference`` does not exist on the GPU box):
+Run once, where the reference tree is mounted at ``/root/reference`` (the tests read only the committed fixtures):
 
     python tests/golden/make_golden.py
 """
