@@ -103,6 +103,8 @@ int sk_hip_potential_from_atoms(const sk_grid* grid, int nslices, const double* 
 
 void sk_hip_potential_release(sk_potential* p);
 int sk_hip_potential_slice_count(const sk_potential* p);
+/* The grid a potential was prepared on (ProjectedPotential.grid, ProjectedPotential.swift:13-40). */
+int sk_hip_potential_grid(const sk_potential* p, sk_grid* grid);
 
 /* Test hooks (Multislice.gpuPotentialSlices, MetalMultislice.swift:104-107): V of every slice as built on the
  * device [nslices x nx x ny] (atoms form only), and the transmission functions as stored (complex64 widened). */
@@ -150,7 +152,7 @@ int sk_hip_detector_shape(const sk_grid* grid, double energy, const sk_detector*
 
 /* Device time of the dominant kernels of the last sk_hip_scan on the calling thread, measured with HIP events on
  * the library's stream: mean milliseconds per launch of the row pass and of the column pass, their launch counts,
- * and the bytes each launch moved by the algorithmic definition (complex64 wave read + written once). */
+ * and the bytes of the stored compact waves each launch reads and writes once. */
 typedef struct sk_timing {
   double row_ms, col_ms;
   long long row_launches, col_launches;
@@ -159,6 +161,26 @@ typedef struct sk_timing {
 } sk_timing;
 int sk_hip_last_timing(sk_timing* t);
 void sk_hip_set_timing(int enabled);
+
+/* --- autotuning ---------------------------------------------------------------------------------------------- */
+
+/* Build-only (no counterpart in the reference; it sets the knobs of ExecutionOptions, ExecutionOptions.swift:21-49):
+ * times sk_hip_scan over `npos` positions spread over the potential's extent for each candidate execution plan
+ * (positions per launch, column tile width at 512, hipGraph replay) on this device and returns the fastest. The
+ * candidates change rounding only, never which numbers are computed. npos <= 0 = 4096. */
+#define SK_TUNE_MAX 32
+typedef struct sk_tune_report {
+  sk_options best;
+  double best_rate;                   /* position-slices per second of `best` */
+  int ncandidates;
+  sk_options candidates[SK_TUNE_MAX];
+  double rates[SK_TUNE_MAX];          /* position-slices per second of each candidate */
+  char device[64];                    /* the device's name and compute-unit count, e.g. "gfx950 256 CU" */
+} sk_tune_report;
+int sk_hip_autotune(const sk_potential* const* pots, int npots, const sk_probe* probe, const sk_detector* dets,
+                    int ndets, int npos, sk_tune_report* report);
+/* Build-only: the device string sk_tune_report.device holds (the key under which a caller may keep tuned plans). */
+int sk_hip_device_name(char* name, int len);
 
 #ifdef __cplusplus
 }

@@ -12,9 +12,11 @@ from .detector import (STEM, AnnularDetector, CenterOfMassDetector, PixelatedDet
 from .multislice import (ExitPlanes, FrozenPhonons, GaussianGenerator, LobatoParameterization, Multislice, Potential,
                          PreparedPotential, ProjectedPotential, Slicing, hip_is_available)
 from .probe import Aberrations, Aperture, GridScan, Probe
+from .tune import TuneResult, autotune
 
 __all__ = ["AtomicStructure", "Backend", "ElectronBeam", "ExecutionOptions", "Grid", "HipAccuracy", "Normalization",
            "REGISTERED", "SimulatoError", "Waves", "STEM", "AnnularDetector", "CenterOfMassDetector",
            "PixelatedDetector", "ScanMeasurement", "ExitPlanes", "FrozenPhonons", "GaussianGenerator",
            "LobatoParameterization", "Multislice", "Potential", "PreparedPotential", "ProjectedPotential", "Slicing",
-           "hip_is_available", "Aberrations", "Aperture", "GridScan", "Probe", "last_timing", "set_timing"]
+           "hip_is_available", "Aberrations", "Aperture", "GridScan", "Probe", "last_timing", "set_timing",
+           "TuneResult", "autotune"]

@@ -42,6 +42,15 @@ class SkTiming(C.Structure):
                 ("total_ms", C.c_double)]
 
 
+SK_TUNE_MAX = 32
+
+
+class SkTuneReport(C.Structure):
+    _fields_ = [("best", SkOptions), ("best_rate", C.c_double), ("ncandidates", C.c_int),
+                ("candidates", SkOptions * SK_TUNE_MAX), ("rates", C.c_double * SK_TUNE_MAX),
+                ("device", C.c_char * 64)]
+
+
 _lib = None
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -75,6 +84,10 @@ def lib():
         L.sk_hip_last_timing.argtypes = [C.POINTER(SkTiming)]
         L.sk_hip_set_timing.argtypes = [C.c_int]
         L.sk_hip_set_timing.restype = None
+        L.sk_hip_potential_grid.argtypes = [_vp, C.POINTER(SkGrid)]
+        L.sk_hip_device_name.argtypes = [C.c_char_p, C.c_int]
+        L.sk_hip_autotune.argtypes = [C.POINTER(_vp), C.c_int, C.POINTER(SkProbe), C.POINTER(SkDetector), C.c_int,
+                                      C.c_int, C.POINTER(SkTuneReport)]
         _lib = L
     return _lib
 
@@ -94,4 +107,5 @@ def iptr(a: np.ndarray):
 EXPORTED = ("sk_hip_available", "sk_hip_last_error", "sk_hip_potential_from_values", "sk_hip_potential_from_atoms",
             "sk_hip_potential_release", "sk_hip_potential_slice_count", "sk_hip_potential_values",
             "sk_hip_potential_transmissions", "sk_hip_run", "sk_hip_run_batch", "sk_hip_probes", "sk_hip_scan",
-            "sk_hip_detector_shape", "sk_hip_last_timing", "sk_hip_set_timing")
+            "sk_hip_detector_shape", "sk_hip_last_timing", "sk_hip_set_timing", "sk_hip_potential_grid",
+            "sk_hip_autotune", "sk_hip_device_name")

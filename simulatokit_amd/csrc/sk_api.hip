@@ -35,6 +35,17 @@ struct sk_potential {
 };
 
 namespace sk {
+
+std::recursive_mutex& api_lock() {
+  static std::recursive_mutex* m = new std::recursive_mutex;  // process lifetime, as the context
+  return *m;
+}
+
+std::string& last_error() {
+  thread_local std::string e;
+  return e;
+}
+
 namespace {
 
 // ---- process context ------------------------------------------------------------------------------------------
@@ -52,10 +63,7 @@ struct Context {
   std::map<int, std::unique_ptr<DevBuf<int>>> identity;
 };
 
-std::recursive_mutex& lock() {
-  static std::recursive_mutex m;
-  return m;
-}
+std::recursive_mutex& lock() { return api_lock(); }
 
 Context& ctx() {
   // process lifetime on purpose: never destroyed, so no HIP call runs after the runtime has shut down
@@ -63,7 +71,6 @@ Context& ctx() {
   return *c;
 }
 
-thread_local std::string g_error;
 
 struct LobatoRow {
   int present;
@@ -112,17 +119,7 @@ Context& require() {
 
 template <class F>
 int guarded(F&& f) {
-  std::lock_guard<std::recursive_mutex> g(lock());
-  try {
-    f();
-    return SK_OK;
-  } catch (const Error& e) {
-    g_error = e.what();
-    return e.code;
-  } catch (const std::exception& e) {
-    g_error = e.what();
-    return SK_ERR_DEVICE;
-  }
+  return guarded_call(std::forward<F>(f));
 }
 
 // ---- host geometry (Grid.swift, Antialias.swift, Detectors.swift) -----------------------------------------------
@@ -724,12 +721,12 @@ int sk_hip_available(void) {
     require();
     return 1;
   } catch (const Error& e) {
-    g_error = e.what();
+    last_error() = e.what();
     return 0;
   }
 }
 
-const char* sk_hip_last_error(void) { return g_error.c_str(); }
+const char* sk_hip_last_error(void) { return last_error().c_str(); }
 
 void sk_hip_set_timing(int enabled) {
   std::lock_guard<std::recursive_mutex> g(lock());
@@ -883,6 +880,12 @@ void sk_hip_potential_release(sk_potential* p) {
 }
 
 int sk_hip_potential_slice_count(const sk_potential* p) { return p ? p->nslices : SK_ERR_INVALID; }
+
+int sk_hip_potential_grid(const sk_potential* p, sk_grid* grid) {
+  if (!p || !grid) return SK_ERR_INVALID;
+  *grid = p->grid;
+  return SK_OK;
+}
 
 int sk_hip_potential_values(const sk_potential* p, double* V) {
   return guarded([&] {

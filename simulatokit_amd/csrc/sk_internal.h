@@ -23,6 +23,27 @@ struct Error : std::runtime_error {
 
 [[noreturn]] inline void fail(int code, const std::string& m) { throw Error(code, m); }
 
+// The C ABI's process-wide lock (one call at a time, as the reference's runLock) and the calling thread's last error
+// message (sk_api.hip).
+std::recursive_mutex& api_lock();
+std::string& last_error();
+
+// Runs f under the lock, turning a thrown Error into its status code and message.
+template <class F>
+int guarded_call(F&& f) {
+  std::lock_guard<std::recursive_mutex> g(api_lock());
+  try {
+    f();
+    return SK_OK;
+  } catch (const Error& e) {
+    last_error() = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    last_error() = e.what();
+    return SK_ERR_DEVICE;
+  }
+}
+
 #define SK_HIP(call)                                                                                         \
   do {                                                                                                       \
     hipError_t e_ = (call);                                                                                  \

@@ -78,19 +78,23 @@ def random_cell(seed: int, extent: float, thickness: float, natoms: int, element
 
 def uneven_frozen_phonon_stack(seed: int = 20261202, extent: float = 40.0, natoms: int = 2000,
                                elements=(8, 14, 22, 38, 79), configurations: int = 16, t_min: float = 40.0,
-                               t_max: float = 120.0):
+                               t_max: float = 120.0, sigma: float = 0.08):
     """The frozen-phonon workload with uneven slice counts: one master cell of depth t_max with `natoms` atoms
-    (uniform positions, seeded), and per configuration a thickness T_c = t_min + (t_max - t_min) u_c (u_c from the
-    same seed); configuration c keeps the atoms with z < T_c in a box of depth T_c. Returns (structures, T)."""
-    rng = np.random.default_rng(seed)
+    (uniform positions, seeded), displaced per configuration as FrozenPhonons(seed:) does (sigma on every axis), and a
+    thickness T_c = t_min + (t_max - t_min) u_c per configuration (u_c from seed + 1); configuration c keeps the
+    displaced atoms with 0 <= z < T_c in a box of depth T_c. Returns (structures, T)."""
+    from .multislice import FrozenPhonons
+
     master = random_cell(seed, extent, t_max, natoms, elements)
-    u = np.random.default_rng(seed + 1).uniform(0, 1, configurations)
-    T = t_min + (t_max - t_min) * u
+    fp = FrozenPhonons(master, sigmas={int(z): sigma for z in np.unique(master.numbers)},
+                       configurations=configurations, seed=seed)
+    T = t_min + (t_max - t_min) * np.random.default_rng(seed + 1).uniform(0, 1, configurations)
     out = []
     for c in range(configurations):
-        keep = master.positions[:, 2] < T[c]
-        out.append(AtomicStructure((extent, extent, float(T[c])), master.numbers[keep], master.positions[keep]))
-    _ = rng
+        moved = fp.configuration(c)
+        z = moved.positions[:, 2]
+        keep = (z >= 0) & (z < T[c])
+        out.append(AtomicStructure((extent, extent, float(T[c])), moved.numbers[keep], moved.positions[keep]))
     return out, T
 
 
