@@ -101,6 +101,15 @@ int sk_hip_potential_from_atoms(const sk_grid* grid, int nslices, const double* 
                                 const double* xy, const int* slice_of_atom, int pi_mode, double energy,
                                 const double tilt[2], sk_potential** out);
 
+/* Build-only (abTEM's Potential(projection="finite"); SimulatoKit has none, docs/parity.md:58): as
+ * sk_hip_potential_from_atoms, but slice j holds the 3-D Lobato potential of every atom integrated over the slice's
+ * own depth range [z_j, z_j + dz_j] only (z from the entrance surface, not periodic), so an atom near a boundary
+ * feeds both neighbours. xyz [natoms x 3] in Angstrom, wrapped into the box. Built on the device in float64 per
+ * frequency; with every atom deep inside its slice it equals the infinite projection. */
+int sk_hip_potential_from_atoms_finite(const sk_grid* grid, int nslices, const double* dz, int natoms,
+                                       const int* numbers, const double* xyz, int pi_mode, double energy,
+                                       const double tilt[2], sk_potential** out);
+
 void sk_hip_potential_release(sk_potential* p);
 int sk_hip_potential_slice_count(const sk_potential* p);
 /* The grid a potential was prepared on (ProjectedPotential.grid, ProjectedPotential.swift:13-40). */

@@ -125,6 +125,20 @@ def potential(nx, ny, numbers, xyz, box, thicknesses, pi_mode=0):
     return V, s
 
 
+def potential_finite(nx, ny, numbers, xyz, box, thicknesses, pi_mode=0):
+    """The finite projection (abTEM's Potential(projection="finite"); none in the reference): V [nslices, nx, ny],
+    each slice the 3-D Lobato potential integrated over its own z range only."""
+    numbers = np.ascontiguousarray(numbers, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    box = np.ascontiguousarray(box, dtype=np.float64)
+    th = np.ascontiguousarray(thicknesses, dtype=np.float64)
+    V = np.empty((len(th), nx, ny))
+    _check(lib().orc_potential_finite(C.c_int(nx), C.c_int(ny), C.c_double(box[0]), C.c_double(box[1]),
+                                      C.c_int(len(numbers)), _i(numbers), _d(xyz), _d(box), C.c_int(len(th)),
+                                      _d(th), C.c_int(pi_mode), _d(V)))
+    return V
+
+
 def transmission(V, extent, energy):  # Multislice.swift:83-103
     V = np.ascontiguousarray(V, dtype=np.float64)
     ns, nx, ny = V.shape

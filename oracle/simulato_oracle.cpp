@@ -20,6 +20,7 @@
 //   GaussianGenerator                         published SplitMix64 / xoshiro256** known answers + statistics
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <complex>
@@ -861,6 +862,97 @@ int orc_potential(int nx, int ny, double ex, double ey, int natoms, const int* Z
       if (s[a] == j) in.push_back(w[a]);
     auto v = sb.potential(in);
     std::copy(v.begin(), v.end(), V + (size_t)j * g.count());
+  }
+  return 0;
+}
+
+// Finite projection (abTEM's Potential(projection="finite"); SimulatoKit has none, docs/parity.md:58 lists it as
+// missing). Slice j holds the Lobato 3-D potential V3(r) = sum_i A_i (2/(B_i r) + 1) e^{-B_i r} of every atom
+// integrated over z in [z0_j, z1_j] only, instead of over all z. With t0 = z0_j - z_a, t1 = z1_j - z_a, the in-plane
+// transform of that integral is F(k) = G(t1) - G(t0), G odd in T:
+//   G(T) = sgn(T) [f(k)/2 - c(k, |T|)],
+//   c(k, T) = 2 pi sum_i A_i e^{-T kappa_i} [2 / (B_i kappa_i^2) + B_i (2 + T kappa_i) / kappa_i^4],
+//   kappa_i = sqrt(B_i^2 + 4 pi^2 k^2),
+// from the 2-D transform at height t of e^{-B r}/r (2 pi e^{-|t| kappa}/kappa) and of e^{-B r} (its -d/dB), integrated
+// over t; c(k, 0) = f(k)/2 and G(+inf) = f(k)/2, so an atom deep inside a slab gets its infinite projection f(k).
+// In-plane, each atom enters as the infinite projection does (SliceBuilder): its four bilinear deltas, transformed,
+// divided by the same S(k); so V_j = Re IFFT(sum_a FFT(delta_a) F_{a,j} / S), and with F = f for the atoms of slice
+// j this is exactly the infinite slice. z is not periodic: tails beyond the stack are dropped.
+double lobato_slab(const double* A, const double* B, double pi, double q, double t0, double t1) {
+  double f = 0, c0 = 0, c1 = 0;
+  auto c = [&](int i, double k2, double kap, double T) {
+    return A[i] * std::exp(-T * kap) * (2 / (B[i] * k2) + B[i] * (2 + T * kap) / (k2 * k2));
+  };
+  for (int i = 0; i < 5; ++i) {
+    const double k2 = B[i] * B[i] + q, kap = std::sqrt(k2);
+    f += A[i] / B[i] / k2 + A[i] * B[i] / (k2 * k2);
+    c0 += c(i, k2, kap, std::fabs(t0));
+    c1 += c(i, k2, kap, std::fabs(t1));
+  }
+  f *= 8 * pi;
+  c0 *= 2 * pi;
+  c1 *= 2 * pi;
+  auto sgn = [](double v) { return double((v > 0) - (v < 0)); };
+  return sgn(t1) * (f / 2 - c1) - sgn(t0) * (f / 2 - c0);
+}
+
+int orc_potential_finite(int nx, int ny, double ex, double ey, int natoms, const int* Z, const double* xyz,
+                         const double* box, int nslices, const double* thick, int pi_mode, double* V) {
+  for (int a = 0; a < natoms; ++a)
+    if (!lobato_supports(Z[a])) { g_err = "unsupported element"; return -1; }
+  Grid g{nx, ny, ex, ey};
+  std::vector<Atom> atoms(natoms);
+  for (int a = 0; a < natoms; ++a) atoms[a] = Atom{Z[a], xyz[3 * a], xyz[3 * a + 1], xyz[3 * a + 2]};
+  auto w = wrapped(atoms, box);
+  SliceBuilder sb(g, pi_mode);  // for S(k)
+  FFT2 fft(nx, ny);
+  const double pi = pi_mode == 1 ? double(float(kPi)) : kPi;
+  const double pi2 = pi_mode == 1 ? double(float(kPi * kPi)) : kPi * kPi;
+  const double sx = g.sx(), sy = g.sy();
+  auto kx = g.kx(), ky = g.ky();
+  // per atom: Lobato A, B and the transform of its bilinear deltas (the four corner phases, weighted)
+  std::vector<std::array<double, 10>> AB(natoms);
+  std::vector<cd> D((size_t)natoms * g.count());
+  for (int a = 0; a < natoms; ++a) {
+    const LobatoRow& p = kLobato[w[a].z - 1];
+    for (int i = 0; i < 5; ++i) {
+      AB[a][i] = kPi * kPi * p.a[i] / std::pow(p.b[i], 1.5) / kKappa;
+      AB[a][5 + i] = 2 * kPi / std::sqrt(p.b[i]);
+    }
+    const double px = w[a].x / sx, py = w[a].y / sy;
+    const int r = (int)std::floor(px), c = (int)std::floor(py);
+    const double x = px - r, y = py - c, xy = x * y;
+    const double wt[4] = {1 + xy - y - x, x - xy, y - xy, xy};
+    const int cx[4] = {r, r + 1, r, r + 1}, cy[4] = {c, c, c + 1, c + 1};
+    for (int ix = 0; ix < nx; ++ix)
+      for (int iy = 0; iy < ny; ++iy) {
+        cd acc(0, 0);
+        for (int q = 0; q < 4; ++q) {
+          // the grid index of the corner modulo n (as the delta lands) times the frequency index, modulo n
+          const long long mx = ((long long)cx[q] % nx + nx) % nx, my = ((long long)cy[q] % ny + ny) % ny;
+          acc += wt[q] * root_of_unity((mx * ix) % nx * ny + (my * iy) % ny * nx, (long long)nx * ny, -1);
+        }
+        D[(size_t)a * g.count() + (size_t)ix * ny + iy] = acc;
+      }
+  }
+  double z0 = 0;
+  std::vector<cd> field(g.count());
+  for (int j = 0; j < nslices; ++j) {
+    const double z1 = z0 + thick[j];
+    std::fill(field.begin(), field.end(), cd(0, 0));
+    for (int a = 0; a < natoms; ++a) {
+      const double t0 = z0 - w[a].zc, t1 = z1 - w[a].zc;
+      for (int ix = 0; ix < nx; ++ix)
+        for (int iy = 0; iy < ny; ++iy) {
+          const size_t i = (size_t)ix * ny + iy;
+          const double q = 4 * pi2 * (kx[ix] * kx[ix] + ky[iy] * ky[iy]);
+          const double F = lobato_slab(AB[a].data(), AB[a].data() + 5, pi, q, t0, t1);
+          field[i] += D[(size_t)a * g.count() + i] * (F / sb.comp[i]);
+        }
+    }
+    fft.inverse(field.data());
+    for (size_t i = 0; i < field.size(); ++i) V[(size_t)j * g.count() + i] = field[i].real();
+    z0 = z1;
   }
   return 0;
 }

@@ -70,6 +70,8 @@ def lib():
                                                    C.POINTER(_vp)]
         L.sk_hip_potential_from_atoms.argtypes = [C.POINTER(SkGrid), C.c_int, _dp, C.c_int, _ip, _dp, _ip, C.c_int,
                                                   C.c_double, _dp, C.POINTER(_vp)]
+        L.sk_hip_potential_from_atoms_finite.argtypes = [C.POINTER(SkGrid), C.c_int, _dp, C.c_int, _ip, _dp,
+                                                         C.c_int, C.c_double, _dp, C.POINTER(_vp)]
         L.sk_hip_potential_release.argtypes = [_vp]
         L.sk_hip_potential_release.restype = None
         L.sk_hip_potential_slice_count.argtypes = [_vp]
@@ -108,4 +110,4 @@ EXPORTED = ("sk_hip_available", "sk_hip_last_error", "sk_hip_potential_from_valu
             "sk_hip_potential_release", "sk_hip_potential_slice_count", "sk_hip_potential_values",
             "sk_hip_potential_transmissions", "sk_hip_run", "sk_hip_run_batch", "sk_hip_probes", "sk_hip_scan",
             "sk_hip_detector_shape", "sk_hip_last_timing", "sk_hip_set_timing", "sk_hip_potential_grid",
-            "sk_hip_autotune", "sk_hip_device_name")
+            "sk_hip_autotune", "sk_hip_device_name", "sk_hip_potential_from_atoms_finite")

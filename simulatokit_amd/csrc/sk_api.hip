@@ -874,6 +874,107 @@ int sk_hip_potential_from_atoms(const sk_grid* grid, int nslices, const double* 
   });
 }
 
+int sk_hip_potential_from_atoms_finite(const sk_grid* grid, int nslices, const double* dz, int natoms,
+                                       const int* numbers, const double* xyz, int pi_mode, double energy,
+                                       const double tilt[2], sk_potential** out) {
+  return guarded([&] {
+    Context& c = require();
+    hipStream_t s = c.stream;
+    if (!out || !dz || (natoms > 0 && (!numbers || !xyz))) fail(SK_ERR_INVALID, "null argument");
+    auto p = new_potential(grid, nslices, dz, energy, tilt);
+    const int nx = grid->nx, ny = grid->ny;
+    const size_t N = (size_t)nx * ny;
+    const double sx = grid->extent_x / nx, sy = grid->extent_y / ny;
+    std::vector<int> zs;
+    for (int a = 0; a < natoms; ++a) {
+      if (numbers[a] < 1 || numbers[a] > 103 || !kLobato[numbers[a] - 1].present)
+        fail(SK_ERR_INVALID, "element Z = " + std::to_string(numbers[a]) + " has no Lobato parameters");
+      for (int d = 0; d < 3; ++d)
+        if (!std::isfinite(xyz[3 * a + d])) fail(SK_ERR_INVALID, "non-finite position");
+      zs.push_back(numbers[a]);
+    }
+    std::sort(zs.begin(), zs.end());
+    zs.erase(std::unique(zs.begin(), zs.end()), zs.end());
+    std::vector<int> zindex(104, -1);
+    for (size_t e = 0; e < zs.size(); ++e) zindex[zs[e]] = (int)e;
+    // Lobato A, B per element; the z range beyond which an atom's share of a slice is below 1e-11 of its total:
+    // c(k, T) decays as e^{-T kappa} with kappa >= min B
+    std::vector<double> AB(10 * std::max<size_t>(1, zs.size()));
+    double tcut = 0;
+    const double kappa = 0.0208865737082965;
+    for (size_t e = 0; e < zs.size(); ++e) {
+      const LobatoRow& row = kLobato[zs[e] - 1];
+      double bmin = 1e300;
+      for (int i = 0; i < 5; ++i) {
+        AB[10 * e + i] = M_PI * M_PI * row.a[i] / std::pow(row.b[i], 1.5) / kappa;
+        AB[10 * e + 5 + i] = 2 * M_PI / std::sqrt(row.b[i]);
+        bmin = std::min(bmin, AB[10 * e + 5 + i]);
+      }
+      tcut = std::max(tcut, 30.0 / bmin);
+    }
+    // atoms sorted by depth, with their bilinear corner and weights (ProjectedPotential.swift:178-200)
+    std::vector<int> order(natoms);
+    for (int a = 0; a < natoms; ++a) order[a] = a;
+    std::stable_sort(order.begin(), order.end(), [&](int u, int v) { return xyz[3 * u + 2] < xyz[3 * v + 2]; });
+    std::vector<int> r(std::max(1, natoms)), cc(std::max(1, natoms)), el(std::max(1, natoms));
+    std::vector<double> w(4 * (size_t)std::max(1, natoms)), zz(std::max(1, natoms));
+    for (int k = 0; k < natoms; ++k) {
+      const int a = order[k];
+      const double px = xyz[3 * a] / sx, py = xyz[3 * a + 1] / sy;
+      const double fr = std::floor(px), fc = std::floor(py);
+      const double x = px - fr, y = py - fc, xyv = x * y;
+      r[k] = (int)(((long long)fr % nx + nx) % nx);
+      cc[k] = (int)(((long long)fc % ny + ny) % ny);
+      w[4 * k] = 1 + xyv - y - x;
+      w[4 * k + 1] = x - xyv;
+      w[4 * k + 2] = y - xyv;
+      w[4 * k + 3] = xyv;
+      zz[k] = xyz[3 * a + 2];
+      el[k] = zindex[numbers[a]];
+    }
+    std::vector<double> z0(nslices), z1(nslices);
+    std::vector<int> lo(nslices), hi(nslices);
+    double acc = 0;
+    for (int j = 0; j < nslices; ++j) {
+      z0[j] = acc;
+      acc += dz[j];
+      z1[j] = acc;
+      lo[j] = int(std::lower_bound(zz.begin(), zz.begin() + natoms, z0[j] - tcut) - zz.begin());
+      hi[j] = int(std::upper_bound(zz.begin(), zz.begin() + natoms, z1[j] + tcut) - zz.begin());
+    }
+    DevBuf<int> r_d, c_d, e_d, lo_d, hi_d;
+    DevBuf<double> w_d, z_d, ab_d, z0_d, z1_d;
+    r_d.upload(r, s);
+    c_d.upload(cc, s);
+    e_d.upload(el, s);
+    w_d.upload(w, s);
+    z_d.upload(zz, s);
+    ab_d.upload(AB, s);
+    FiniteAtoms at{r_d.p, c_d.p, w_d.p, z_d.p, e_d.p, ab_d.p};
+    DevBuf<double> Vd;
+    Vd.alloc(N * nslices);
+    const int chunk = std::max<int>(1, (int)std::min<size_t>(nslices, (size_t(1) << 30) / (N * sizeof(float2))));
+    DevBuf<float2> Sp;
+    Sp.alloc((size_t)chunk * N);
+    for (int j0 = 0; j0 < nslices; j0 += chunk) {
+      const int cnt = std::min(chunk, nslices - j0);
+      z0_d.upload(z0.data() + j0, cnt, s);
+      z1_d.upload(z1.data() + j0, cnt, s);
+      lo_d.upload(lo.data() + j0, cnt, s);
+      hi_d.upload(hi.data() + j0, cnt, s);
+      launch_finite_slices(nx, ny, grid->extent_x, grid->extent_y, pi_mode, at, z0_d.p, z1_d.p, lo_d.p, hi_d.p, cnt,
+                           Sp.p, s);
+      fft2_batch(Sp.p, nx, ny, cnt, +1, nullptr, float(1.0 / double(N)), s);
+      launch_real_part(Sp.p, Vd.p + (size_t)j0 * N, (size_t)cnt * N, s);
+      SK_HIP(hipStreamSynchronize(s));
+    }
+    finish_potential(*p, Vd.p);
+    p->V = std::move(Vd);
+    p->has_values = true;
+    *out = p.release();
+  });
+}
+
 void sk_hip_potential_release(sk_potential* p) {
   std::lock_guard<std::recursive_mutex> g(lock());
   delete p;

@@ -271,6 +271,18 @@ void launch_lobato_kernels(int nx, int ny, double ex, double ey, const int* zs, 
 void launch_combine_elements(const float2* D, const float* K, int nslices, int nz, size_t N, float2* out,
                              hipStream_t s);
 void launch_real_part(const float2* in, double* out, size_t n, hipStream_t s);
+// finite projection: atoms sorted by z, with their bilinear corner (r, c), weights w[4], depth z, element index e,
+// and per element the Lobato A[5], B[5]
+struct FiniteAtoms {
+  const int* r;
+  const int* c;
+  const double* w;
+  const double* z;
+  const int* e;
+  const double* AB;
+};
+void launch_finite_slices(int nx, int ny, double ex, double ey, int pi_mode, const FiniteAtoms& at, const double* z0,
+                          const double* z1, const int* lo, const int* hi, int cnt, float2* out, hipStream_t s);
 void launch_finalise_outputs(const double* acc, size_t n, double inv, void* out, int f32, hipStream_t s);
 void launch_gather_scalar(const double* acc, int nslots, int slot, int npos, double inv, void* out, int stride,
                           int f32, hipStream_t s);

@@ -311,6 +311,76 @@ void launch_lobato_kernels(int nx, int ny, double ex, double ey, const int* zs, 
   SK_HIP(hipGetLastError());
 }
 
+// Finite projection (abTEM's Potential(projection="finite"); no counterpart in the reference): the spectrum of slice
+// s is sum_a FFT(bilinear deltas of atom a)(k) * F_{a,s}(k) / S(k), with F the in-plane transform of the atom's 3-D
+// Lobato potential V3(r) = sum_i A_i (2/(B_i r) + 1) e^{-B_i r} integrated over the slice's z range [z0, z1]:
+//   F = G(z1 - z_a) - G(z0 - z_a),  G(T) = sgn(T) [f(k)/2 - c(k, |T|)],
+//   c(k, T) = 2 pi sum_i A_i e^{-T kappa_i} [2 / (B_i kappa_i^2) + B_i (2 + T kappa_i) / kappa_i^4],
+//   kappa_i = sqrt(B_i^2 + 4 pi^2 k^2)  (f(k) = 2 G(+inf), the infinite projection).
+// One thread per frequency and slice, float64 throughout; the atoms of a slice are those within the host's z cutoff
+// (atoms sorted by z, [lo, hi) per slice). The delta transform is the atom's corner phase times
+// w0 + w1 e^{-2 pi i ix/nx} + w2 e^{-2 pi i iy/ny} + w3 e^{-2 pi i (ix/nx + iy/ny)}, the corner phase taken from integer
+// products reduced modulo the grid, as the deltas land modulo the grid.
+__device__ double lobato_slab(const double* AB, double pi, double q, double t0, double t1) {
+  double f = 0, c0 = 0, c1 = 0;
+  const double T0 = fabs(t0), T1 = fabs(t1);
+  for (int i = 0; i < 5; ++i) {
+    const double A = AB[i], B = AB[5 + i];
+    const double k2 = B * B + q, kap = sqrt(k2), k4 = k2 * k2;
+    f += A / B / k2 + A * B / k4;
+    c0 += A * exp(-T0 * kap) * (2 / (B * k2) + B * (2 + T0 * kap) / k4);
+    c1 += A * exp(-T1 * kap) * (2 / (B * k2) + B * (2 + T1 * kap) / k4);
+  }
+  f *= 8 * pi;
+  c0 *= 2 * pi;
+  c1 *= 2 * pi;
+  const double s1 = double((t1 > 0) - (t1 < 0)), s0 = double((t0 > 0) - (t0 < 0));
+  return s1 * (f / 2 - c1) - s0 * (f / 2 - c0);
+}
+
+__global__ void finite_slices_kernel(int nx, int ny, double ex, double ey, int pi_mode, FiniteAtoms at,
+                                     const double* z0, const double* z1, const int* lo, const int* hi, int cnt,
+                                     float2* out) {
+  const size_t N = (size_t)nx * ny;
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const int sl = blockIdx.y;
+  if (i >= N || sl >= cnt) return;
+  const int ix = int(i / ny), iy = int(i % ny);
+  const double dx = ex / nx, dy = ey / ny;
+  const double kx = fftfreq_d(ix, nx, dx), ky = fftfreq_d(iy, ny, dy);
+  const double pi = pi_mode ? (double)(float)M_PI : M_PI;
+  const double pi2 = pi_mode ? (double)(float)(M_PI * M_PI) : M_PI * M_PI;
+  const double q = 4 * pi2 * __dadd_rn(__dmul_rn(kx, kx), __dmul_rn(ky, ky));
+  const double a = kx * dx, b = ky * dy;
+  const double kk = sqrt(__dadd_rn(__dmul_rn(a, a), __dmul_rn(b, b)));
+  const double S = (ix == 0 && iy == 0) ? dx * dy : sin(kk) / kk * (dx * dy);
+  double sxr, sxi, syr, syi;  // e^{-2 pi i ix/nx}, e^{-2 pi i iy/ny}
+  sincospi(-2.0 * ix / nx, &sxi, &sxr);
+  sincospi(-2.0 * iy / ny, &syi, &syr);
+  const double sxyr = sxr * syr - sxi * syi, sxyi = sxr * syi + sxi * syr;
+  double accr = 0, acci = 0;
+  const double zl = z0[sl], zh = z1[sl];
+  for (int k = lo[sl]; k < hi[sl]; ++k) {
+    const int rx = int(((long long)at.r[k] * ix) % nx), cy = int(((long long)at.c[k] * iy) % ny);
+    double pr, pim;
+    sincospi(-2.0 * ((double)rx / nx + (double)cy / ny), &pim, &pr);
+    const double* w = at.w + 4 * (size_t)k;
+    const double dr = w[0] + w[1] * sxr + w[2] * syr + w[3] * sxyr;
+    const double di = w[1] * sxi + w[2] * syi + w[3] * sxyi;
+    const double F = lobato_slab(at.AB + 10 * at.e[k], pi, q, zl - at.z[k], zh - at.z[k]);
+    accr += F * (pr * dr - pim * di);
+    acci += F * (pr * di + pim * dr);
+  }
+  out[(size_t)sl * N + i] = make_float2((float)(accr / S), (float)(acci / S));
+}
+void launch_finite_slices(int nx, int ny, double ex, double ey, int pi_mode, const FiniteAtoms& at, const double* z0,
+                          const double* z1, const int* lo, const int* hi, int cnt, float2* out, hipStream_t s) {
+  dim3 grid((unsigned)(((size_t)nx * ny + kThreads - 1) / kThreads), cnt);
+  hipLaunchKernelGGL(finite_slices_kernel, grid, dim3(kThreads), 0, s, nx, ny, ex, ey, pi_mode, at, z0, z1, lo, hi,
+                     cnt, out);
+  SK_HIP(hipGetLastError());
+}
+
 // Spec[s] = sum_e K_e * D[s, e]  (ProjectedPotential.swift:164-171, linearity of the inverse transform)
 __global__ void combine_elements_kernel(const float2* D, const float* K, int nslices, int nz, size_t N, float2* out) {
   size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;

@@ -233,7 +233,13 @@ class Potential:
                  slice_thickness: Optional[float] = None, slice_thicknesses: Optional[Sequence[float]] = None,
                  sampling: Optional[Union[float, Sequence[float]]] = None,
                  parameterization: LobatoParameterization = LobatoParameterization(),
-                 maximum_slice_count: int = MAXIMUM_SLICE_COUNT):
+                 maximum_slice_count: int = MAXIMUM_SLICE_COUNT, projection: str = "infinite"):
+        """`projection`: "infinite" (the reference's: each atom's whole potential in the slice that holds it) or
+        "finite" (abTEM's projection="finite", which the reference lacks: each slice integrates every atom's 3-D
+        potential over its own depth range only)."""
+        if projection not in ("infinite", "finite"):
+            raise ValueError(f"projection must be 'infinite' or 'finite', got {projection!r}")
+        self.projection = projection
         if sampling is not None:
             sx, sy = (sampling, sampling) if np.isscalar(sampling) else sampling
             g = Grid.from_sampling(structure.box[0], structure.box[1], float(sx), float(sy))
@@ -267,6 +273,12 @@ class Potential:
         num = np.ascontiguousarray(self.numbers, dtype=np.int32)
         xy = np.ascontiguousarray(self.positions[:, :2], dtype=np.float64)
         soa = np.ascontiguousarray(self.slice_of_atom, dtype=np.int32)
+        if self.projection == "finite":
+            xyz = np.ascontiguousarray(self.positions, dtype=np.float64)
+            _raise(N.lib().sk_hip_potential_from_atoms_finite(
+                C.byref(_sk_grid(self.grid)), self.slice_count, N.dptr(dz), len(num), N.iptr(num), N.dptr(xyz),
+                self.parameterization.pi_mode, beam.energy, N.dptr(t), C.byref(h)), "prepare")
+            return PreparedPotential(h.value, self.grid, beam, as_float_pair(tilt), self.slice_thicknesses)
         _raise(N.lib().sk_hip_potential_from_atoms(C.byref(_sk_grid(self.grid)), self.slice_count, N.dptr(dz),
                                                    len(num), N.iptr(num), N.dptr(xy), N.iptr(soa),
                                                    self.parameterization.pi_mode, beam.energy, N.dptr(t),
