@@ -75,7 +75,9 @@ typedef struct sk_options {
                        time; -1 = the library's choice (lanes when npots > 1 and the grid has fast passes) */
   int lane_waves;   /* pipeline: positions per lane (0 = the library's choice) */
   int col_tile;     /* 512-point columns: columns per block of the column pass, 8 or 16 (0 = the library's choice) */
-  int reserved[2];
+  int precision;    /* transforms of the slice loop: 0 = float32 (default); 1 = split-BF16 on the matrix cores
+                       (bfloat16 hi/lo pairs, three products, float32 accumulation; grid lengths 256 and 512) */
+  int reserved[1];
 } sk_options;
 
 /* --- availability (MetalEngine.require / Multislice.metalIsAvailable, MetalEngine.swift:63-77,

@@ -33,7 +33,7 @@ class SkDetector(C.Structure):
 class SkOptions(C.Structure):
     _fields_ = [("batch", C.c_int), ("use_graph", C.c_int), ("fast_path", C.c_int), ("pipeline", C.c_int),
                 ("lane_waves", C.c_int), ("col_tile", C.c_int),
-                ("reserved", C.c_int * 2)]
+                ("precision", C.c_int), ("reserved", C.c_int * 1)]
 
 
 class SkTiming(C.Structure):

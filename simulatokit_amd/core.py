@@ -206,6 +206,7 @@ class ExecutionOptions:
                              # potential at a time, -1 the library's choice (lanes when there are several potentials)
     lane_waves: int = 0      # positions per lane (0 = the library's choice)
     col_tile: int = 0        # 512-point columns: columns per column-pass block, 8 or 16 (0 = the library's choice)
+    precision: str = "float32"  # the slice loop's transforms: "float32", or "split_bf16" (matrix cores; 256, 512)
 
 
 @dataclass(frozen=True)

@@ -58,7 +58,8 @@ struct Context {
   // timing
   bool timing = false;
   sk_timing last{};
-  int col_tile = 0;  // the running call's sk_options.col_tile
+  int col_tile = 0;   // the running call's sk_options.col_tile
+  int precision = 0;  // the running call's sk_options.precision
   // identity keep maps per length (full-width transforms)
   std::map<int, std::unique_ptr<DevBuf<int>>> identity;
 };
@@ -438,6 +439,15 @@ void set_col_tile(Context& c, const sk_options* opt) {
   const int t = opt ? opt->col_tile : 0;
   if (t != 0 && t != 8 && t != 16) fail(SK_ERR_INVALID, "col_tile must be 0, 8 or 16");
   c.col_tile = t;
+  const int pr = opt ? opt->precision : 0;
+  if (pr != 0 && pr != 1) fail(SK_ERR_INVALID, "precision must be 0 (float32) or 1 (split-BF16)");
+  c.precision = pr;
+}
+
+// the transform plan of length n for the slice loop under the running call's precision
+LineFFT slice_plan(int n, hipStream_t s) {
+  if (ctx().precision == 1) return mfma_line_plan(n, s);
+  return line_plan(n, s).dev;
 }
 
 int default_batch(const sk_potential& p, int count, const sk_options* opt) {
@@ -455,7 +465,7 @@ RowArgs row_args(const sk_potential& p, float2* G) {
   r.nx = p.grid.nx;
   r.ny = p.grid.ny;
   r.nc = p.nc;
-  r.fy = line_plan(p.grid.ny, ctx().stream).dev;
+  r.fy = slice_plan(p.grid.ny, ctx().stream);
   r.keep = p.keep_d.p;
   r.keepinv = p.keepinv_d.p;
   r.G = G;
@@ -478,7 +488,7 @@ ColArgs col_args(const sk_potential& p, float2* G) {
   c.nx = p.grid.nx;
   c.ny = p.grid.ny;
   c.nc = p.nc;
-  c.fx = line_plan(p.grid.nx, ctx().stream).dev;
+  c.fx = slice_plan(p.grid.nx, ctx().stream);
   c.keep = p.keep_d.p;
   c.G = G;
   c.col_tile = fast_col_tile(p.grid.nx, ctx().col_tile);
@@ -1026,7 +1036,7 @@ static void run_waves(const sk_potential* p, int count, const double* in_re, con
     if (planes[q] >= 0) plane_out[planes[q]] = slot++;
   const int nslot = slot;
   const int B = default_batch(*p, count, opt);
-  const bool fast = !opt || opt->fast_path != 0;
+  const bool fast = (!opt || opt->fast_path != 0) && c.precision == 0;  // split-BF16 runs in the generic passes
   Workspace& w = ws();
   w.G.ensure((size_t)B * p->grid.nx * p->nc);
   w.full.ensure((size_t)B * N);
@@ -1331,7 +1341,7 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
     w.acc.ensure((size_t)npos * nslots);
     SK_HIP(hipMemsetAsync(w.acc.p, 0, sizeof(double) * (size_t)npos * nslots, s));
     // ---- batches ----
-    const bool fast = !opt || opt->fast_path != 0;
+    const bool fast = (!opt || opt->fast_path != 0) && c.precision == 0;  // split-BF16 runs in the generic passes
     int B = default_batch(p0, npos, opt);
     // lane pipeline: lanes of W positions, each on its own potential and slice at every step
     int pipe = opt ? opt->pipeline : 0;

@@ -103,6 +103,11 @@ struct LineFFT {
   const float2* roots;  // m entries, e^{-2 pi i k / m}
   const float2* chirp;  // n entries, e^{-pi i k^2 / n} (Bluestein)
   const float2* khat;   // m entries, FFT_m of the conjugated chirp, times 1/m (Bluestein)
+  // split-BF16 MFMA line transform (n = 256 or 512 only; sk_mfma.h): 0 = the float32 Stockham stages above
+  int use_mfma;
+  const void* mtab16;   // DFT-16 block matrices in MFMA B-operand order, bfloat16 hi/lo
+  const void* mtabn2;   // DFT-(n/16) block matrices, same order
+  const float2* mtw;    // e^{-2 pi i j2 k1 / n}, [k1 16][j2 n/16]
 };
 
 // Host-owned tables of one length.
@@ -113,6 +118,9 @@ struct LinePlan {
 
 // Returns the shared plan of length n (built once per process and device).
 const LinePlan& line_plan(int n, hipStream_t s);
+// The plan of length n with the split-BF16 MFMA line transform (n = 256 or 512), or fails.
+const LineFFT& mfma_line_plan(int n, hipStream_t s);
+bool mfma_supported(int n);
 
 // ---- lane pipeline ----------------------------------------------------------------------------------------------
 // The scan's waves are split into lanes of lane_waves positions. At every step each lane advances one slice of its

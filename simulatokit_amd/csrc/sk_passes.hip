@@ -14,11 +14,14 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
+#include <cstring>
 #include <complex>
 #include <map>
 
 #include "sk_dft.h"
 #include "sk_internal.h"
+#include "sk_mfma.h"
 
 namespace sk {
 
@@ -111,6 +114,12 @@ __device__ float2* lds_stockham(float2* a, float2* b, int lines, int ls, const L
 // two power-of-two transforms of length m.
 __device__ float2* lds_line_transform(float2* a, float2* b, int lines, int ls, const LineFFT& p, int sign, int tid,
                                       int T) {
+  if (p.use_mfma) {
+    const bf16x8* t16 = static_cast<const bf16x8*>(p.mtab16);
+    const bf16x8* tn2 = static_cast<const bf16x8*>(p.mtabn2);
+    return p.n == 256 ? mfma_lines<16>(a, b, lines, ls, t16, tn2, p.mtw, sign, tid)
+                      : mfma_lines<32>(a, b, lines, ls, t16, tn2, p.mtw, sign, tid);
+  }
   if (!p.blue) return lds_stockham(a, b, lines, ls, p, sign, tid, T);
   const int n = p.n, m = p.m;
   for (int idx = tid; idx < lines * m; idx += T) {
@@ -446,6 +455,75 @@ const LinePlan& line_plan(int n, hipStream_t s) {
   auto& ref = *lp;
   ps[n] = std::move(lp);
   return ref;
+}
+
+bool mfma_supported(int n) { return n == 256 || n == 512; }
+
+namespace {
+// float -> bfloat16 bits, round to nearest even (finite inputs)
+uint16_t bf16_bits(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return uint16_t(u >> 16);
+}
+float bf16_value(uint16_t b) {
+  const uint32_t u = uint32_t(b) << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+// The real block matrix of the length-L DFT in MFMA B-operand order [sign][k-step][tile][hi/lo][lane][8]:
+// row k = 16 s + 8 (lane / 32) + j (input j1 = k / 2, re/im k % 2), column 32 t + lane % 32 (output k1, re/im).
+std::vector<uint16_t> dft_block_table(int L) {
+  const int ksteps = L / 8, ntiles = L / 16;
+  std::vector<uint16_t> out((size_t)2 * ksteps * ntiles * 2 * 64 * 8);
+  for (int si = 0; si < 2; ++si)
+    for (int s = 0; s < ksteps; ++s)
+      for (int t = 0; t < ntiles; ++t)
+        for (int lane = 0; lane < 64; ++lane)
+          for (int j = 0; j < 8; ++j) {
+            const int k = 16 * s + 8 * (lane / 32) + j, nn = 32 * t + lane % 32;
+            const int j1 = k >> 1, p = k & 1, k1 = nn >> 1, q = nn & 1;
+            const double ang = (si ? 2.0 : -2.0) * M_PI * double((long long)j1 * k1 % L) / double(L);
+            const double wr = std::cos(ang), wi = std::sin(ang);
+            const double R = p == 0 ? (q == 0 ? wr : wi) : (q == 0 ? -wi : wr);
+            const uint16_t hi = bf16_bits((float)R);
+            const uint16_t lo = bf16_bits((float)(R - (double)bf16_value(hi)));
+            const size_t base = ((((size_t)si * ksteps + s) * ntiles + t) * 2) * 64 * 8;
+            out[base + (size_t)lane * 8 + j] = hi;
+            out[base + 64 * 8 + (size_t)lane * 8 + j] = lo;
+          }
+  return out;
+}
+struct MfmaPlan {
+  LineFFT dev{};
+  DevBuf<uint16_t> t16, tn2;
+  DevBuf<float2> tw;
+};
+}  // namespace
+
+const LineFFT& mfma_line_plan(int n, hipStream_t s) {
+  static std::map<int, std::unique_ptr<MfmaPlan>>* plans = new std::map<int, std::unique_ptr<MfmaPlan>>;
+  auto it = plans->find(n);
+  if (it != plans->end()) return it->second->dev;
+  if (!mfma_supported(n)) fail(SK_ERR_INVALID, "the split-BF16 MFMA transforms support grid lengths 256 and 512");
+  auto mp = std::make_unique<MfmaPlan>();
+  mp->dev = line_plan(n, s).dev;
+  const int n2 = n / 16;
+  mp->t16.upload(dft_block_table(16), s);
+  mp->tn2.upload(dft_block_table(n2), s);
+  std::vector<cd> tw((size_t)16 * n2);
+  for (int k1 = 0; k1 < 16; ++k1)
+    for (int j2 = 0; j2 < n2; ++j2) tw[(size_t)k1 * n2 + j2] = root((long long)k1 * j2 % n, n);
+  mp->tw.upload(to_f2(tw), s);
+  mp->dev.use_mfma = 1;
+  mp->dev.mtab16 = mp->t16.p;
+  mp->dev.mtabn2 = mp->tn2.p;
+  mp->dev.mtw = mp->tw.p;
+  const LineFFT& d = mp->dev;
+  (*plans)[n] = std::move(mp);
+  return d;
 }
 
 }  // namespace sk

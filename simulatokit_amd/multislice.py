@@ -57,6 +57,9 @@ def _options(opt: Optional[ExecutionOptions]) -> N.SkOptions:
     s.pipeline = int(o.pipeline)
     s.lane_waves = int(o.lane_waves)
     s.col_tile = int(o.col_tile)
+    if o.precision not in ("float32", "split_bf16"):
+        raise ValueError(f"precision must be 'float32' or 'split_bf16', got {o.precision!r}")
+    s.precision = 1 if o.precision == "split_bf16" else 0
     return s
 
 
