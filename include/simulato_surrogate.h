@@ -1,0 +1,51 @@
+/* simulato_surrogate.h — the layers of the HAADF surrogate network (libsimulato_hip.so).
+ *
+ * Build-only: the reference has no learned part (its detectors are computed, never predicted). These are the
+ * network's forward, backward and update steps as hand-written HIP kernels, for callers that train or run the
+ * surrogate (simulatokit_amd/surrogate.py). All pointers are device memory; `stream` is a hipStream_t (NULL = the
+ * default stream); every call only enqueues work. Activations are float32 [batch][channels][h][w] with h and w
+ * multiples of 16; weights are bfloat16 bit patterns [cout][cin][5][5]; padding is circular (the cells are periodic).
+ * Every reduction runs in a fixed order. Returns 0 or a negative SK_ERR_* status (message: sk_hip_last_error). */
+#ifndef SIMULATO_SURROGATE_H
+#define SIMULATO_SURROGATE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* y = bias + conv5(x, wt), ReLU applied when relu != 0 */
+int sk_conv5_forward(const float* x, int batch, int cin, int h, int w, const uint16_t* wt, const float* bias, int cout,
+                     int relu, float* y, void* stream);
+
+/* dx = the transpose of conv5 applied to dy; wt_scratch holds cin * cout * 25 bfloat16 values */
+int sk_conv5_backward_data(const float* dy, int batch, int cin, int h, int w, const uint16_t* wt, int cout, float* dx,
+                           uint16_t* wt_scratch, void* stream);
+
+/* dw [cout][cin][5][5] and db [cout] (db may be NULL) of sum(dy * conv5(x, w) + dy * b) */
+int sk_conv5_backward_weights(const float* x, const float* dy, int batch, int cin, int h, int w, int cout, float* dw,
+                              float* db, void* stream);
+
+/* dy *= (y > 0), y the output of a ReLU layer */
+int sk_relu_backward(float* dy, const float* y, long long n, void* stream);
+
+/* out[b][u][v] = sum_{i,j} A[u][i] A[v][j] in[b][i][j]; A m x n, in [batch][n][n], tmp [batch][n][m], out [batch][m][m]
+ * (the output's resampling to the scan grid, and with A transposed its backward pass) */
+int sk_separable_map(const float* in, int batch, int n, const float* A, int m, float* tmp, float* out, void* stream);
+
+/* grad = 2 (pred - label) / n; partials[(n + 255) / 256] receive the squared error's per-block sums (float64) */
+int sk_mse_grad(const float* pred, const float* label, long long n, float* grad, double* partials, void* stream);
+
+/* one Adam step (bias-corrected, step >= 1) of the float32 masters p with gradient g and moments m, v; p_bf16 (may be
+ * NULL) receives the bfloat16 rounding (to nearest, ties to even) of the updated p */
+int sk_adam_step(float* p, const float* g, float* m, float* v, uint16_t* p_bf16, long long n, float lr, float beta1,
+                 float beta2, float eps, int step, void* stream);
+
+/* out = bfloat16 rounding (to nearest, ties to even) of in */
+int sk_to_bf16(const float* in, uint16_t* out, long long n, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SIMULATO_SURROGATE_H */

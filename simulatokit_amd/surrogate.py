@@ -1,0 +1,414 @@
+"""The HAADF surrogate (build-only; the reference has no learned part): a six-layer convolutional network that maps a
+cell's summed projected potential (128 x 128) and its thickness to the HAADF image of a 96 x 96 scan over the cell,
+trained on images the multislice kernels generate, with plain multislice re-simulating a random 5 % of the positions
+as its error judge and standing in for the whole image when that error exceeds the tolerance.
+
+Network: 5 x 5 convolutions with circular padding (the cells are periodic), 2 -> 32 -> 32 -> 32 -> 32 -> 32 -> 1
+channels, ReLU between layers, bfloat16 weights, float32 activations and accumulation, all in the library's own HIP
+kernels (sk_conv5_forward / _backward_data / _backward_weights, sk_adam_step); the 128 x 128 output is resampled to
+the 96 x 96 scan grid by a fixed periodic linear interpolation (sk_separable_map). Torch only holds device memory.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _native as N
+from .core import AtomicStructure, ElectronBeam, ExecutionOptions
+from .detector import STEM, AnnularDetector
+from .multislice import Potential, _raise
+from .probe import Aperture, GridScan, Probe
+from .workloads import random_cell
+
+CHANNELS = [2, 32, 32, 32, 32, 32, 1]
+INPUT_N = 128
+SCAN_N = 96
+ELEMENTS = (6, 8, 14, 22, 38, 79)
+
+
+# ---- the cells and their images --------------------------------------------------------------------------------------
+@dataclass(frozen=True)
+class SurrogateWorkload:
+    """Seeded random cells (extent x extent, thickness uniform in [t_min, t_max], atom count uniform in
+    [n_min, n_max], Z from `elements`, 1.2 Angstrom minimum separation) scanned as the HAADF workload is: 200 keV,
+    20 mrad soft aperture, annular 50-200 mrad, `grid` x `grid` waves, 2 Angstrom slices, a scan_n x scan_n scan."""
+
+    extent: float = 30.0
+    t_min: float = 20.0
+    t_max: float = 100.0
+    n_min: int = 200
+    n_max: int = 800
+    elements: Tuple[int, ...] = ELEMENTS
+    min_separation: float = 1.2
+    grid: int = 512
+    scan_n: int = SCAN_N
+    input_n: int = INPUT_N
+    energy: float = 200e3
+    semiangle: float = 20.0
+    inner: float = 50.0
+    outer: float = 200.0
+    slice_thickness: float = 2.0
+
+    def cell(self, seed: int) -> AtomicStructure:
+        rng = np.random.default_rng([seed, 0x5EED])
+        t = float(rng.uniform(self.t_min, self.t_max))
+        n = int(rng.integers(self.n_min, self.n_max + 1))
+        return random_cell(seed, self.extent, t, n, self.elements, min_separation=self.min_separation)
+
+    def scan(self) -> GridScan:
+        return GridScan((0.0, 0.0), (self.extent, self.extent), self.scan_n, self.scan_n)
+
+    def probe(self, potential: Potential) -> Probe:
+        return Probe(ElectronBeam(self.energy), potential.grid, Aperture(self.semiangle))
+
+    def potential(self, st: AtomicStructure) -> Potential:
+        return Potential(st, self.grid, self.grid, slice_thickness=self.slice_thickness)
+
+    def haadf(self, st: AtomicStructure, positions: Optional[np.ndarray] = None,
+              options: Optional[ExecutionOptions] = None) -> np.ndarray:
+        """Plain multislice HAADF at `positions` (default: the whole scan, as a scan_n x scan_n image)."""
+        pot = self.potential(st)
+        probe = self.probe(pot)
+        prepared = pot.prepare(probe.beam)
+        scan = self.scan() if positions is None else positions
+        out = STEM.scan(probe, [prepared], scan, [AnnularDetector(self.inner, self.outer)],
+                        options or ExecutionOptions(batch=1024))[0].values
+        prepared.release()
+        return out.reshape(self.scan_n, self.scan_n) if positions is None else out
+
+    def features(self, st: AtomicStructure) -> np.ndarray:
+        """The network's input: the summed projected potential on an input_n^2 grid and a thickness channel."""
+        pot = Potential(st, self.input_n, self.input_n, slice_thickness=self.slice_thickness)
+        prepared = pot.prepare(ElectronBeam(self.energy))
+        vsum = prepared.values().sum(axis=0)
+        prepared.release()
+        return np.stack([vsum, np.full_like(vsum, st.box[2])]).astype(np.float32)
+
+
+@dataclass
+class Dataset:
+    seeds: List[int]
+    x: np.ndarray   # [n, 2, input_n, input_n] float32 (summed V in V*Angstrom, thickness in Angstrom)
+    y: np.ndarray   # [n, scan_n, scan_n] float32 HAADF (fraction of the incident intensity)
+
+
+def make_dataset(work: SurrogateWorkload, seeds: Sequence[int], log=None) -> Dataset:
+    xs, ys = [], []
+    t0 = time.perf_counter()
+    for k, s in enumerate(seeds):
+        st = work.cell(int(s))
+        xs.append(work.features(st))
+        ys.append(work.haadf(st).astype(np.float32))
+        if log and (k + 1) % max(1, len(seeds) // 20) == 0:
+            log(f"cells {k + 1}/{len(seeds)} in {time.perf_counter() - t0:.1f} s")
+    return Dataset(list(seeds), np.stack(xs), np.stack(ys))
+
+
+# ---- the network ----------------------------------------------------------------------------------------------------
+def _check(rc, what):
+    _raise(rc, what)
+
+
+def _bind():
+    L = N.lib()
+    if getattr(L, "_surrogate_bound", False):
+        return L
+    vp, ll = C.c_void_p, C.c_longlong
+    L.sk_conv5_forward.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, C.c_int, C.c_int, vp, vp]
+    L.sk_conv5_backward_data.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, vp, vp]
+    L.sk_conv5_backward_weights.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp]
+    L.sk_relu_backward.argtypes = [vp, vp, ll, vp]
+    L.sk_separable_map.argtypes = [vp, C.c_int, C.c_int, vp, C.c_int, vp, vp, vp]
+    L.sk_mse_grad.argtypes = [vp, vp, ll, vp, vp, vp]
+    L.sk_adam_step.argtypes = [vp, vp, vp, vp, vp, ll, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, vp]
+    L.sk_to_bf16.argtypes = [vp, vp, ll, vp]
+    L._surrogate_bound = True
+    return L
+
+
+def resample_matrix(m: int, n: int) -> np.ndarray:
+    """Periodic linear interpolation from n to m samples over one period (output u at input coordinate u n / m)."""
+    A = np.zeros((m, n), dtype=np.float64)
+    for u in range(m):
+        t = u * n / m
+        i = int(math.floor(t))
+        f = t - i
+        A[u, i % n] += 1 - f
+        A[u, (i + 1) % n] += f
+    return A
+
+
+class HaadfSurrogate:
+    """The network on the device. Weights: float32 masters (trained) and their bfloat16 rounding (used)."""
+
+    def __init__(self, seed: int = 0, channels: Sequence[int] = CHANNELS, input_n: int = INPUT_N,
+                 output_n: int = SCAN_N):
+        import torch
+
+        self.torch = torch
+        self.dev = torch.device("cuda:0")
+        self.channels = list(channels)
+        self.input_n, self.output_n = input_n, output_n
+        rng = np.random.default_rng(seed)
+        self.w32, self.wbf, self.b = [], [], []
+        for k, (ci, co) in enumerate(zip(self.channels[:-1], self.channels[1:])):
+            std = math.sqrt(2.0 / (ci * 25)) * (0.1 if k == len(self.channels) - 2 else 1.0)
+            w = rng.normal(0.0, std, size=(co, ci, 5, 5)).astype(np.float32)
+            self.w32.append(torch.tensor(w, device=self.dev))
+            self.wbf.append(torch.empty(w.size, dtype=torch.int16, device=self.dev))
+            self.b.append(torch.zeros(co, dtype=torch.float32, device=self.dev))
+        self.A = torch.tensor(resample_matrix(output_n, input_n), dtype=torch.float32, device=self.dev)
+        self.At = self.A.t().contiguous()
+        self.norm = {"v": 1.0, "t": 100.0, "y": 1.0}
+        self._publish()
+
+    # -- plumbing
+    @property
+    def _stream(self):
+        return C.c_void_p(self.torch.cuda.current_stream().cuda_stream)
+
+    def _publish(self):
+        L = _bind()
+        for w, wb in zip(self.w32, self.wbf):
+            _check(L.sk_to_bf16(w.data_ptr(), wb.data_ptr(), w.numel(), self._stream), "to_bf16")
+
+    def parameters(self) -> List:
+        return [t for pair in zip(self.w32, self.b) for t in pair]
+
+    # -- normalisation
+    def fit_normalisation(self, data: Dataset):
+        self.norm = {"v": float(np.std(data.x[:, 0])) or 1.0, "t": 100.0, "y": float(np.mean(data.y)) or 1.0}
+
+    def _inputs(self, x: np.ndarray):
+        x = np.array(x, dtype=np.float32, copy=True)
+        x[:, 0] /= self.norm["v"]
+        x[:, 1] /= self.norm["t"]
+        return self.torch.tensor(x, device=self.dev)
+
+    # -- forward / backward
+    def forward(self, x, keep: bool = False):
+        """x [B, 2, n, n] on the device -> the output_n^2 image [B, output_n, output_n] (normalised units);
+        keep=True also returns every activation (for backward)."""
+        torch, L = self.torch, _bind()
+        B = x.shape[0]
+        acts = [x]
+        for k in range(len(self.w32)):
+            ci, co = self.channels[k], self.channels[k + 1]
+            y = torch.empty((B, co, self.input_n, self.input_n), dtype=torch.float32, device=self.dev)
+            _check(L.sk_conv5_forward(acts[-1].data_ptr(), B, ci, self.input_n, self.input_n,
+                                      self.wbf[k].data_ptr(), self.b[k].data_ptr(), co,
+                                      1 if k < len(self.w32) - 1 else 0, y.data_ptr(), self._stream), "conv5_forward")
+            acts.append(y)
+        tmp = torch.empty((B, self.input_n, self.output_n), dtype=torch.float32, device=self.dev)
+        out = torch.empty((B, self.output_n, self.output_n), dtype=torch.float32, device=self.dev)
+        _check(L.sk_separable_map(acts[-1].data_ptr(), B, self.input_n, self.A.data_ptr(), self.output_n,
+                                  tmp.data_ptr(), out.data_ptr(), self._stream), "separable_map")
+        return (out, acts) if keep else out
+
+    def backward(self, acts, dout):
+        """Gradients [(dw, db), ...] of sum(dout * output) from the kept activations."""
+        torch, L = self.torch, _bind()
+        B = dout.shape[0]
+        tmp = torch.empty((B, self.output_n, self.input_n), dtype=torch.float32, device=self.dev)
+        g = torch.empty((B, 1, self.input_n, self.input_n), dtype=torch.float32, device=self.dev)
+        _check(L.sk_separable_map(dout.data_ptr(), B, self.output_n, self.At.data_ptr(), self.input_n,
+                                  tmp.data_ptr(), g.data_ptr(), self._stream), "separable_map")
+        grads = [None] * len(self.w32)
+        for k in range(len(self.w32) - 1, -1, -1):
+            ci, co = self.channels[k], self.channels[k + 1]
+            if k < len(self.w32) - 1:
+                _check(L.sk_relu_backward(g.data_ptr(), acts[k + 1].data_ptr(), g.numel(), self._stream), "relu")
+            dw = torch.empty((co, ci, 5, 5), dtype=torch.float32, device=self.dev)
+            db = torch.empty(co, dtype=torch.float32, device=self.dev)
+            _check(L.sk_conv5_backward_weights(acts[k].data_ptr(), g.data_ptr(), B, ci, self.input_n, self.input_n,
+                                               co, dw.data_ptr(), db.data_ptr(), self._stream), "backward_weights")
+            grads[k] = (dw, db)
+            if k > 0:
+                gx = torch.empty((B, ci, self.input_n, self.input_n), dtype=torch.float32, device=self.dev)
+                scratch = torch.empty(co * ci * 25, dtype=torch.int16, device=self.dev)
+                _check(L.sk_conv5_backward_data(g.data_ptr(), B, ci, self.input_n, self.input_n,
+                                                self.wbf[k].data_ptr(), co, gx.data_ptr(), scratch.data_ptr(),
+                                                self._stream), "backward_data")
+                g = gx
+        return grads
+
+    # -- training
+    def train(self, data: Dataset, epochs: int = 60, batch: int = 16, lr: float = 2e-3, seed: int = 0,
+              log=None) -> List[float]:
+        torch, L = self.torch, _bind()
+        self.fit_normalisation(data)
+        X = self._inputs(data.x)
+        Y = torch.tensor(data.y / self.norm["y"], dtype=torch.float32, device=self.dev)
+        m = [torch.zeros_like(p) for p in self.parameters()]
+        v = [torch.zeros_like(p) for p in self.parameters()]
+        rng = np.random.default_rng(seed)
+        step, history = 0, []
+        n = X.shape[0]
+        for ep in range(epochs):
+            order = rng.permutation(n)
+            total, count = 0.0, 0
+            rate = lr * 0.5 * (1 + math.cos(math.pi * ep / epochs))  # cosine schedule
+            for s0 in range(0, n - batch + 1, batch):
+                idx = torch.tensor(order[s0:s0 + batch], device=self.dev)
+                xb, yb = X.index_select(0, idx).contiguous(), Y.index_select(0, idx).contiguous()
+                out, acts = self.forward(xb, keep=True)
+                g = torch.empty_like(out)
+                nb = out.numel()
+                partial = torch.empty((nb + 255) // 256, dtype=torch.float64, device=self.dev)
+                _check(L.sk_mse_grad(out.data_ptr(), yb.data_ptr(), nb, g.data_ptr(), partial.data_ptr(),
+                                     self._stream), "mse_grad")
+                grads = self.backward(acts, g)
+                step += 1
+                for k, (dw, db) in enumerate(grads):
+                    for p, gp, mm, vv, pub in ((self.w32[k], dw, m[2 * k], v[2 * k], self.wbf[k]),
+                                               (self.b[k], db, m[2 * k + 1], v[2 * k + 1], None)):
+                        _check(L.sk_adam_step(p.data_ptr(), gp.data_ptr(), mm.data_ptr(), vv.data_ptr(),
+                                              pub.data_ptr() if pub is not None else None, p.numel(), rate, 0.9,
+                                              0.999, 1e-8, step, self._stream), "adam")
+                total += float(partial.sum().item())
+                count += nb
+            history.append(total / max(1, count))
+            if log:
+                log(f"epoch {ep + 1}/{epochs}: mse {history[-1]:.4e}")
+        torch.cuda.synchronize()
+        return history
+
+    # -- inference
+    def predict(self, x: np.ndarray) -> np.ndarray:
+        """HAADF images [B, output_n, output_n] (fraction of the incident intensity) for features x [B, 2, n, n]."""
+        out = self.forward(self._inputs(x))
+        self.torch.cuda.synchronize()
+        return out.cpu().numpy() * self.norm["y"]
+
+    # -- persistence
+    def state(self) -> Dict[str, np.ndarray]:
+        d = {f"w{k}": w.cpu().numpy() for k, w in enumerate(self.w32)}
+        d.update({f"b{k}": b.cpu().numpy() for k, b in enumerate(self.b)})
+        d["norm"] = np.array([self.norm["v"], self.norm["t"], self.norm["y"]])
+        d["channels"] = np.array(self.channels)
+        return d
+
+    def save(self, path: str):
+        np.savez(path, **self.state())
+
+    def load(self, path: str):
+        d = np.load(path)
+        for k in range(len(self.w32)):
+            self.w32[k].copy_(self.torch.tensor(d[f"w{k}"], device=self.dev))
+            self.b[k].copy_(self.torch.tensor(d[f"b{k}"], device=self.dev))
+        nv, nt, ny = d["norm"]
+        self.norm = {"v": float(nv), "t": float(nt), "y": float(ny)}
+        self._publish()
+
+
+# ---- the surrogate scan with its multislice judge ---------------------------------------------------------------------
+@dataclass
+class JudgedScan:
+    image: np.ndarray          # the delivered HAADF image (surrogate, or multislice after a fallback)
+    surrogate: np.ndarray      # the network's image
+    checked: np.ndarray        # flat indices of the re-simulated positions
+    spot_error: float          # relative L2 error of the network on the re-simulated positions
+    fallback: bool
+    seconds: Dict[str, float] = field(default_factory=dict)
+
+
+def judged_scan(model: HaadfSurrogate, work: SurrogateWorkload, st: AtomicStructure, fraction: float = 0.05,
+                tolerance: float = 0.02, seed: int = 0) -> JudgedScan:
+    """The surrogate's HAADF image of cell `st`, judged by plain multislice on a random `fraction` of the positions;
+    when the network's relative L2 error there exceeds `tolerance`, the whole image is re-simulated."""
+    t0 = time.perf_counter()
+    img = model.predict(work.features(st)[None])[0]
+    t1 = time.perf_counter()
+    npos = work.scan_n * work.scan_n
+    k = max(1, int(round(fraction * npos)))
+    checked = np.sort(np.random.default_rng(seed).choice(npos, size=k, replace=False))
+    positions = work.scan().positions[checked]
+    truth = work.haadf(st, positions)
+    t2 = time.perf_counter()
+    pred = img.reshape(-1)[checked]
+    err = float(np.linalg.norm(pred - truth) / max(np.linalg.norm(truth), 1e-300))
+    fallback = err > tolerance
+    final = img
+    if fallback:
+        final = work.haadf(st)
+    t3 = time.perf_counter()
+    return JudgedScan(final, img, checked, err, fallback,
+                      {"surrogate": t1 - t0, "judge": t2 - t1, "fallback": t3 - t2, "total": t3 - t0})
+
+
+__all__ = ["SurrogateWorkload", "Dataset", "make_dataset", "HaadfSurrogate", "JudgedScan", "judged_scan",
+           "resample_matrix", "CHANNELS"]
+
+
+# ---- the study: train on generated cells, judge on held-out cells -----------------------------------------------------
+def run_study(train: int = 256, held_out: int = 32, epochs: int = 60, batch: int = 16, seed: int = 20261202,
+              work: SurrogateWorkload = SurrogateWorkload(), tolerance: float = 0.02, fraction: float = 0.05,
+              weights: Optional[str] = None, log=print) -> Dict:
+    """Generates `train` + `held_out` seeded cells with the multislice kernels, trains the network on the first, and
+    scans the others both ways: plain multislice (the truth and the baseline time) and the judged surrogate. Returns
+    the errors, fallback count and the speed-up."""
+    import torch
+
+    seeds = [seed + i for i in range(train + held_out)]
+    t0 = time.perf_counter()
+    data = make_dataset(work, seeds[:train], log=log)
+    gen_s = time.perf_counter() - t0
+    model = HaadfSurrogate(seed=seed)
+    t0 = time.perf_counter()
+    history = model.train(data, epochs=epochs, batch=batch, seed=seed, log=log)
+    train_s = time.perf_counter() - t0
+    if weights:
+        model.save(weights)
+    rows = []
+    for s in seeds[train:]:
+        st = work.cell(s)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        truth = work.haadf(st)
+        plain_s = time.perf_counter() - t0
+        js = judged_scan(model, work, st, fraction=fraction, tolerance=tolerance, seed=s)
+        err_net = float(np.linalg.norm(js.surrogate - truth) / np.linalg.norm(truth))
+        err_out = float(np.linalg.norm(js.image - truth) / np.linalg.norm(truth))
+        slices = math.ceil(st.box[2] / work.slice_thickness)
+        rows.append({"seed": s, "slices": slices, "plain_s": plain_s, "judged_s": js.seconds["total"],
+                     "surrogate_err": err_net, "delivered_err": err_out, "spot_err": js.spot_error,
+                     "fallback": js.fallback})
+        log(f"held-out {s}: network {err_net:.4f}, spot {js.spot_error:.4f}, fallback {js.fallback}, "
+            f"plain {plain_s * 1e3:.0f} ms, judged {js.seconds['total'] * 1e3:.0f} ms")
+    npos = work.scan_n ** 2
+    plain_rate = sum(npos * r["slices"] for r in rows) / sum(r["plain_s"] for r in rows)
+    judged_rate = sum(npos * r["slices"] for r in rows) / sum(r["judged_s"] for r in rows)
+    return {"train_cells": train, "held_out_cells": held_out, "epochs": epochs, "generate_s": gen_s,
+            "train_s": train_s, "final_mse": history[-1], "first_mse": history[0],
+            "network_err_mean": float(np.mean([r["surrogate_err"] for r in rows])),
+            "delivered_err_max": float(np.max([r["delivered_err"] for r in rows])),
+            "fallbacks": int(sum(r["fallback"] for r in rows)),
+            "plain_pos_slices_per_s": plain_rate, "judged_pos_slices_per_s": judged_rate,
+            "speedup": judged_rate / plain_rate, "rows": rows}
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="train the HAADF surrogate on generated cells and judge it")
+    ap.add_argument("--train", type=int, default=256)
+    ap.add_argument("--held-out", type=int, default=32)
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--weights", default=None)
+    ap.add_argument("--out", default=None, help="write the summary JSON here")
+    a = ap.parse_args()
+    res = run_study(a.train, a.held_out, a.epochs, a.batch, a.seed, weights=a.weights,
+                    log=lambda m: print(m, flush=True))
+    summary = {k: v for k, v in res.items() if k != "rows"}
+    print(json.dumps(summary), flush=True)
+    if a.out:
+        with open(a.out, "w") as f:
+            json.dump(res, f, indent=1)

@@ -22,9 +22,12 @@ from simulatokit_amd import _native as N
 
 
 def header_symbols():
-    with open(os.path.join(ROOT, "include", "simulato_hip.h")) as fh:
-        text = fh.read()
-    return sorted(set(re.findall(r"\b(sk_hip_\w+)\s*\(", text)))
+    text = ""
+    for name in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if name.endswith(".h"):
+            with open(os.path.join(ROOT, "include", name)) as fh:
+                text += fh.read()
+    return sorted(set(re.findall(r"\b(sk_\w+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():
