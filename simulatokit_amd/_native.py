@@ -63,6 +63,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"the HIP library is not built ({LIB_PATH}); run `python -m simulatokit_amd.build`")
+        # torch, when installed, is loaded first: its wheel bundles its own ROCm runtime libraries, and loaded after
+        # this library has initialised the system's runtime it finds no devices; loaded first, both share its HSA
+        # runtime (the device memory the surrogate and the benchmarks use is torch's)
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         L.sk_hip_available.restype = C.c_int
         L.sk_hip_last_error.restype = C.c_char_p
