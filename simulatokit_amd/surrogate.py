@@ -237,9 +237,31 @@ class HaadfSurrogate:
                 g = gx
         return grads
 
+    # -- augmentation: the exact symmetries of a periodic cell scanned on a grid that starts at the origin
+    def _augment(self, xb, yb, rng):
+        """A random one of the 8 rotations/reflections and a random circular shift by a whole number of `q` input
+        pixels (= `p` output pixels, the shifts that map both grids onto themselves), applied to inputs and labels."""
+        torch = self.torch
+        n_in, n_out = self.input_n, self.output_n
+        g = math.gcd(n_in, n_out)
+        step_in, step_out = n_in // g, n_out // g
+        k = int(rng.integers(8))
+        sx, sy = (int(v) for v in rng.integers(g, size=2))
+
+        def transform(a, n, step):
+            i = torch.arange(n, device=self.dev)
+            neg = (-i) % n
+            if k & 1:  # reflection x -> -x
+                a = a.index_select(-2, neg)
+            for _ in range(k >> 1):  # rotation by 90 degrees about the origin: out[i][j] = in[-j][i]
+                a = a.index_select(-1, neg).transpose(-1, -2)
+            return torch.roll(a, shifts=(sx * step, sy * step), dims=(-2, -1))
+
+        return transform(xb, n_in, step_in).contiguous(), transform(yb, n_out, step_out).contiguous()
+
     # -- training
     def train(self, data: Dataset, epochs: int = 60, batch: int = 16, lr: float = 2e-3, seed: int = 0,
-              log=None) -> List[float]:
+              log=None, augment: bool = True) -> List[float]:
         torch, L = self.torch, _bind()
         self.fit_normalisation(data)
         X = self._inputs(data.x)
@@ -256,6 +278,8 @@ class HaadfSurrogate:
             for s0 in range(0, n - batch + 1, batch):
                 idx = torch.tensor(order[s0:s0 + batch], device=self.dev)
                 xb, yb = X.index_select(0, idx).contiguous(), Y.index_select(0, idx).contiguous()
+                if augment:
+                    xb, yb = self._augment(xb, yb, rng)
                 out, acts = self.forward(xb, keep=True)
                 g = torch.empty_like(out)
                 nb = out.numel()
