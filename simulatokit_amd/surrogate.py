@@ -372,7 +372,7 @@ __all__ = ["SurrogateWorkload", "Dataset", "make_dataset", "HaadfSurrogate", "Ju
 # ---- the study: train on generated cells, judge on held-out cells -----------------------------------------------------
 def run_study(train: int = 256, held_out: int = 32, epochs: int = 60, batch: int = 16, seed: int = 20261202,
               work: SurrogateWorkload = SurrogateWorkload(), tolerance: float = 0.02, fraction: float = 0.05,
-              weights: Optional[str] = None, log=print) -> Dict:
+              weights: Optional[str] = None, log=print, lr: float = 2e-3) -> Dict:
     """Generates `train` + `held_out` seeded cells with the multislice kernels, trains the network on the first, and
     scans the others both ways: plain multislice (the truth and the baseline time) and the judged surrogate. Returns
     the errors, fallback count and the speed-up."""
@@ -384,7 +384,7 @@ def run_study(train: int = 256, held_out: int = 32, epochs: int = 60, batch: int
     gen_s = time.perf_counter() - t0
     model = HaadfSurrogate(seed=seed)
     t0 = time.perf_counter()
-    history = model.train(data, epochs=epochs, batch=batch, seed=seed, log=log)
+    history = model.train(data, epochs=epochs, batch=batch, lr=lr, seed=seed, log=log)
     train_s = time.perf_counter() - t0
     if weights:
         model.save(weights)
@@ -400,6 +400,7 @@ def run_study(train: int = 256, held_out: int = 32, epochs: int = 60, batch: int
         err_out = float(np.linalg.norm(js.image - truth) / np.linalg.norm(truth))
         slices = math.ceil(st.box[2] / work.slice_thickness)
         rows.append({"seed": s, "slices": slices, "plain_s": plain_s, "judged_s": js.seconds["total"],
+                     "surrogate_and_judge_s": js.seconds["surrogate"] + js.seconds["judge"],
                      "surrogate_err": err_net, "delivered_err": err_out, "spot_err": js.spot_error,
                      "fallback": js.fallback})
         log(f"held-out {s}: network {err_net:.4f}, spot {js.spot_error:.4f}, fallback {js.fallback}, "
@@ -407,13 +408,16 @@ def run_study(train: int = 256, held_out: int = 32, epochs: int = 60, batch: int
     npos = work.scan_n ** 2
     plain_rate = sum(npos * r["slices"] for r in rows) / sum(r["plain_s"] for r in rows)
     judged_rate = sum(npos * r["slices"] for r in rows) / sum(r["judged_s"] for r in rows)
+    accepted_rate = sum(npos * r["slices"] for r in rows) / sum(r["surrogate_and_judge_s"] for r in rows)
     return {"train_cells": train, "held_out_cells": held_out, "epochs": epochs, "generate_s": gen_s,
             "train_s": train_s, "final_mse": history[-1], "first_mse": history[0],
             "network_err_mean": float(np.mean([r["surrogate_err"] for r in rows])),
             "delivered_err_max": float(np.max([r["delivered_err"] for r in rows])),
             "fallbacks": int(sum(r["fallback"] for r in rows)),
             "plain_pos_slices_per_s": plain_rate, "judged_pos_slices_per_s": judged_rate,
-            "speedup": judged_rate / plain_rate, "rows": rows}
+            "speedup": judged_rate / plain_rate,
+            "accepted_pos_slices_per_s": accepted_rate, "speedup_when_accepted": accepted_rate / plain_rate,
+            "network_err_median": float(np.median([r["surrogate_err"] for r in rows])), "rows": rows}
 
 
 if __name__ == "__main__":
@@ -426,11 +430,12 @@ if __name__ == "__main__":
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--weights", default=None)
     ap.add_argument("--out", default=None, help="write the summary JSON here")
     a = ap.parse_args()
     res = run_study(a.train, a.held_out, a.epochs, a.batch, a.seed, weights=a.weights,
-                    log=lambda m: print(m, flush=True))
+                    log=lambda m: print(m, flush=True), lr=a.lr)
     summary = {k: v for k, v in res.items() if k != "rows"}
     print(json.dumps(summary), flush=True)
     if a.out:
