@@ -461,8 +461,10 @@ __global__ void __launch_bounds__(TC * 64) col512_kernel(ColArgs a, const float2
     for (int r = 0; r < 8; ++r) tile[tile_slot(lane + T * r, wave)] = v[r];
     __syncthreads();
     float2* dst = a.G + b * gw + c0 + cb;
+    if (iyb >= 0) {  // padding columns stay as the row pass leaves them: never read, so never written
 #pragma unroll
-    for (int r = 0; r < 8; ++r) dst[(long long)(xi + T * r) * a.nc] = tile[tile_slot(xi + T * r, cb)];
+      for (int r = 0; r < 8; ++r) dst[(long long)(xi + T * r) * a.nc] = tile[tile_slot(xi + T * r, cb)];
+    }
     __syncthreads();
   }
 }
@@ -552,7 +554,7 @@ __global__ void step_increment_kernel(int* step) {
 bool is_fast_length(int n) { return n == 256 || n == 512; }
 
 constexpr int kRowPB = 8;  // positions per row-pass block
-constexpr int kColPB = 4;  // positions per column-pass block
+constexpr int kColPB = 8;  // positions per column-pass block (P and the twiddles loaded once per 8 positions)
 
 template <int N, int F>
 void row_go(const RowArgs& a, int batch, hipStream_t s, const float2* tw) {
