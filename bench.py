@@ -127,6 +127,7 @@ def main():
     achieved = alg_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
     stored_bytes = tm["col_bytes"]  # per launch: the compact columns of every wave read and written once
     haadf = out.cpu().numpy()
+    copy_gbs = copy_bandwidth(torch)
 
     result = {
         "metric": "probe-positions*slices/sec (HAADF scan) + fraction of HBM roofline at <=1e-4 rel err",
@@ -148,7 +149,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(f"{dom} pass"),
                      "kernel": f"{dom} pass", "kernel_ms": dom_ms, "row_ms": row_ms, "col_ms": col_ms,
-                     "algorithmic_bytes_per_launch": alg_bytes, "stored_bytes_per_launch": stored_bytes},
+                     "algorithmic_bytes_per_launch": alg_bytes, "stored_bytes_per_launch": stored_bytes,
+                     "copy_GBs": copy_gbs},
         "prepare_ms": prepare_ms,
         "scan_pos_slices": npos * S,
     }
@@ -157,6 +159,24 @@ def main():
         result["cpu_baseline"], result["rel_err_vs_fp64"] = cpu_baseline(args, w, st, pot, positions, haadf, S)
 
     print(json.dumps(result), flush=True)
+
+
+def copy_bandwidth(torch, nbytes: int = 1 << 30, reps: int = 5) -> float:
+    """The device-to-device copy rate of this GPU (read + write bytes per second, GB/s): the practical ceiling of a
+    pass that reads and writes every byte once, measured beside the published 8 TB/s peak."""
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda:0").fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return 2.0 * nbytes / (ms * 1e-3) / 1e9
 
 
 def cpu_baseline(args, w, st, pot, positions, haadf, S):

@@ -51,18 +51,22 @@ typedef struct sk_probe {
   double tilt[2];         /* mrad, enters the propagators only */
 } sk_probe;
 
-/* Detector kinds (Sources/SimulatoDetector/Detectors.swift:91-206). */
-enum sk_detector_kind { SK_DET_ANNULAR = 0, SK_DET_PIXELATED = 1, SK_DET_COM = 2 };
+/* Detector kinds (Sources/SimulatoDetector/Detectors.swift:91-263). */
+enum sk_detector_kind { SK_DET_ANNULAR = 0, SK_DET_PIXELATED = 1, SK_DET_COM = 2, SK_DET_SEGMENTED = 3 };
 enum sk_max_angle { SK_ANGLE_CUTOFF = 0, SK_ANGLE_VALID = 1, SK_ANGLE_FULL = 2 };
 
 typedef struct sk_detector {
   int kind;              /* sk_detector_kind */
-  double inner;          /* annular: inner angle, mrad */
-  double outer;          /* annular: outer angle, mrad; < 0 = abTEM's default (floor of the cutoff angle) */
+  double inner;          /* annular, segmented: inner angle, mrad */
+  double outer;          /* annular: outer angle, mrad; < 0 = abTEM's default (floor of the cutoff angle); segmented:
+                            outer angle, mrad (> inner) */
   int max_angle;         /* pixelated / COM region: sk_max_angle */
   int units;             /* COM: 0 = mrad, 1 = 1/Angstrom */
   int crop_nx, crop_ny;  /* pixelated: a numeric centred crop of the fftshifted pattern (0, 0 = by max_angle) */
   int out_f32;           /* 1: this detector's output array is float32 instead of float64 */
+  int bins_radial;       /* segmented (SegmentedDetector, Detectors.swift:207-263): radial bins between inner and outer */
+  int bins_azimuthal;    /* segmented: azimuthal bins; values per position [radial][azimuthal] */
+  double rotation;       /* segmented: the azimuth's origin, rad */
 } sk_detector;
 
 /* Execution knobs (ExecutionOptions.swift:21-49). None changes which numbers are computed, only their rounding. */

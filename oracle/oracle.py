@@ -245,22 +245,26 @@ MAX_ANGLE = {"cutoff": 0, "valid": 1, "full": 2}
 
 
 def det_annular(inner=0.0, outer=None):  # Detectors.swift:91-136
-    return ("annular", float(inner), -1.0 if outer is None else float(outer), 0, 0)
+    return ("annular", float(inner), -1.0 if outer is None else float(outer), 0, 0, 1, 1, 0.0)
 
 
 def det_pixelated(max_angle="valid"):  # Detectors.swift:140-167
-    return ("pixelated", 0.0, 0.0, MAX_ANGLE[max_angle], 0)
+    return ("pixelated", 0.0, 0.0, MAX_ANGLE[max_angle], 0, 1, 1, 0.0)
 
 
 def det_com(max_angle="valid", units="inverse_angstroms"):  # Detectors.swift:172-206
-    return ("com", 0.0, 0.0, MAX_ANGLE[max_angle], 0 if units == "mrad" else 1)
+    return ("com", 0.0, 0.0, MAX_ANGLE[max_angle], 0 if units == "mrad" else 1, 1, 1, 0.0)
 
 
-_KIND = {"annular": 0, "pixelated": 1, "com": 2}
+def det_segmented(radial, azimuthal, inner, outer, rotation=0.0):  # Detectors.swift:207-263
+    return ("segmented", float(inner), float(outer), 0, 0, int(radial), int(azimuthal), float(rotation))
+
+
+_KIND = {"annular": 0, "pixelated": 1, "com": 2, "segmented": 3}
 
 
 def _spec(d):
-    return np.array([_KIND[d[0]], d[1], d[2], d[3], d[4]], dtype=np.float64)
+    return np.array([_KIND[d[0]], d[1], d[2], d[3], d[4], d[5], d[6], d[7]], dtype=np.float64)
 
 
 def pixelated_size(nx, ny, extent, max_angle):
@@ -275,6 +279,8 @@ def detector_shape(nx, ny, extent, d):
         return ()
     if d[0] == "com":
         return (2,)
+    if d[0] == "segmented":
+        return (d[5], d[6])
     return pixelated_size(nx, ny, extent, {0: "cutoff", 1: "valid", 2: "full"}[d[3]])
 
 

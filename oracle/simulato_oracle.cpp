@@ -682,7 +682,7 @@ std::vector<cd> probe_build(const Grid& g, const FFT2& fft, const ProbeParams& p
 // ---------------------------------------------------------------------------------------------------------------
 // Detectors (Detectors.swift:28-206)
 // ---------------------------------------------------------------------------------------------------------------
-enum DetKind { kAnnular = 0, kPixelated = 1, kCom = 2 };
+enum DetKind { kAnnular = 0, kPixelated = 1, kCom = 2, kSegmented = 3 };
 enum MaxAngle { kCutoff = 0, kValid = 1, kFull = 2 };
 
 struct DetSpec {
@@ -691,7 +691,16 @@ struct DetSpec {
   double outer;  // < 0: default outer
   int max_angle;
   int units;  // COM: 0 mrad, 1 inverse angstrom
+  int nr = 1, nphi = 1;  // segmented: radial and azimuthal bins
+  double rotation = 0;   // segmented: azimuth origin [rad]
 };
+DetSpec det_spec(const double* s) {
+  DetSpec d{(int)s[0], s[1], s[2], (int)s[3], (int)s[4]};
+  d.nr = (int)s[5];
+  d.nphi = (int)s[6];
+  d.rotation = s[7];
+  return d;
+}
 
 void pixelated_size(const Grid& g, int max_angle, int& mx, int& my) {
   if (max_angle == kFull) { mx = g.nx; my = g.ny; return; }
@@ -710,6 +719,7 @@ void pixelated_size(const Grid& g, int max_angle, int& mx, int& my) {
 int det_count(const Grid& g, const DetSpec& d) {
   if (d.kind == kAnnular) return 1;
   if (d.kind == kCom) return 2;
+  if (d.kind == kSegmented) return d.nr * d.nphi;
   int mx, my;
   pixelated_size(g, d.max_angle, mx, my);
   return mx * my;
@@ -747,6 +757,32 @@ void detect(const Grid& g, double lambda, const DetSpec& d, const double* P, dou
         if (a2 >= i2 && a2 < o2) sum += P[(size_t)ix * g.ny + iy];
       }
     out[0] = sum;
+    return;
+  }
+  if (d.kind == kSegmented) {
+    // SegmentedDetector.detect (Detectors.swift:232-262): bins on the crop that holds the outer angle, in FFT order
+    auto m = [&](int n, double dd) {
+      int k = int(2 * std::ceil(d.outer / dd)) + 1;
+      if (k % 2 != n % 2) k += 1;
+      return std::min(k, n);
+    };
+    const int mx = m(g.nx, ax), my = m(g.ny, ay);
+    auto sx = crop_indices(g.nx, mx), sy = crop_indices(g.ny, my);
+    auto angx = fftfreq(mx, 1 / ax / double(mx)), angy = fftfreq(my, 1 / ay / double(my));
+    std::fill(out, out + d.nr * d.nphi, 0.0);
+    const double two_pi = 2 * kPi;
+    for (int cx = 0; cx < mx; ++cx)
+      for (int cy = 0; cy < my; ++cy) {
+        const double alpha = std::sqrt(angx[cx] * angx[cx] + angy[cy] * angy[cy]);
+        if (!(alpha >= d.inner && alpha < d.outer)) continue;
+        double phi = std::fmod(std::atan2(angy[cy], angx[cx]) - d.rotation, two_pi);
+        if (phi < 0) phi += two_pi;
+        const int r = (int)std::floor(double(d.nr) * (alpha - d.inner) / (d.outer - d.inner));
+        if (r >= d.nr) continue;
+        const double f = std::floor(double(d.nphi) * (phi / two_pi));
+        const int az = (int)std::min(std::max(f, 0.0), double(d.nphi - 1));
+        out[az + r * d.nphi] += P[(size_t)sx[cx] * g.ny + sy[cy]];
+      }
     return;
   }
   int mx, my;
@@ -1077,10 +1113,11 @@ int orc_pixelated_size(int nx, int ny, double ex, double ey, int max_angle, int*
   return 0;
 }
 
-// One detector on one intensity (Detectors.swift:119-205). spec: kind, inner, outer (<0 default), max_angle, units
+// One detector on one intensity (Detectors.swift:119-262). spec: kind, inner, outer (<0 default), max_angle, units,
+// radial bins, azimuthal bins, rotation
 int orc_detect(int nx, int ny, double ex, double ey, double energy, const double* spec, const double* P,
                double* out) {
-  DetSpec d{(int)spec[0], spec[1], spec[2], (int)spec[3], (int)spec[4]};
+  DetSpec d = det_spec(spec);
   try {
     detect(Grid{nx, ny, ex, ey}, wavelength(energy), d, P, out);
   } catch (const DetError& e) {
@@ -1092,7 +1129,7 @@ int orc_detect(int nx, int ny, double ex, double ey, double energy, const double
 
 // STEM.scan(.cpu) (STEM.swift:49-56, 95-115, 137-150): probe over positions through every potential, detectors
 // on each exit intensity, summed in potential order and divided by the potential count.
-// V: npots blocks of (nslices[k] x N); dz likewise concatenated. dets: ndet x 5 doubles. out: concatenated per
+// V: npots blocks of (nslices[k] x N); dz likewise concatenated. dets: ndet x 8 doubles (as orc_detect's spec). out: concatenated per
 // detector, npos x count_d each.
 int orc_stem_scan(int nx, int ny, double ex, double ey, double energy, double cutoff_mrad, int soft,
                   const double* aberr, double tx, double ty, int npots, const int* nslices, const double* V,
@@ -1106,8 +1143,7 @@ int orc_stem_scan(int nx, int ny, double ex, double ey, double energy, double cu
   size_t off = 0;
   double lambda = wavelength(energy);
   for (int d = 0; d < ndet; ++d) {
-    const double* s = dets + 5 * d;
-    specs.push_back(DetSpec{(int)s[0], s[1], s[2], (int)s[3], (int)s[4]});
+    specs.push_back(det_spec(dets + 8 * d));
     counts.push_back(det_count(g, specs.back()));
     offsets.push_back(off);
     off += (size_t)counts.back() * npos;

@@ -27,7 +27,8 @@ class SkProbe(C.Structure):
 
 class SkDetector(C.Structure):
     _fields_ = [("kind", C.c_int), ("inner", C.c_double), ("outer", C.c_double), ("max_angle", C.c_int),
-                ("units", C.c_int), ("crop_nx", C.c_int), ("crop_ny", C.c_int), ("out_f32", C.c_int)]
+                ("units", C.c_int), ("crop_nx", C.c_int), ("crop_ny", C.c_int), ("out_f32", C.c_int),
+                ("bins_radial", C.c_int), ("bins_azimuthal", C.c_int), ("rotation", C.c_double)]
 
 
 class SkOptions(C.Structure):

@@ -271,6 +271,19 @@ int detector_count(const sk_grid& g, double lambda, const sk_detector& d, int* s
     if (shape) shape[0] = 2;
     return 2;
   }
+  if (d.kind == SK_DET_SEGMENTED) {  // SegmentedDetector.init (Detectors.swift:218-227)
+    if (d.bins_radial < 1 || d.bins_azimuthal < 1) fail(SK_ERR_INVALID, "invalid detector: bins must be at least 1");
+    if (!(std::isfinite(d.inner) && std::isfinite(d.outer) && d.inner >= 0 && d.outer > d.inner &&
+          std::isfinite(d.rotation)))
+      fail(SK_ERR_INVALID, "invalid detector: need 0 <= inner < outer, finite, and a finite rotation");
+    if ((long long)d.bins_radial * d.bins_azimuthal > kMaxSlots)
+      fail(SK_ERR_INVALID, "invalid detector: at most " + std::to_string(kMaxSlots) + " segments");
+    if (shape) {
+      shape[0] = d.bins_radial;
+      shape[1] = d.bins_azimuthal;
+    }
+    return d.bins_radial * d.bins_azimuthal;
+  }
   if (d.kind == SK_DET_PIXELATED) {
     int mx, my;
     pixelated_size(g, d, mx, my);
@@ -282,6 +295,39 @@ int detector_count(const sk_grid& g, double lambda, const sk_detector& d, int* s
     return mx * my;
   }
   fail(SK_ERR_INVALID, "invalid detector kind");
+}
+
+// SegmentedDetector.detect (Detectors.swift:232-262): the bin of every pixel of the full grid, or -1. The bins live on
+// the crop that holds the outer angle (2 ceil(outer / da) + 1 points with the grid's parity, at most n), whose
+// pixels in FFT order are the full pattern's at the crop's source indices; alpha = sqrt(ax^2 + ay^2) and the
+// azimuth atan2(ay, ax) - rotation reduced into [0, 2 pi), in float64 without contraction, as the reference
+// rounds them.
+std::vector<int> segment_map(const sk_grid& g, double ax, double ay, const sk_detector& d) {
+#pragma clang fp contract(off)
+  auto m = [&](int n, double da) {
+    int k = int(2 * std::ceil(d.outer / da)) + 1;
+    if (k % 2 != n % 2) k += 1;
+    return std::min(k, n);
+  };
+  const int mx = m(g.nx, ax), my = m(g.ny, ay);
+  const auto sx = crop_indices(g.nx, mx), sy = crop_indices(g.ny, my);
+  const auto angx = fftfreq(mx, 1 / ax / double(mx)), angy = fftfreq(my, 1 / ay / double(my));
+  std::vector<int> map((size_t)g.nx * g.ny, -1);
+  const double two_pi = 2 * M_PI;
+  for (int cx = 0; cx < mx; ++cx)
+    for (int cy = 0; cy < my; ++cy) {
+      const double sq = angx[cx] * angx[cx] + angy[cy] * angy[cy];
+      const double alpha = std::sqrt(sq);
+      if (!(alpha >= d.inner && alpha < d.outer)) continue;
+      double phi = std::fmod(std::atan2(angy[cy], angx[cx]) - d.rotation, two_pi);
+      if (phi < 0) phi += two_pi;
+      const int r = (int)std::floor(double(d.bins_radial) * (alpha - d.inner) / (d.outer - d.inner));
+      if (r >= d.bins_radial) continue;  // alpha just below outer can round to the last edge: dropped, as abTEM does
+      const double f = std::floor(double(d.bins_azimuthal) * (phi / two_pi));
+      const int az = (int)std::min(std::max(f, 0.0), double(d.bins_azimuthal - 1));
+      map[(size_t)sx[cx] * g.ny + sy[cy]] = az + r * d.bins_azimuthal;
+    }
+  return map;
 }
 
 // ---- FFT pass helpers (full width: identity maps) ----------------------------------------------------------------
@@ -1238,11 +1284,12 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
     Workspace& w = ws();
     std::vector<int> det_slot(ndets, -1), det_pix(ndets, -1), det_count(ndets, 0);
     std::vector<double> sqx, sqy;
-    int nmask = 0, ncom = 0, npix = 0;
+    int nmask = 0, ncom = 0, npix = 0, nseg = 0;
+    dp.ny = ny;
     w.maps.clear();
     w.weights.clear();
     w.pix.clear();
-    w.maps.reserve(4 * kMaxCom + 2 * kMaxPix);
+    w.maps.reserve(4 * kMaxCom + 2 * kMaxPix + kMaxSeg);
     w.weights.reserve(2 * kMaxCom);
     w.pix.reserve(kMaxPix);
     const double rxs = 1 / (double(nx) * (g.extent_x / nx)), rys = 1 / (double(ny) * (g.extent_y / ny));
@@ -1301,6 +1348,22 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
         dp.slot_arg[dp.nslots + 1] = ncom;
         dp.nslots += 2;
         ncom++;
+      } else if (D.kind == SK_DET_SEGMENTED) {
+        const int nb = D.bins_radial * D.bins_azimuthal;
+        if (dp.nslots + nb > kMaxSlots || nseg >= kMaxSeg)
+          fail(SK_ERR_INVALID, "too many detectors: the segmented detectors' bins and the other sums exceed " +
+                                   std::to_string(kMaxSlots));
+        w.maps.emplace_back();
+        w.maps.back().upload(segment_map(g, ax, ay, D), s);
+        dp.seg_map[nseg] = w.maps.back().p;
+        det_slot[d] = dp.nslots;
+        for (int b = 0; b < nb; ++b) {
+          dp.slot_kind[dp.nslots + b] = 3;
+          dp.slot_arg[dp.nslots + b] = nseg;
+          dp.slot_bin[dp.nslots + b] = b;
+        }
+        dp.nslots += nb;
+        nseg++;
       } else {
         if (npix >= kMaxPix) fail(SK_ERR_INVALID, "too many pixelated detectors");
         int mx, my;
@@ -1480,6 +1543,11 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
         launch_gather_scalar(w.acc.p, nslots, det_slot[d], npos, inv, dst, 2, D.out_f32, s);
         char* dst1 = static_cast<char*>(dst) + esz;
         launch_gather_scalar(w.acc.p, nslots, det_slot[d] + 1, npos, inv, dst1, 2, D.out_f32, s);
+      } else if (D.kind == SK_DET_SEGMENTED) {
+        const int nb = D.bins_radial * D.bins_azimuthal;
+        for (int b = 0; b < nb; ++b)
+          launch_gather_scalar(w.acc.p, nslots, det_slot[d] + b, npos, inv, static_cast<char*>(dst) + b * esz, nb,
+                               D.out_f32, s);
       } else {
         launch_finalise_outputs(w.pix[det_pix[d]].p, cnt, inv, dst, D.out_f32, s);
       }

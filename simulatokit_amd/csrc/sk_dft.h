@@ -56,6 +56,8 @@ __device__ inline void detect_pixel(const DetPlan& d, int ix, int iy, double I, 
     const int kind = d.slot_kind[q], arg = d.slot_arg[q];
     if (kind == 0) {
       if (a2 >= d.slot_i2[q] && a2 < d.slot_o2[q]) s[q] += I;
+    } else if (kind == 3) {
+      if (d.seg_map[arg][(long long)ix * d.ny + iy] == d.slot_bin[q]) s[q] += I;
     } else {
       const int mx = d.com_mapx[arg][ix], my = d.com_mapy[arg][iy];
       if (mx >= 0 && my >= 0) s[q] += I * (kind == 1 ? d.com_wx[arg][ix] : d.com_wy[arg][iy]);

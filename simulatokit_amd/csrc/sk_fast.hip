@@ -301,6 +301,12 @@ __device__ __forceinline__ void detect_block(const float2 (&v)[8], const DetPlan
           const double a2 = __dadd_rn(d.sqx[i + T * r], sy);  // float64, as the host rounds it
           if (a2 >= i2 && a2 < o2) val += I[r];
         }
+      } else if (kind == 3) {
+        const int* m = d.seg_map[arg] + iy;
+        const int bin = d.slot_bin[q];
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+          if (m[(long long)(i + T * r) * d.ny] == bin) val += I[r];
       } else if (d.com_mapy[arg][iy] >= 0) {
         const double wy = d.com_wy[arg][iy];
 #pragma unroll

@@ -187,15 +187,19 @@ enum ColFlags : int {
   COL_OUT = 64         // store back into G
 };
 
-constexpr int kMaxSlots = 16;
+constexpr int kMaxSlots = 32;
+constexpr int kMaxSeg = 4;
 constexpr int kMaxPix = 4;
 constexpr int kMaxCom = 4;
 
 // Detector reduction plan of the final column pass.
 struct DetPlan {
   int nslots;                   // scalar sums per wave
-  int slot_kind[kMaxSlots];     // 0: annular, 1: COM x, 2: COM y
-  int slot_arg[kMaxSlots];      // COM index (COM slots)
+  int slot_kind[kMaxSlots];     // 0: annular, 1: COM x, 2: COM y, 3: a segmented detector's bin
+  int slot_arg[kMaxSlots];      // COM index (COM slots), segmented detector index (bin slots)
+  int slot_bin[kMaxSlots];      // bin slots: the bin (radial * azimuthal bins + azimuthal)
+  const int* seg_map[kMaxSeg];  // [nx][ny] bin of each pixel, or -1 (outside the bins)
+  int ny;
   double slot_i2[kMaxSlots];    // annular: inner^2, outer^2 [mrad^2], half-open inner^2 <= a2 < outer^2
   double slot_o2[kMaxSlots];
   const double* sqx;            // [nx] angle_x^2 of the pattern's pixels [mrad^2] (float64, host-rounded)

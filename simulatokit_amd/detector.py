@@ -15,7 +15,7 @@ import numpy as np
 from . import _native as N
 from .core import ExecutionOptions, SimulatoError
 from .multislice import AnyPotential, PreparedPotential, _options, _prepared, _raise, _sk_grid
-from .probe import GridScan, Probe
+from .probe import GridScan, LineScan, Probe
 
 MAX_ANGLE = {"cutoff": 0, "valid": 1, "full": 2}
 
@@ -65,7 +65,33 @@ class CenterOfMassDetector:
         return N.SkDetector(2, 0.0, 0.0, MAX_ANGLE[self.max_angle], 0 if self.units == "mrad" else 1, 0, 0, 0)
 
 
-Detector = Union[AnnularDetector, PixelatedDetector, CenterOfMassDetector]
+@dataclass(frozen=True)
+class SegmentedDetector:
+    """radial_bins x azimuthal_bins bins between inner and outer [mrad], the azimuth measured from `rotation` [rad],
+    summed in float64 (abTEM's SegmentedDetector / polar binning; Detectors.swift:207-263). Values per position
+    [radial][azimuthal]."""
+
+    radial_bins: int
+    azimuthal_bins: int
+    inner: float
+    outer: float
+    rotation: float = 0.0
+
+    def __post_init__(self):
+        if self.radial_bins < 1 or self.azimuthal_bins < 1:
+            raise SimulatoError("invalidDetector", "bins must be at least 1")
+        if not (math.isfinite(self.inner) and math.isfinite(self.outer) and self.inner >= 0
+                and self.outer > self.inner and math.isfinite(self.rotation)):
+            raise SimulatoError("invalidDetector", "need 0 <= inner < outer, finite, and a finite rotation")
+
+    def descriptor(self) -> N.SkDetector:
+        d = N.SkDetector(3, float(self.inner), float(self.outer), 0, 0, 0, 0, 0)
+        d.bins_radial, d.bins_azimuthal, d.rotation = int(self.radial_bins), int(self.azimuthal_bins), \
+            float(self.rotation)
+        return d
+
+
+Detector = Union[AnnularDetector, PixelatedDetector, CenterOfMassDetector, SegmentedDetector]
 
 
 @dataclass
@@ -96,6 +122,8 @@ def detector_shape(d: Detector, grid, energy: float) -> Tuple[int, ...]:
         return ()
     if isinstance(d, CenterOfMassDetector):
         return (2,)
+    if isinstance(d, SegmentedDetector):
+        return (int(d.radial_bins), int(d.azimuthal_bins))
     return (int(shape[0]), int(shape[1]))
 
 
@@ -110,8 +138,8 @@ class STEM:
         arrays; the returned measurements then hold None."""
         if not potentials:
             raise SimulatoError("invalidScan", "invalid scan: no potential")
-        if isinstance(scan, GridScan):
-            positions, scan_shape = scan.positions, scan.shape
+        if isinstance(scan, (GridScan, LineScan)):
+            positions, scan_shape = scan.positions, tuple(scan.shape)
         else:
             positions = np.asarray(scan, dtype=np.float64).reshape(-1, 2)
             scan_shape = (len(positions),)
@@ -151,5 +179,5 @@ def set_timing(enabled: bool):
     N.lib().sk_hip_set_timing(1 if enabled else 0)
 
 
-__all__ = ["AnnularDetector", "PixelatedDetector", "CenterOfMassDetector", "ScanMeasurement", "STEM", "last_timing",
-           "set_timing", "detector_shape"]
+__all__ = ["AnnularDetector", "PixelatedDetector", "CenterOfMassDetector", "SegmentedDetector", "ScanMeasurement",
+           "STEM", "last_timing", "set_timing", "detector_shape"]

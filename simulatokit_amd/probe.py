@@ -6,9 +6,10 @@ Mirrors Sources/SimulatoProbe/{Probe,Aberrations,GridScan}.swift. Probes are bui
 from __future__ import annotations
 
 import ctypes as C
+import copy
 import math
 from dataclasses import dataclass, fields
-from typing import List, Sequence, Tuple
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -178,4 +179,97 @@ class GridScan:
         return np.stack(np.meshgrid(xs, ys, indexing="ij"), axis=-1).reshape(-1, 2)
 
 
-__all__ = ["Aberrations", "Aperture", "Probe", "GridScan", "ABERRATION_NAMES"]
+class LineScan:
+    """Probe positions along a straight line (abTEM's LineScan; LineScan.swift:26-198): `gpts` points from `start` to
+    `end` [Angstrom], the last at `end` with `endpoint`. A given `sampling` wins over `gpts`: the count becomes
+    ceil(length / sampling) and the realised sampling length / (count - 1) with `endpoint` and more than one point,
+    else length / count (abTEM's _adjust_gpts / _adjust_sampling)."""
+
+    def __init__(self, start=(0.0, 0.0), end=None, gpts: Optional[int] = None, sampling: Optional[float] = None,
+                 endpoint: bool = True, fractional: bool = False, extent=None):
+        if end is None:
+            raise SimulatoError("invalidScan", "invalid scan: a line scan needs an end point")
+        s, e = as_float_pair(start), as_float_pair(end)
+        if fractional:
+            if extent is None:
+                raise SimulatoError("invalidScan", "provide the potential's extent for fractional coordinates")
+            ex = as_float_pair(extent)
+            s, e = (ex[0] * s[0], ex[1] * s[1]), (ex[0] * e[0], ex[1] * e[1])
+        if not all(math.isfinite(v) for v in s + e):
+            raise SimulatoError("invalidScan", "start and end must be finite")
+        if gpts is not None and not (1 <= gpts < 1_000_000_000):
+            raise SimulatoError("invalidScan", f"gpts must be at least 1 and below 1e9, got {gpts}")
+        if sampling is not None and not (math.isfinite(sampling) and sampling > 0):
+            raise SimulatoError("invalidScan", f"sampling must be finite and positive, got {sampling} A")
+        if gpts is None and sampling is None:
+            raise SimulatoError("invalidScan", "provide gpts or sampling for a line scan")
+        self.start, self.end, self.endpoint = s, e, bool(endpoint)
+        self.gpts = int(gpts) if gpts is not None else 0
+        self.sampling = float(sampling) if sampling is not None else 0.0
+        if sampling is not None:
+            self._adjust_gpts()
+        self._adjust_sampling()
+
+    @property
+    def extent(self) -> float:
+        dx, dy = self.end[0] - self.start[0], self.end[1] - self.start[1]
+        return math.sqrt(dx * dx + dy * dy)
+
+    @property
+    def direction(self) -> Tuple[float, float]:
+        L = self.extent
+        return ((self.end[0] - self.start[0]) / L, (self.end[1] - self.start[1]) / L) if L > 0 else (math.nan, math.nan)
+
+    @property
+    def angle(self) -> float:
+        d = self.direction
+        return math.atan2(d[1], d[0])
+
+    @property
+    def shape(self) -> Tuple[int]:
+        return (self.gpts,)
+
+    def _adjust_gpts(self):
+        n = math.ceil(self.extent / self.sampling)
+        if not (1 <= n < 1e9):
+            raise SimulatoError("invalidScan", f"the line of {self.extent} A at sampling {self.sampling} A has {n} positions")
+        self.gpts = int(n)
+        self._adjust_sampling()
+
+    def _adjust_sampling(self):
+        self.sampling = self.extent / (self.gpts - 1) if self.endpoint and self.gpts > 1 else self.extent / self.gpts
+
+    def adding_margin(self, before: float, after: Optional[float] = None) -> "LineScan":
+        """The line with `before` added at the start and `after` at the end along its direction (abTEM add_margin);
+        the count is recomputed from the sampling after each end moves, as abTEM's setters do."""
+        after = before if after is None else after
+        if not (math.isfinite(before) and math.isfinite(after)):
+            raise SimulatoError("invalidScan", "the margin must be finite")
+        if not self.extent > 0:
+            raise SimulatoError("invalidScan", "a line of zero length has no direction for a margin")
+        line = copy.copy(self)
+        d = self.direction
+        line.start = (self.start[0] - d[0] * before, self.start[1] - d[1] * before)
+        line._adjust_gpts()
+        line.end = (self.end[0] + d[0] * after, self.end[1] + d[1] * after)
+        line._adjust_gpts()
+        return line
+
+    @staticmethod
+    def at_position(center, extent: float = 1.0, angle: float = 0.0, gpts: Optional[int] = None,
+                    sampling: Optional[float] = None, endpoint: bool = True) -> "LineScan":
+        """A line of `extent` centred on `center` at `angle` degrees to the first axis (abTEM at_position)."""
+        a = angle * (math.pi / 180)
+        c = as_float_pair(center)
+        dx, dy = math.cos(a), math.sin(a)
+        return LineScan((c[0] - extent / 2 * dx, c[1] - extent / 2 * dy), (c[0] + extent / 2 * dx, c[1] + extent / 2 * dy),
+                        gpts=gpts, sampling=sampling, endpoint=endpoint)
+
+    @property
+    def positions(self) -> np.ndarray:
+        xs = GridScan.linspace(self.start[0], self.end[0], self.gpts, self.endpoint)
+        ys = GridScan.linspace(self.start[1], self.end[1], self.gpts, self.endpoint)
+        return np.stack([np.broadcast_to(xs, (self.gpts,)), np.broadcast_to(ys, (self.gpts,))], axis=-1)
+
+
+__all__ = ["Aberrations", "Aperture", "Probe", "GridScan", "LineScan", "ABERRATION_NAMES"]

@@ -78,6 +78,15 @@ def main():
     meta = {k: v for k, v in js.items() if k != "arrays"}
     dump("slice4scan", js, blob, names, meta)
 
+    # line scans: positions and STEM over a line with an annular detector (tools/pins/linescan/pins.py)
+    js, blob = load("linescan-pins.json")
+    names = sorted(n for n in js["arrays"]["index"] if n.startswith("line.") or n.endswith((".adf", ".positions")))
+    meta = {k: js[k] for k in ("versions", "box", "gpts", "sliceThickness", "energy", "semiangle", "atoms",
+                               "lineScans")}
+    meta["scans"] = [{k: v for k, v in s.items() if k in ("label", "start", "end", "gpts", "sampling", "endpoint")}
+                     for s in js["scans"]]
+    dump("linescan", js, blob, names, meta)
+
 
 if __name__ == "__main__":
     main()

@@ -157,7 +157,8 @@ def test_run_batch_equals_single_runs_and_the_oracle():
 # ---- STEM scans --------------------------------------------------------------------------------------------------
 DETS = {"BF": sk.AnnularDetector(0, 20), "ADF": sk.AnnularDetector(30, 60), "HAADF": sk.AnnularDetector(70, 130),
         "annular_default_outer": sk.AnnularDetector(0, None), "pixelated_cutoff": sk.PixelatedDetector("cutoff"),
-        "pixelated_valid": sk.PixelatedDetector("valid"), "pixelated_full": sk.PixelatedDetector("full")}
+        "pixelated_valid": sk.PixelatedDetector("valid"), "pixelated_full": sk.PixelatedDetector("full"),
+        "DPC": sk.SegmentedDetector(2, 4, 0, 20), "DPC_rotated": sk.SegmentedDetector(2, 4, 0, 20, 0.3)}
 
 
 @pytest.mark.parametrize("fast", [True, False])
@@ -177,8 +178,10 @@ def test_stem_detectors_equal_abtem(cid, fast):
     ms = sk.STEM.scan(probe, pots, scan, [DETS[n] for n in names], sk.ExecutionOptions(fast_path=fast))
     for name, m in zip(names, ms):
         entry = c["detectors"][name]
-        if entry["kind"] == "annular":
-            want = S2.array(f"{cid}.{name}").reshape(-1)
+        if entry["kind"] in ("annular", "segmented"):
+            # segmented: the float64 bin sums (STEMPinsTests.swift:43-47)
+            key = f"{cid}.{name}" + (".float64" if entry["kind"] == "segmented" else "")
+            want = S2.array(key).reshape(-1)
             bound = ACC.detector_bound(np.abs(want).max())
             assert np.abs(m.values - want).max() <= bound, name
         else:

@@ -250,7 +250,8 @@ def grid_positions(scan):
 
 DETS = {"BF": O.det_annular(0, 20), "ADF": O.det_annular(30, 60), "HAADF": O.det_annular(70, 130),
         "annular_default_outer": O.det_annular(0, None), "pixelated_cutoff": O.det_pixelated("cutoff"),
-        "pixelated_valid": O.det_pixelated("valid"), "pixelated_full": O.det_pixelated("full")}
+        "pixelated_valid": O.det_pixelated("valid"), "pixelated_full": O.det_pixelated("full"),
+        "DPC": O.det_segmented(2, 4, 0, 20), "DPC_rotated": O.det_segmented(2, 4, 0, 20, 0.3)}
 
 
 @pytest.mark.parametrize("cid", ["S1", "S2", "S3"])
@@ -273,6 +274,13 @@ def test_detectors_equal_abtem(cid, pi):
         if entry["kind"] == "annular":
             assert scaled_max(m.reshape(-1), S2.array(f"{cid}.{name}").reshape(-1), entry["scale"]) <= \
                 (1e-10 if tight else 1e-6), name
+        elif entry["kind"] == "segmented":
+            # STEMPinsTests.swift:43-47: against the float64 bin sums, and abTEM's float32 sums to 1e-6
+            assert entry["params"] == {"inner": 0.0, "outer": 20.0, "nbins_radial": 2, "nbins_azimuthal": 4,
+                                       "rotation": 0.3 if name == "DPC_rotated" else 0.0}
+            assert scaled_max(m.reshape(-1), S2.array(f"{cid}.{name}.float64").reshape(-1), entry["scale"]) <= \
+                (1e-10 if tight else 1e-6), name
+            assert scaled_max(m.reshape(-1), S2.array(f"{cid}.{name}").reshape(-1), entry["scale"]) <= 1e-6, name
         else:
             sums = np.array(entry["sums"]).reshape(-1)
             mine = m.reshape(len(pos), -1).sum(axis=1)
