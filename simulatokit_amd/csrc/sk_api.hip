@@ -1431,7 +1431,7 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
     if (!fits) w.full.ensure((size_t)B * N);
     int bpw = (p0.nc + col_generic_lines(line_plan(nx, s).dev.m) - 1) / col_generic_lines(line_plan(nx, s).dev.m);
     if (fast && fast_col_supported(nx, p0.nc, COL_IN_G | COL_FWD | COL_MUL_P | COL_DETECT))
-      bpw = fast_col_blocks_per_wave(nx, p0.nc, pipe ? 16 : c.col_tile);
+      bpw = fast_col_blocks_per_wave(nx, p0.nc, pipe);
     dp.blocks_per_wave = bpw;
     w.partial.ensure((size_t)B * nslots * bpw);
     dp.partial = w.partial.p;

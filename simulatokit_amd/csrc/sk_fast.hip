@@ -587,10 +587,16 @@ template <int N, int F>
 void col_go(const ColArgs& a, int batch, hipStream_t s, const float2* tw) {
   const int by = (batch + kColPB - 1) / kColPB;
   if constexpr (N == 512) {
-    if (fast_col_tile(512, a.col_tile) == 8)
+    // the detecting pass always runs 8-column tiles: its reductions need more than the 128 VGPRs of a 1024-thread
+    // block, and 512-thread blocks give it 256 without spilling
+    if constexpr (F == COL_FINAL) {
       hipLaunchKernelGGL((col512_kernel<F, kColPB, 8>), dim3(a.nc / 8, by), dim3(8 * 64), 0, s, a, tw, batch);
-    else
-      hipLaunchKernelGGL((col512_kernel<F, kColPB, 16>), dim3(a.nc / 16, by), dim3(16 * 64), 0, s, a, tw, batch);
+    } else {
+      if (fast_col_tile(512, a.col_tile) == 8)
+        hipLaunchKernelGGL((col512_kernel<F, kColPB, 8>), dim3(a.nc / 8, by), dim3(8 * 64), 0, s, a, tw, batch);
+      else
+        hipLaunchKernelGGL((col512_kernel<F, kColPB, 16>), dim3(a.nc / 16, by), dim3(16 * 64), 0, s, a, tw, batch);
+    }
   } else {
     hipLaunchKernelGGL((col_fast_kernel<N, F, kColPB>), dim3(a.nc / kColTile, by), dim3(kColTile * (N / 8)), 0, s,
                        a, tw, batch);
@@ -627,7 +633,10 @@ int fast_col_tile(int nx, int requested) {
   return requested == 8 || requested == 16 ? requested : kCol512Tile;
 }
 
-int fast_col_blocks_per_wave(int nx, int nc, int col_tile) { return nc / fast_col_tile(nx, col_tile); }
+int fast_col_blocks_per_wave(int nx, int nc, int pipeline) {
+  if (pipeline || nx != 512) return nc / kColTile;
+  return nc / 8;  // the 512-point detecting pass's tile
+}
 
 void launch_row_fast(const RowArgs& a, int batch, hipStream_t s) {
   const float2* tw = line_plan(a.ny, s).dev.roots;

@@ -251,7 +251,7 @@ bool fast_col_supported(int nx, int nc, int flags);
 void launch_row_fast(const RowArgs& a, int batch, hipStream_t s);
 void launch_col_fast(const ColArgs& a, int batch, hipStream_t s);
 int fast_col_tile(int nx, int requested);  // columns per block the fast column pass uses
-int fast_col_blocks_per_wave(int nx, int nc, int col_tile);
+int fast_col_blocks_per_wave(int nx, int nc, int pipeline);  // blocks of the detecting column pass per wave
 // lane pipeline launches (sk_fast.hip): the row pass and the column pass of one step, the partial-sum reduction of
 // the lanes that finished an item, and the step counter's increment
 bool pipeline_supported(int nx, int ny, int nc, int lane_waves);
