@@ -159,6 +159,13 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
                 const sk_detector* dets, int ndets, void* const* outputs, int outputs_on_device,
                 const sk_options* opt);
 
+/* The complex exit wave of every position's probe through one potential (WavesDetector, STEM.scan(probe:potentials:
+ * scan:detector:), WavesDetector.swift:107-215; abTEM's WavesDetector(gpts)): rows x cols > 0 resamples each wave in
+ * Fourier space to that many points over the same extent (Waves.resampled, WavesResample.swift:16-49); 0 keeps the
+ * simulation grid. out_re / out_im: npos x rows x cols, position order kept. */
+int sk_hip_scan_waves(const sk_potential* p, const sk_probe* probe, int npos, const double* positions, int rows,
+                      int cols, double* out_re, double* out_im, const sk_options* opt);
+
 /* The values each detector returns per position (Detector.shape(for:beam:), Detectors.swift:84-88): writes the
  * shape (up to 2 ints) and returns the count, or a negative status for a refused detector. */
 int sk_hip_detector_shape(const sk_grid* grid, double energy, const sk_detector* det, int* shape);

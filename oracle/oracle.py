@@ -371,3 +371,42 @@ def frozen_displacements(n_atoms_numbers, sigmas, configurations, seed, directio
             if directions[ax]:
                 out[c, :, ax] = s * g[:, ax]
     return out
+
+
+# ---- exit waves over a scan (WavesDetector.swift:33-217, WavesResample.swift:16-66) ----------------------------
+def crop_indices(n, m):  # Waves.swift:90-96
+    if m >= n:
+        return list(range(n))
+    low = m // 2 + 1 if m % 2 == 1 else m // 2
+    return list(range(low)) + list(range(n - m // 2, n))
+
+
+def resample_map(n, m):  # WavesResample.swift:58-66: (source, destination) of every surviving frequency
+    if m <= n:
+        return [(0, 0)] if m == 1 else [(s, d) for d, s in enumerate(crop_indices(n, m))]
+    if n == 1:
+        return [(0, 0)]
+    positive = n // 2 if n % 2 == 0 else n // 2 + 1
+    return [(i, i if i < positive else m - n + i) for i in range(n)]
+
+
+def resample_wave(w, rows, cols):  # Waves.resampled (WavesResample.swift:16-49)
+    nx, ny = w.shape
+    F = fft2(np.ascontiguousarray(w, dtype=complex))
+    out = np.zeros((rows, cols), dtype=complex)
+    mx, my = resample_map(nx, rows), resample_map(ny, cols)
+    for sx, dx in mx:
+        for sy, dy in my:
+            out[dx, dy] = F[sx, sy]
+    return fft2(out, inverse=True) * (rows * cols / (nx * ny))
+
+
+def scan_waves(V, dz, extent, energy, cutoff_mrad, positions, gpts=None):
+    """STEM.scan(..., detector: WavesDetector) on the CPU path (WavesDetector.swift:107-215): the exit wave of every
+    position's probe, resampled to gpts = (rows, cols) when given; [npos, rows, cols] complex."""
+    nx, ny = V.shape[1:]
+    probes = [probe(nx, ny, extent, energy, cutoff_mrad, tuple(p)) for p in positions]
+    exits = multislice_batch(V, dz, extent, energy, probes)
+    if gpts is None:
+        return np.stack(exits)
+    return np.stack([resample_wave(e, gpts[0], gpts[1]) for e in exits])

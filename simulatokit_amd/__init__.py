@@ -7,8 +7,8 @@ device is missing, every entry point raises.
 """
 from .core import (AtomicStructure, Backend, ElectronBeam, ExecutionOptions, Grid, HipAccuracy, Normalization,
                    REGISTERED, SimulatoError, Waves)
-from .detector import (STEM, AnnularDetector, CenterOfMassDetector, PixelatedDetector, ScanMeasurement,
-                       SegmentedDetector, last_timing, set_timing)
+from .detector import (STEM, AnnularDetector, CenterOfMassDetector, PixelatedDetector, ScanMeasurement, ScanWaves,
+                       SegmentedDetector, WavesDetector, last_timing, set_timing)
 from .multislice import (ExitPlanes, FrozenPhonons, GaussianGenerator, LobatoParameterization, Multislice, Potential,
                          PreparedPotential, ProjectedPotential, Slicing, hip_is_available)
 from .probe import Aberrations, Aperture, GridScan, LineScan, Probe
@@ -19,4 +19,4 @@ __all__ = ["AtomicStructure", "Backend", "ElectronBeam", "ExecutionOptions", "Gr
            "PixelatedDetector", "ScanMeasurement", "ExitPlanes", "FrozenPhonons", "GaussianGenerator",
            "LobatoParameterization", "Multislice", "Potential", "PreparedPotential", "ProjectedPotential", "Slicing",
            "hip_is_available", "Aberrations", "Aperture", "GridScan", "Probe", "last_timing", "set_timing",
-           "TuneResult", "autotune", "SegmentedDetector", "LineScan"]
+           "TuneResult", "autotune", "SegmentedDetector", "LineScan", "WavesDetector", "ScanWaves"]

@@ -90,6 +90,8 @@ def lib():
         L.sk_hip_probes.argtypes = [C.POINTER(SkGrid), C.POINTER(SkProbe), C.c_int, _dp, _dp, _dp]
         L.sk_hip_scan.argtypes = [C.POINTER(_vp), C.c_int, C.POINTER(SkProbe), C.c_int, _dp, C.POINTER(SkDetector),
                                   C.c_int, C.POINTER(_vp), C.c_int, C.POINTER(SkOptions)]
+        L.sk_hip_scan_waves.argtypes = [_vp, C.POINTER(SkProbe), C.c_int, _dp, C.c_int, C.c_int, _dp, _dp,
+                                        C.POINTER(SkOptions)]
         L.sk_hip_detector_shape.argtypes = [C.POINTER(SkGrid), C.c_double, C.POINTER(SkDetector), _ip]
         L.sk_hip_last_timing.argtypes = [C.POINTER(SkTiming)]
         L.sk_hip_set_timing.argtypes = [C.c_int]
@@ -118,7 +120,7 @@ EXPORTED = ("sk_hip_available", "sk_hip_last_error", "sk_hip_potential_from_valu
             "sk_hip_potential_release", "sk_hip_potential_slice_count", "sk_hip_potential_values",
             "sk_hip_potential_transmissions", "sk_hip_run", "sk_hip_run_batch", "sk_hip_probes", "sk_hip_scan",
             "sk_hip_detector_shape", "sk_hip_last_timing", "sk_hip_set_timing", "sk_hip_potential_grid",
-            "sk_hip_autotune", "sk_hip_device_name", "sk_hip_potential_from_atoms_finite",
+            "sk_hip_autotune", "sk_hip_device_name", "sk_hip_potential_from_atoms_finite", "sk_hip_scan_waves",
             # include/simulato_surrogate.h
             "sk_conv5_forward", "sk_conv5_backward_data", "sk_conv5_backward_weights", "sk_relu_backward",
             "sk_separable_map", "sk_mse_grad", "sk_adam_step", "sk_to_bf16")

@@ -1237,6 +1237,101 @@ int sk_hip_probes(const sk_grid* grid, const sk_probe* probe, int npos, const do
   });
 }
 
+// (destination -> source) map of a Fourier resampling from n to m points (WavesResample.swift:58-66): the lowest m
+// frequencies of a crop, all n of a zero-pad, positive frequencies first; -1 for a destination no source fills
+static std::vector<int> resample_sources(int n, int m) {
+  std::vector<int> src(m, -1);
+  if (m <= n) {
+    if (m == 1) {
+      src[0] = 0;
+      return src;
+    }
+    const auto s = crop_indices(n, m);
+    for (int d = 0; d < m; ++d) src[d] = s[d];
+    return src;
+  }
+  if (n == 1) {
+    src[0] = 0;
+    return src;
+  }
+  const int positive = n % 2 == 0 ? n / 2 : n / 2 + 1;
+  for (int i = 0; i < n; ++i) src[i < positive ? i : m - n + i] = i;
+  return src;
+}
+
+int sk_hip_scan_waves(const sk_potential* p, const sk_probe* probe, int npos, const double* positions, int rows,
+                      int cols, double* out_re, double* out_im, const sk_options* opt) {
+  return guarded([&] {
+    Context& c = require();
+    hipStream_t s = c.stream;
+    set_col_tile(c, opt);
+    if (!p || !positions || !out_re || !out_im || npos < 1) fail(SK_ERR_INVALID, "invalid waves scan");
+    check_probe(probe);
+    const sk_grid g = p->grid;
+    const int nx = g.nx, ny = g.ny;
+    const size_t N = (size_t)nx * ny;
+    if (rows <= 0 || cols <= 0) {
+      rows = nx;
+      cols = ny;
+    }
+    if (rows < 2 || cols < 2) fail(SK_ERR_INVALID, "the waves detector needs at least 2 points per axis");
+    const bool resample = rows != nx || cols != ny;
+    const size_t M = (size_t)rows * cols;
+    const bool fast = (!opt || opt->fast_path != 0) && c.precision == 0;
+    const int B = default_batch(*p, npos, opt);
+    Workspace& w = ws();
+    w.G.ensure((size_t)B * nx * p->nc);
+    w.full.ensure((size_t)B * N);
+    w.planes.ensure((size_t)B * std::max(N, M));
+    w.rx.ensure((size_t)B * nx);
+    w.ry.ensure((size_t)B * ny);
+    w.tmp_re.ensure((size_t)B * M);
+    w.tmp_im.ensure((size_t)B * M);
+    w.spec.ensure(N);
+    const bool fits = build_spectrum(g, *probe, 1.0 / double(N), &p->keepinv, w.spec.p);
+    w.positions.upload(positions, 2 * (size_t)npos, s);
+    DevBuf<int> srcx, srcy;
+    DevBuf<float2> spectra;
+    if (resample) {
+      srcx.upload(resample_sources(nx, rows), s);
+      srcy.upload(resample_sources(ny, cols), s);
+      spectra.alloc((size_t)B * M);
+    }
+    std::vector<int> plane_out(p->nslices, -1);
+    plane_out[p->nslices - 1] = 0;
+    Timer tm;
+    for (int b0 = 0; b0 < npos; b0 += B) {
+      const int nb = std::min(B, npos - b0);
+      launch_ramps(nx, ny, g.extent_x / nx, g.extent_y / ny, w.positions.p, b0, nullptr, nb, w.rx.p, w.ry.p, s);
+      ColArgs init = col_args(*p, fits ? w.G.p : w.full.p);
+      init.spec = w.spec.p;
+      init.rx = w.rx.p;
+      init.ry = w.ry.p;
+      init.flags = COL_GENERATE | COL_INV | COL_OUT;
+      if (!fits) {
+        init.nc = ny;
+        init.keep = identity_map(ny);
+      }
+      col_pass(init, nb, fast && fits, s);
+      // every slice, then the exit wave in real space (the probe's own normalisation: sum |psi|^2 = 1 / N)
+      slice_loop(*p, nb, w.G.p, false, w.full.p, fits ? 0 : 1, plane_out, w.planes.p, nullptr, fast, tm);
+      const float2* result = w.planes.p;
+      if (resample) {
+        // WavesResample.swift:16-49: FFT, keep / pad the lowest frequencies, inverse FFT times (rows cols) / N:
+        // the inverse runs unscaled and the whole factor is 1 / N
+        fft2_batch(w.planes.p, nx, ny, nb, -1, nullptr, 1.0f, s);
+        launch_resample_gather(w.planes.p, nx, ny, srcx.p, srcy.p, rows, cols, nb, spectra.p, s);
+        fft2_batch(spectra.p, rows, cols, nb, +1, nullptr, float(1.0 / double(N)), s);
+        result = spectra.p;
+      }
+      launch_complex_to_split(result, w.tmp_re.p, w.tmp_im.p, nb * M, 1.0, s);
+      SK_HIP(hipMemcpyAsync(out_re + (size_t)b0 * M, w.tmp_re.p, nb * M * sizeof(double), hipMemcpyDeviceToHost, s));
+      SK_HIP(hipMemcpyAsync(out_im + (size_t)b0 * M, w.tmp_im.p, nb * M * sizeof(double), hipMemcpyDeviceToHost, s));
+      SK_HIP(hipStreamSynchronize(s));
+    }
+  });
+}
+
 int sk_hip_detector_shape(const sk_grid* grid, double energy, const sk_detector* det, int* shape) {
   int count = 0;
   int rc = guarded([&] {

@@ -412,6 +412,24 @@ void launch_real_part(const float2* in, double* out, size_t n, hipStream_t s) {
   SK_HIP(hipGetLastError());
 }
 
+// out[b][dx][dy] = in[b][srcx[dx]][srcy[dy]], zero where a source is -1 (a Fourier crop or zero-pad)
+__global__ void resample_gather_kernel(const float2* in, int nx, int ny, const int* srcx, const int* srcy, int rows,
+                                       int cols, size_t batch, float2* out) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t M = (size_t)rows * cols;
+  if (i >= batch * M) return;
+  const size_t b = i / M, r = i % M;
+  const int dx = int(r / cols), dy = int(r % cols);
+  const int sx = srcx[dx], sy = srcy[dy];
+  out[i] = (sx >= 0 && sy >= 0) ? in[b * (size_t)nx * ny + (size_t)sx * ny + sy] : make_float2(0.f, 0.f);
+}
+void launch_resample_gather(const float2* in, int nx, int ny, const int* srcx, const int* srcy, int rows, int cols,
+                            size_t batch, float2* out, hipStream_t s) {
+  hipLaunchKernelGGL(resample_gather_kernel, grid_for(batch * rows * cols), dim3(kThreads), 0, s, in, nx, ny, srcx,
+                     srcy, rows, cols, batch, out);
+  SK_HIP(hipGetLastError());
+}
+
 // out = acc * inv (float64 or float32)
 __global__ void finalise_kernel(const double* acc, size_t n, double inv, void* out, int f32) {
   size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;

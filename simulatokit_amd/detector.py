@@ -13,7 +13,7 @@ from typing import List, Optional, Sequence, Tuple, Union
 import numpy as np
 
 from . import _native as N
-from .core import ExecutionOptions, SimulatoError
+from .core import ElectronBeam, ExecutionOptions, Grid, Normalization, SimulatoError, Waves
 from .multislice import AnyPotential, PreparedPotential, _options, _prepared, _raise, _sk_grid
 from .probe import GridScan, LineScan, Probe
 
@@ -127,6 +127,44 @@ def detector_shape(d: Detector, grid, energy: float) -> Tuple[int, ...]:
     return (int(shape[0]), int(shape[1]))
 
 
+@dataclass(frozen=True)
+class WavesDetector:
+    """The complex exit wave (abTEM's WavesDetector; WavesDetector.swift:33-65): on the simulation grid, or with rows
+    and columns resampled in Fourier space to that many points over the same extent."""
+
+    rows: Optional[int] = None
+    columns: Optional[int] = None
+
+    def __post_init__(self):
+        if (self.rows is None) != (self.columns is None):
+            raise SimulatoError("invalidDetector", "give both rows and columns, or neither")
+        if self.rows is not None and (self.rows < 2 or self.columns < 2):
+            raise SimulatoError("unsupportedGridLength", "the waves detector needs at least 2 points per axis")
+
+
+@dataclass
+class ScanWaves:
+    """The exit waves of a scan, [positions..., rows, columns] complex128 (ScanWaves, WavesDetector.swift:67-105)."""
+
+    scan_shape: Tuple[int, ...]
+    grid: Grid
+    beam: ElectronBeam
+    values: np.ndarray
+
+    @property
+    def rows(self) -> int:
+        return self.grid.nx
+
+    @property
+    def columns(self) -> int:
+        return self.grid.ny
+
+    def __getitem__(self, ix_iy) -> Waves:
+        ix, iy = ix_iy
+        field = self.values.reshape(self.scan_shape + (self.rows, self.columns))[ix, iy]
+        return Waves(field, self.grid, self.beam, Normalization.RECIPROCAL_SPACE)  # a probe's exit wave
+
+
 class STEM:
     @staticmethod
     def scan(probe: Probe, potentials: Sequence[AnyPotential], scan: Union[GridScan, np.ndarray],
@@ -167,6 +205,46 @@ class STEM:
         return [ScanMeasurement(tuple(scan_shape), s, None if on_device else a.reshape(-1))
                 for s, a in zip(shapes, outs)]
 
+    @staticmethod
+    def scan_waves(probe: Probe, potential: AnyPotential, scan: Union[GridScan, np.ndarray],
+                   detector: WavesDetector = WavesDetector(), options: Optional[ExecutionOptions] = None) -> ScanWaves:
+        """The exit wave of every position's probe through one potential (STEM.scan(probe:potentials:scan:detector:),
+        WavesDetector.swift:107-124), resampled when the detector has rows and columns."""
+        if isinstance(potential, (list, tuple)):
+            if len(potential) != 1:
+                raise SimulatoError("severalPotentials", "the waves detector takes one potential; scan the "
+                                    "configurations with STEM.scan_frozen_phonon_waves")
+            potential = potential[0]
+        if not potential.grid.matches(probe.grid):
+            raise SimulatoError("gridMismatch", "the waves and the potential are on different grids")
+        if isinstance(scan, (GridScan, LineScan)):
+            positions, scan_shape = scan.positions, tuple(scan.shape)
+        else:
+            positions = np.asarray(scan, dtype=np.float64).reshape(-1, 2)
+            scan_shape = (len(positions),)
+        prepared = _prepared(potential, probe.beam, probe.tilt)
+        rows = detector.rows or probe.grid.nx
+        cols = detector.columns or probe.grid.ny
+        pos = np.ascontiguousarray(positions, dtype=np.float64)
+        re = np.empty((len(pos), rows, cols))
+        im = np.empty_like(re)
+        opt = _options(options)
+        _raise(N.lib().sk_hip_scan_waves(prepared.handle, C.byref(probe.descriptor()), len(pos), N.dptr(pos),
+                                         int(detector.rows or 0), int(detector.columns or 0), N.dptr(re), N.dptr(im),
+                                         C.byref(opt)), "STEM.scan_waves")
+        grid = Grid(rows, cols, probe.grid.extent_x, probe.grid.extent_y)
+        return ScanWaves(scan_shape, grid, probe.beam, re + 1j * im)
+
+    @staticmethod
+    def scan_frozen_phonon_waves(probe: Probe, potentials: Sequence[AnyPotential], scan,
+                                 detector: WavesDetector = WavesDetector(),
+                                 options: Optional[ExecutionOptions] = None) -> List[ScanWaves]:
+        """One ScanWaves per configuration (STEM.scanFrozenPhonons, WavesDetector.swift:143-181): complex waves are
+        not averaged over configurations."""
+        if not potentials:
+            raise SimulatoError("invalidScan", "invalid scan: no potential")
+        return [STEM.scan_waves(probe, p, scan, detector, options) for p in potentials]
+
 
 def last_timing() -> dict:
     """Device timing of the last scan (see sk_hip_last_timing)."""
@@ -180,4 +258,4 @@ def set_timing(enabled: bool):
 
 
 __all__ = ["AnnularDetector", "PixelatedDetector", "CenterOfMassDetector", "SegmentedDetector", "ScanMeasurement",
-           "STEM", "last_timing", "set_timing", "detector_shape"]
+           "STEM", "WavesDetector", "ScanWaves", "last_timing", "set_timing", "detector_shape"]

@@ -87,6 +87,12 @@ def main():
                      for s in js["scans"]]
     dump("linescan", js, blob, names, meta)
 
+    # exit waves over a scan, on the simulation grid and Fourier-resampled (tools/pins/waves/pins.py)
+    js, blob = load("waves-pins.json")
+    names = sorted(js["arrays"]["index"].keys())
+    meta = {k: v for k, v in js.items() if k != "arrays"}
+    dump("waves", js, blob, names, meta)
+
 
 if __name__ == "__main__":
     main()
