@@ -139,6 +139,26 @@ int sk_hip_run(const sk_potential* p, const double* in_re, const double* in_im, 
 int sk_hip_run_batch(const sk_potential* p, int count, const double* in_re, const double* in_im, double* out_re,
                      double* out_im, const sk_options* opt);
 
+/* --- HRTEM imaging -------------------------------------------------------------------------------------------- */
+
+/* An objective lens (ContrastTransferFunction, ContrastTransferFunction.swift:36-71). */
+typedef struct sk_ctf {
+  double aberrations[25];  /* sk_probe order */
+  double semiangle_mrad;   /* objective aperture; +inf (or <= 0 is refused) for none */
+  int soft;                /* soft aperture edge */
+  double focal_spread;     /* Angstrom, 1/e half-width Delta of the temporal envelope; 0 for none */
+  double angular_spread;   /* mrad, beta of the spatial envelope; 0 for none */
+} sk_ctf;
+
+/* The CTF kernel H on a grid (ContrastTransferFunction.kernel, ContrastTransferFunction.swift:84-111), evaluated in
+ * float64 on the device: out_re / out_im nx x ny, FFT order. */
+int sk_hip_ctf_kernel(const sk_grid* grid, double energy, const sk_ctf* ctf, double* out_re, double* out_im);
+
+/* `count` waves with the CTF applied (Waves.applying, ContrastTransferFunction.swift:140-153): IFFT(FFT(psi) H);
+ * the intensity of the result is the HRTEM image. in/out: count x nx x ny split float64 (may alias). */
+int sk_hip_apply_ctf(const sk_grid* grid, double energy, const sk_ctf* ctf, int count, const double* in_re,
+                     const double* in_im, double* out_re, double* out_im);
+
 /* --- probes ------------------------------------------------------------------------------------------------- */
 
 /* The probes at `npos` positions [npos x 2] Angstrom as the scan builds them (Probe.build(at:),

@@ -410,3 +410,23 @@ def scan_waves(V, dz, extent, energy, cutoff_mrad, positions, gpts=None):
     if gpts is None:
         return np.stack(exits)
     return np.stack([resample_wave(e, gpts[0], gpts[1]) for e in exits])
+
+
+# ---- contrast transfer function (ContrastTransferFunction.swift:36-153) ----------------------------------------
+def ctf_kernel(nx, ny, extent, energy, aberrations=None, cutoff_mrad=np.inf, soft=True, focal_spread=0.0,
+               angular_spread=0.0):
+    out = np.empty((nx, ny), dtype=complex)
+    lib().orc_ctf_kernel(C.c_int(nx), C.c_int(ny), C.c_double(extent[0]), C.c_double(extent[1]), C.c_double(energy),
+                         _d(aberration_vector(aberrations)), C.c_double(cutoff_mrad), C.c_int(1 if soft else 0),
+                         C.c_double(focal_spread), C.c_double(angular_spread), out.ctypes.data_as(_dp))
+    return out
+
+
+def apply_ctf(wave, extent, energy, aberrations=None, cutoff_mrad=np.inf, soft=True, focal_spread=0.0,
+              angular_spread=0.0):
+    w = np.array(wave, dtype=complex, copy=True, order="C")
+    nx, ny = w.shape
+    lib().orc_apply_ctf(C.c_int(nx), C.c_int(ny), C.c_double(extent[0]), C.c_double(extent[1]), C.c_double(energy),
+                        _d(aberration_vector(aberrations)), C.c_double(cutoff_mrad), C.c_int(1 if soft else 0),
+                        C.c_double(focal_spread), C.c_double(angular_spread), w.ctypes.data_as(_dp))
+    return w

@@ -645,6 +645,66 @@ std::vector<double> aperture_values(const Grid& g, double cutoff_mrad, int soft,
   return out;
 }
 
+// abTEM's dchi/dalpha and the azimuthal term dchi/dphi (ContrastTransferFunction.swift:12-33)
+void chi_derivatives(const double* c, double a, double p, double lambda, double& dk, double& dp) {
+  enum { C10, C12, phi12, C21, phi21, C23, phi23, C30, C32, phi32, C34, phi34, C41, phi41, C43, phi43, C45, phi45,
+         C50, C52, phi52, C54, phi54, C56, phi56 };
+  const double t1 = (c[C12] * std::cos(2 * (p - c[phi12])) + c[C10]) * a;
+  const double t2 = (c[C23] * std::cos(3 * (p - c[phi23])) + c[C21] * std::cos(1 * (p - c[phi21]))) * std::pow(a, 2);
+  const double t3 = (c[C34] * std::cos(4 * (p - c[phi34])) + c[C32] * std::cos(2 * (p - c[phi32])) + c[C30]) *
+                    std::pow(a, 3);
+  const double t4 = (c[C45] * std::cos(5 * (p - c[phi45])) + c[C43] * std::cos(3 * (p - c[phi43])) +
+                     c[C41] * std::cos(1 * (p - c[phi41]))) * std::pow(a, 4);
+  const double t5 = (c[C56] * std::cos(6 * (p - c[phi56])) + c[C54] * std::cos(4 * (p - c[phi54])) +
+                     c[C52] * std::cos(2 * (p - c[phi52])) + c[C50]) * std::pow(a, 5);
+  dk = 2 * kPi / lambda * (t1 + t2 + t3 + t4 + t5);
+  const double u1 = 1 / 2.0 * (2.0 * c[C12] * std::sin(2.0 * (p - c[phi12]))) * a;
+  const double u2 = 1 / 3.0 * (3.0 * c[C23] * std::sin(3.0 * (p - c[phi23])) + 1.0 * c[C21] * std::sin(1.0 * (p - c[phi21]))) *
+                    std::pow(a, 2);
+  const double u3 = 1 / 4.0 * (4.0 * c[C34] * std::sin(4.0 * (p - c[phi34])) + 2.0 * c[C32] * std::sin(2.0 * (p - c[phi32]))) *
+                    std::pow(a, 3);
+  const double u4 = 1 / 5.0 * (5.0 * c[C45] * std::sin(5.0 * (p - c[phi45])) + 3.0 * c[C43] * std::sin(3.0 * (p - c[phi43])) +
+                               1.0 * c[C41] * std::sin(1.0 * (p - c[phi41]))) * std::pow(a, 4);
+  const double u5 = (1 / 6.0) * (6.0 * c[C56] * std::sin(6.0 * (p - c[phi56])) + 4.0 * c[C54] * std::sin(4.0 * (p - c[phi54])) +
+                                 2.0 * c[C52] * std::sin(2.0 * (p - c[phi52]))) * std::pow(a, 5);
+  dp = -2 * kPi / lambda * (u1 + u2 + u3 + u4 + u5);
+}
+
+// ContrastTransferFunction.kernel (ContrastTransferFunction.swift:84-111): e^{-i chi} x E_s (beta != 0) x E_t
+// (Delta != 0) x aperture (finite cutoff); E_t = exp(-((pi/2/lambda) Delta alpha^2)^2),
+// E_s = exp(-sgn(beta) (beta/2)^2 ((dchi/dalpha)^2 + (dchi/dphi)^2)), beta in mrad / 1e3
+std::vector<cd> ctf_kernel(const Grid& g, double lambda, const double* ab, double cutoff_mrad, int soft,
+                           double focal_spread, double angular_spread) {
+  const bool has_ap = std::isfinite(cutoff_mrad);
+  std::vector<double> ap;
+  if (has_ap) ap = aperture_values(g, cutoff_mrad, soft, lambda);
+  auto kx = g.kx(), ky = g.ky();
+  std::vector<cd> h(g.count());
+  for (int ix = 0; ix < g.nx; ++ix)
+    for (int iy = 0; iy < g.ny; ++iy) {
+      const size_t i = (size_t)ix * g.ny + iy;
+      const double alpha = std::sqrt(kx[ix] * kx[ix] + ky[iy] * ky[iy]) * lambda;
+      const double phi = std::atan2(ky[iy], kx[ix]);
+      const double x = chi(ab, alpha, phi, lambda);
+      double re = std::cos(-x), im = std::sin(-x);
+      double factor = 1.0;
+      if (angular_spread != 0) {
+        const double b = angular_spread / 1e3;
+        double dk, dp;
+        chi_derivatives(ab, alpha, phi, lambda, dk, dp);
+        const double sgn = b > 0 ? 1.0 : (b < 0 ? -1.0 : 0.0);
+        factor *= std::exp(-sgn * (b / 2) * (b / 2) * (dk * dk + dp * dp));
+      }
+      if (focal_spread != 0) {
+        const double t = 0.5 * kPi / lambda * focal_spread * alpha * alpha;
+        factor *= std::exp(-(t * t));
+      }
+      if (has_ap) factor *= ap[i];
+      h[i] = cd(re * factor, im * factor);
+    }
+  return h;
+}
+
 // Probe.build(at:) (Probe.swift:72-109): Psi(k) = shift * A * e^{-i chi}, normalised, then IFFT
 std::vector<cd> probe_build(const Grid& g, const FFT2& fft, const ProbeParams& pr, double posx, double posy) {
   double lambda = wavelength(pr.energy);
@@ -1106,6 +1166,28 @@ void orc_diffraction_pattern(int nx, int ny, double ex, double ey, const double*
     auto c = cropped(g, P.data(), mx, my);
     std::copy(c.begin(), c.end(), out);
   }
+}
+
+// The CTF kernel H [nx x ny] interleaved complex (ContrastTransferFunction.swift:84-111); cutoff_mrad may be +inf.
+void orc_ctf_kernel(int nx, int ny, double ex, double ey, double energy, const double* ab, double cutoff_mrad,
+                    int soft, double focal_spread, double angular_spread, double* out) {
+  auto h = ctf_kernel(Grid{nx, ny, ex, ey}, wavelength(energy), ab, cutoff_mrad, soft, focal_spread, angular_spread);
+  std::copy(reinterpret_cast<const double*>(h.data()), reinterpret_cast<const double*>(h.data()) + 2 * h.size(), out);
+}
+
+// Waves.applying(ctf) (ContrastTransferFunction.swift:140-153): IFFT(FFT(psi) H), in place on `wave`.
+void orc_apply_ctf(int nx, int ny, double ex, double ey, double energy, const double* ab, double cutoff_mrad, int soft,
+                   double focal_spread, double angular_spread, double* wave) {
+  Grid g{nx, ny, ex, ey};
+  auto h = ctf_kernel(g, wavelength(energy), ab, cutoff_mrad, soft, focal_spread, angular_spread);
+  FFT2 fft(nx, ny);
+  cd* w = reinterpret_cast<cd*>(wave);
+  fft.forward(w);
+  for (size_t i = 0; i < h.size(); ++i) {
+    const double a = w[i].real(), b = w[i].imag(), c = h[i].real(), d = h[i].imag();
+    w[i] = cd(a * c - b * d, a * d + b * c);
+  }
+  fft.inverse(w);
 }
 
 int orc_pixelated_size(int nx, int ny, double ex, double ey, int max_angle, int* mx, int* my) {

@@ -11,7 +11,7 @@ from .detector import (STEM, AnnularDetector, CenterOfMassDetector, PixelatedDet
                        SegmentedDetector, WavesDetector, last_timing, set_timing)
 from .multislice import (ExitPlanes, FrozenPhonons, GaussianGenerator, LobatoParameterization, Multislice, Potential,
                          PreparedPotential, ProjectedPotential, Slicing, hip_is_available)
-from .probe import Aberrations, Aperture, GridScan, LineScan, Probe
+from .probe import Aberrations, Aperture, ContrastTransferFunction, GridScan, LineScan, Probe
 from .tune import TuneResult, autotune
 
 __all__ = ["AtomicStructure", "Backend", "ElectronBeam", "ExecutionOptions", "Grid", "HipAccuracy", "Normalization",
@@ -19,4 +19,5 @@ __all__ = ["AtomicStructure", "Backend", "ElectronBeam", "ExecutionOptions", "Gr
            "PixelatedDetector", "ScanMeasurement", "ExitPlanes", "FrozenPhonons", "GaussianGenerator",
            "LobatoParameterization", "Multislice", "Potential", "PreparedPotential", "ProjectedPotential", "Slicing",
            "hip_is_available", "Aberrations", "Aperture", "GridScan", "Probe", "last_timing", "set_timing",
-           "TuneResult", "autotune", "SegmentedDetector", "LineScan", "WavesDetector", "ScanWaves"]
+           "TuneResult", "autotune", "SegmentedDetector", "LineScan", "WavesDetector", "ScanWaves",
+           "ContrastTransferFunction"]

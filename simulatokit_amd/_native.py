@@ -43,6 +43,11 @@ class SkTiming(C.Structure):
                 ("total_ms", C.c_double)]
 
 
+class SkCtf(C.Structure):
+    _fields_ = [("aberrations", C.c_double * 25), ("semiangle_mrad", C.c_double), ("soft", C.c_int),
+                ("focal_spread", C.c_double), ("angular_spread", C.c_double)]
+
+
 SK_TUNE_MAX = 32
 
 
@@ -92,6 +97,8 @@ def lib():
                                   C.c_int, C.POINTER(_vp), C.c_int, C.POINTER(SkOptions)]
         L.sk_hip_scan_waves.argtypes = [_vp, C.POINTER(SkProbe), C.c_int, _dp, C.c_int, C.c_int, _dp, _dp,
                                         C.POINTER(SkOptions)]
+        L.sk_hip_ctf_kernel.argtypes = [C.POINTER(SkGrid), C.c_double, C.POINTER(SkCtf), _dp, _dp]
+        L.sk_hip_apply_ctf.argtypes = [C.POINTER(SkGrid), C.c_double, C.POINTER(SkCtf), C.c_int, _dp, _dp, _dp, _dp]
         L.sk_hip_detector_shape.argtypes = [C.POINTER(SkGrid), C.c_double, C.POINTER(SkDetector), _ip]
         L.sk_hip_last_timing.argtypes = [C.POINTER(SkTiming)]
         L.sk_hip_set_timing.argtypes = [C.c_int]
@@ -121,6 +128,7 @@ EXPORTED = ("sk_hip_available", "sk_hip_last_error", "sk_hip_potential_from_valu
             "sk_hip_potential_transmissions", "sk_hip_run", "sk_hip_run_batch", "sk_hip_probes", "sk_hip_scan",
             "sk_hip_detector_shape", "sk_hip_last_timing", "sk_hip_set_timing", "sk_hip_potential_grid",
             "sk_hip_autotune", "sk_hip_device_name", "sk_hip_potential_from_atoms_finite", "sk_hip_scan_waves",
+            "sk_hip_ctf_kernel", "sk_hip_apply_ctf",
             # include/simulato_surrogate.h
             "sk_conv5_forward", "sk_conv5_backward_data", "sk_conv5_backward_weights", "sk_relu_backward",
             "sk_separable_map", "sk_mse_grad", "sk_adam_step", "sk_to_bf16")

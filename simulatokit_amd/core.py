@@ -189,6 +189,12 @@ class Waves:
     def intensity(self) -> np.ndarray:
         return self.field.real ** 2 + self.field.imag ** 2
 
+    def apply_ctf(self, ctf) -> "Waves":
+        """The waves with a ContrastTransferFunction applied on the device (Waves.applying,
+        ContrastTransferFunction.swift:140-153)."""
+        from .probe import apply_ctf
+        return apply_ctf(self, ctf)
+
     @property
     def total_intensity(self) -> float:
         return float(self.intensity().sum())

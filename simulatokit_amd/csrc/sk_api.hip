@@ -1237,6 +1237,78 @@ int sk_hip_probes(const sk_grid* grid, const sk_probe* probe, int npos, const do
   });
 }
 
+static CtfParams ctf_params(const sk_ctf* ctf) {
+  if (!ctf) fail(SK_ERR_INVALID, "no CTF");
+  for (int k = 0; k < 25; ++k)
+    if (!std::isfinite(ctf->aberrations[k])) fail(SK_ERR_INVALID, "invalid aberration: every coefficient must be finite");
+  if (!(ctf->semiangle_mrad > 0)) fail(SK_ERR_INVALID, "invalid aperture: the semiangle must be positive");
+  if (!(std::isfinite(ctf->focal_spread) && ctf->focal_spread >= 0))
+    fail(SK_ERR_INVALID, "invalid aberration: focal spread must be finite and >= 0");
+  if (!(std::isfinite(ctf->angular_spread) && ctf->angular_spread >= 0))
+    fail(SK_ERR_INVALID, "invalid aberration: angular spread must be finite and >= 0");
+  CtfParams c{};
+  for (int k = 0; k < 25; ++k) c.ab[k] = ctf->aberrations[k];
+  c.cutoff_mrad = ctf->semiangle_mrad;
+  c.soft = ctf->soft;
+  c.focal_spread = ctf->focal_spread;
+  c.angular_spread = ctf->angular_spread;
+  return c;
+}
+
+int sk_hip_ctf_kernel(const sk_grid* grid, double energy, const sk_ctf* ctf, double* out_re, double* out_im) {
+  return guarded([&] {
+    Context& c = require();
+    hipStream_t s = c.stream;
+    check_grid(grid);
+    if (!out_re || !out_im || !(std::isfinite(energy) && energy > 0)) fail(SK_ERR_INVALID, "invalid argument");
+    const CtfParams cp = ctf_params(ctf);
+    const size_t N = (size_t)grid->nx * grid->ny;
+    DevBuf<double> re, im;
+    re.alloc(N);
+    im.alloc(N);
+    launch_ctf_kernel(grid->nx, grid->ny, grid->extent_x, grid->extent_y, wavelength(energy), cp, 1.0f, nullptr,
+                      re.p, im.p, s);
+    SK_HIP(hipMemcpyAsync(out_re, re.p, N * sizeof(double), hipMemcpyDeviceToHost, s));
+    SK_HIP(hipMemcpyAsync(out_im, im.p, N * sizeof(double), hipMemcpyDeviceToHost, s));
+    SK_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int sk_hip_apply_ctf(const sk_grid* grid, double energy, const sk_ctf* ctf, int count, const double* in_re,
+                     const double* in_im, double* out_re, double* out_im) {
+  return guarded([&] {
+    Context& c = require();
+    hipStream_t s = c.stream;
+    check_grid(grid);
+    if (count < 0 || (count > 0 && (!in_re || !in_im || !out_re || !out_im)) || !(std::isfinite(energy) && energy > 0))
+      fail(SK_ERR_INVALID, "invalid argument");
+    const CtfParams cp = ctf_params(ctf);
+    if (count == 0) return;
+    const int nx = grid->nx, ny = grid->ny;
+    const size_t N = (size_t)nx * ny;
+    // the kernel H (complex64); the transforms run unscaled and the inverse carries 1 / N
+    DevBuf<float2> H, buf;
+    H.alloc(N);
+    launch_ctf_kernel(nx, ny, grid->extent_x, grid->extent_y, wavelength(energy), cp, 1.0f, H.p, nullptr, nullptr, s);
+    const int B = (int)std::max<size_t>(1, std::min<size_t>(count, (size_t(1) << 30) / (N * sizeof(float2))));
+    buf.alloc((size_t)B * N);
+    DevBuf<double> re, im;
+    re.alloc((size_t)B * N);
+    im.alloc((size_t)B * N);
+    for (int b0 = 0; b0 < count; b0 += B) {
+      const int nb = std::min(B, count - b0);
+      SK_HIP(hipMemcpyAsync(re.p, in_re + (size_t)b0 * N, nb * N * sizeof(double), hipMemcpyHostToDevice, s));
+      SK_HIP(hipMemcpyAsync(im.p, in_im + (size_t)b0 * N, nb * N * sizeof(double), hipMemcpyHostToDevice, s));
+      launch_real_to_complex(re.p, im.p, buf.p, nb * N, s);
+      fft2_batch(buf.p, nx, ny, nb, -1, H.p, float(1.0 / double(N)), s);
+      launch_complex_to_split(buf.p, re.p, im.p, nb * N, 1.0, s);
+      SK_HIP(hipMemcpyAsync(out_re + (size_t)b0 * N, re.p, nb * N * sizeof(double), hipMemcpyDeviceToHost, s));
+      SK_HIP(hipMemcpyAsync(out_im + (size_t)b0 * N, im.p, nb * N * sizeof(double), hipMemcpyDeviceToHost, s));
+      SK_HIP(hipStreamSynchronize(s));
+    }
+  });
+}
+
 // (destination -> source) map of a Fourier resampling from n to m points (WavesResample.swift:58-66): the lowest m
 // frequencies of a crop, all n of a zero-pad, positive frequencies first; -1 for a destination no source fills
 static std::vector<int> resample_sources(int n, int m) {

@@ -265,6 +265,16 @@ int pipe_positions_per_block();
 void launch_phase_to_complex(const double* V, double sigma, float2* out, size_t n, hipStream_t s);
 void launch_real_to_complex(const double* re, const double* im, float2* out, size_t n, hipStream_t s);
 void launch_complex_to_split(const float2* in, double* re, double* im, size_t n, double scale, hipStream_t s);
+// the CTF's parameters on the device (ContrastTransferFunction.swift:36-71)
+struct CtfParams {
+  double ab[25];
+  double cutoff_mrad;  // +inf: no aperture
+  int soft;
+  double focal_spread;
+  double angular_spread;
+};
+void launch_ctf_kernel(int nx, int ny, double ex, double ey, double lambda, const CtfParams& c, float scale,
+                       float2* out, double* re, double* im, hipStream_t s);
 void launch_resample_gather(const float2* in, int nx, int ny, const int* srcx, const int* srcy, int rows, int cols,
                             size_t batch, float2* out, hipStream_t s);
 void launch_antialias(int nx, int ny, double ex, double ey, double* out, hipStream_t s);

@@ -148,6 +148,21 @@ struct Aberr {
 };
 
 // Probe.centredSpectrum before normalisation (Probe.swift:113-129; Aperture.values, Probe.swift:25-48)
+// Aperture.values (Aperture.swift:25-48) at frequency (ix, iy): 1 inside alpha_c; the soft edge
+// clip((alpha_c - alpha) / d + 1/2, 0, 1) with d the angular step along the direction phi, A(0) = 1; +inf = no aperture
+__device__ double aperture_d(int ix, int iy, int nx, int ny, double sx, double sy, double alpha, double phi,
+                             double lambda, double cutoff_mrad, int soft) {
+  if (isinf(cutoff_mrad)) return 1.0;
+  const double cutoff = cutoff_mrad * 1e-3;
+  if (!soft) return alpha <= cutoff ? 1.0 : 0.0;
+  if (ix == 0 && iy == 0) return 1.0;
+  const double rsx = 1 / (double(nx) * sx), rsy = 1 / (double(ny) * sy);
+  const double dax = rsx * 1e3 * lambda * 1e-3, day = rsy * 1e3 * lambda * 1e-3;
+  const double a = cos(phi) * dax, b = sin(phi) * day;
+  const double d = sqrt(__dadd_rn(__dmul_rn(a, a), __dmul_rn(b, b)));
+  return fmin(fmax((cutoff - alpha) / d + 0.5, 0.0), 1.0);
+}
+
 __global__ void probe_spectrum_kernel(int nx, int ny, double ex, double ey, double lambda, double cutoff_mrad,
                                       int soft, Aberr ab, double* re, double* im) {
   size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -157,23 +172,7 @@ __global__ void probe_spectrum_kernel(int nx, int ny, double ex, double ey, doub
   const double kx = fftfreq_d(ix, nx, sx), ky = fftfreq_d(iy, ny, sy);
   const double alpha = sqrt(__dadd_rn(__dmul_rn(kx, kx), __dmul_rn(ky, ky))) * lambda;
   const double phi = atan2(ky, kx);
-  double ap = 1.0;
-  if (!isinf(cutoff_mrad)) {
-    const double cutoff = cutoff_mrad * 1e-3;
-    if (soft) {
-      if (ix == 0 && iy == 0) {
-        ap = 1.0;
-      } else {
-        const double rsx = 1 / (double(nx) * sx), rsy = 1 / (double(ny) * sy);
-        const double dax = rsx * 1e3 * lambda * 1e-3, day = rsy * 1e3 * lambda * 1e-3;
-        const double a = cos(phi) * dax, b = sin(phi) * day;
-        const double d = sqrt(__dadd_rn(__dmul_rn(a, a), __dmul_rn(b, b)));
-        ap = fmin(fmax((cutoff - alpha) / d + 0.5, 0.0), 1.0);
-      }
-    } else {
-      ap = alpha <= cutoff ? 1.0 : 0.0;
-    }
-  }
+  const double ap = aperture_d(ix, iy, nx, ny, sx, sy, alpha, phi, lambda, cutoff_mrad, soft);
   const double ch = chi_d(ab.c, alpha, phi, lambda);
   double sn, cs;
   sincos(-ch, &sn, &cs);
@@ -186,6 +185,70 @@ void launch_probe_spectrum(int nx, int ny, double ex, double ey, double lambda, 
   for (int k = 0; k < 25; ++k) ab.c[k] = abv[k];
   hipLaunchKernelGGL(probe_spectrum_kernel, grid_for((size_t)nx * ny), dim3(kThreads), 0, s, nx, ny, ex, ey, lambda,
                      cutoff, soft, ab, re, im);
+  SK_HIP(hipGetLastError());
+}
+
+// abTEM's dchi/dalpha and azimuthal term dchi/dphi (ContrastTransferFunction.swift:12-33), coefficients in the
+// sk_probe order C10 C12 phi12 C21 phi21 C23 phi23 C30 C32 phi32 C34 phi34 C41 phi41 C43 phi43 C45 phi45 C50 C52
+// phi52 C54 phi54 C56 phi56
+__device__ void chi_derivatives_d(const double* c, double a, double p, double lambda, double& dk, double& dp) {
+  const double a2 = a * a, a3 = a2 * a, a4 = a3 * a, a5 = a4 * a;
+  const double t = (c[1] * cos(2 * (p - c[2])) + c[0]) * a +
+                   (c[5] * cos(3 * (p - c[6])) + c[3] * cos(p - c[4])) * a2 +
+                   (c[10] * cos(4 * (p - c[11])) + c[8] * cos(2 * (p - c[9])) + c[7]) * a3 +
+                   (c[16] * cos(5 * (p - c[17])) + c[14] * cos(3 * (p - c[15])) + c[12] * cos(p - c[13])) * a4 +
+                   (c[23] * cos(6 * (p - c[24])) + c[21] * cos(4 * (p - c[22])) + c[19] * cos(2 * (p - c[20])) +
+                    c[18]) * a5;
+  dk = 2 * M_PI / lambda * t;
+  const double u = 1 / 2.0 * (2.0 * c[1] * sin(2.0 * (p - c[2]))) * a +
+                   1 / 3.0 * (3.0 * c[5] * sin(3.0 * (p - c[6])) + 1.0 * c[3] * sin(1.0 * (p - c[4]))) * a2 +
+                   1 / 4.0 * (4.0 * c[10] * sin(4.0 * (p - c[11])) + 2.0 * c[8] * sin(2.0 * (p - c[9]))) * a3 +
+                   1 / 5.0 * (5.0 * c[16] * sin(5.0 * (p - c[17])) + 3.0 * c[14] * sin(3.0 * (p - c[15])) +
+                              1.0 * c[12] * sin(1.0 * (p - c[13]))) * a4 +
+                   (1 / 6.0) * (6.0 * c[23] * sin(6.0 * (p - c[24])) + 4.0 * c[21] * sin(4.0 * (p - c[22])) +
+                                2.0 * c[19] * sin(2.0 * (p - c[20]))) * a5;
+  dp = -2 * M_PI / lambda * u;
+}
+
+// ContrastTransferFunction.kernel (ContrastTransferFunction.swift:84-111) in float64: e^{-i chi} times the spatial
+// envelope exp(-sgn(beta) (beta/2)^2 ((dchi/dalpha)^2 + (dchi/dphi)^2)) (beta != 0, mrad / 1e3), the temporal
+// envelope exp(-((pi/2/lambda) Delta alpha^2)^2) (Delta != 0) and the aperture; times `scale` into out (complex64)
+// and, when re is given, unscaled into re / im (float64).
+__global__ void ctf_kernel_kernel(int nx, int ny, double ex, double ey, double lambda, CtfParams c, float scale,
+                                  float2* out, double* re, double* im) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= (size_t)nx * ny) return;
+  const int ix = int(i / ny), iy = int(i % ny);
+  const double sx = ex / nx, sy = ey / ny;
+  const double kx = fftfreq_d(ix, nx, sx), ky = fftfreq_d(iy, ny, sy);
+  const double alpha = sqrt(__dadd_rn(__dmul_rn(kx, kx), __dmul_rn(ky, ky))) * lambda;
+  const double phi = atan2(ky, kx);
+  const double x = chi_d(c.ab, alpha, phi, lambda);
+  double sn, cs;
+  sincos(-x, &sn, &cs);
+  double factor = 1.0;
+  if (c.angular_spread != 0) {
+    const double b = c.angular_spread / 1e3;
+    double dk, dp;
+    chi_derivatives_d(c.ab, alpha, phi, lambda, dk, dp);
+    const double sgn = b > 0 ? 1.0 : (b < 0 ? -1.0 : 0.0);
+    factor *= exp(-sgn * (b / 2) * (b / 2) * (dk * dk + dp * dp));
+  }
+  if (c.focal_spread != 0) {
+    const double t = 0.5 * M_PI / lambda * c.focal_spread * alpha * alpha;
+    factor *= exp(-(t * t));
+  }
+  factor *= aperture_d(ix, iy, nx, ny, sx, sy, alpha, phi, lambda, c.cutoff_mrad, c.soft);
+  if (out) out[i] = make_float2((float)(cs * factor * scale), (float)(sn * factor * scale));
+  if (re) {
+    re[i] = cs * factor;
+    im[i] = sn * factor;
+  }
+}
+void launch_ctf_kernel(int nx, int ny, double ex, double ey, double lambda, const CtfParams& c, float scale,
+                       float2* out, double* re, double* im, hipStream_t s) {
+  hipLaunchKernelGGL(ctf_kernel_kernel, grid_for((size_t)nx * ny), dim3(kThreads), 0, s, nx, ny, ex, ey, lambda, c,
+                     scale, out, re, im);
   SK_HIP(hipGetLastError());
 }
 

@@ -179,6 +179,90 @@ class GridScan:
         return np.stack(np.meshgrid(xs, ys, indexing="ij"), axis=-1).reshape(-1, 2)
 
 
+def _check(rc: int, what: str):
+    from .multislice import _raise
+    _raise(rc, what)
+
+
+def _grid_desc(g: Grid):
+    from .multislice import _sk_grid
+    return _sk_grid(g)
+
+
+class ContrastTransferFunction:
+    """The objective lens of HRTEM imaging (abTEM's CTF; ContrastTransferFunction.swift:36-135): the aberration phase
+    e^{-i chi}, the spatial (angular spread beta, mrad) and temporal (focal spread Delta, Angstrom) envelopes of the
+    quasi-coherent approximation, and the objective aperture (semiangle mrad; inf for none)."""
+
+    def __init__(self, aberrations: Optional[Aberrations] = None, semiangle_cutoff: float = math.inf,
+                 soft_aperture: bool = True, focal_spread: float = 0.0, angular_spread: float = 0.0):
+        self.aberrations = aberrations or Aberrations()
+        if not all(math.isfinite(v) for v in self.aberrations.vector()):
+            raise SimulatoError("invalidAberration", "every coefficient must be finite")
+        if not (semiangle_cutoff > 0):
+            raise SimulatoError("invalidAperture", f"invalid aperture: {semiangle_cutoff}")
+        if not (math.isfinite(focal_spread) and focal_spread >= 0):
+            raise SimulatoError("invalidAberration", f"focal spread must be finite and >= 0 A, got {focal_spread}")
+        if not (math.isfinite(angular_spread) and angular_spread >= 0):
+            raise SimulatoError("invalidAberration", f"angular spread must be finite and >= 0 mrad, got {angular_spread}")
+        self.semiangle_cutoff, self.soft_aperture = float(semiangle_cutoff), bool(soft_aperture)
+        self.focal_spread, self.angular_spread = float(focal_spread), float(angular_spread)
+
+    def descriptor(self) -> N.SkCtf:
+        d = N.SkCtf()
+        for k, v in enumerate(self.aberrations.vector()):
+            d.aberrations[k] = v
+        d.semiangle_mrad, d.soft = self.semiangle_cutoff, 1 if self.soft_aperture else 0
+        d.focal_spread, d.angular_spread = self.focal_spread, self.angular_spread
+        return d
+
+    def kernel(self, grid: Grid, energy: float) -> np.ndarray:
+        """H on the grid in FFT order, evaluated in float64 on the device (ContrastTransferFunction.swift:84-111)."""
+        re = np.empty((grid.nx, grid.ny))
+        im = np.empty_like(re)
+        _check(N.lib().sk_hip_ctf_kernel(C.byref(_grid_desc(grid)), energy, C.byref(self.descriptor()), N.dptr(re),
+                                         N.dptr(im)), "ctf kernel")
+        return re + 1j * im
+
+    @staticmethod
+    def scherzer_defocus(Cs: float, energy: float, factor: float = 1.5) -> float:
+        """sign(Cs) sqrt(factor |Cs| lambda) [Angstrom] (ContrastTransferFunction.swift:113-120)."""
+        if not (math.isfinite(factor) and factor > 0):
+            raise SimulatoError("invalidAberration", "the Scherzer factor must be finite and positive")
+        lam = ElectronBeam(energy).wavelength
+        s = 1.0 if Cs > 0 else (-1.0 if Cs < 0 else 0.0)
+        return s * math.sqrt(factor * abs(Cs) * lam)
+
+    @staticmethod
+    def point_resolution(Cs: float, energy: float, divisor: float = 6.0) -> float:
+        """(lambda^3 |Cs| / divisor)^(1/4) [Angstrom] (ContrastTransferFunction.swift:122-127)."""
+        if not (math.isfinite(divisor) and divisor > 0):
+            raise SimulatoError("invalidAberration", "the divisor must be finite and positive")
+        lam = ElectronBeam(energy).wavelength
+        return math.pow(math.pow(lam, 3) * abs(Cs) / divisor, 1.0 / 4)
+
+    @staticmethod
+    def nyquist_sampling(semiangle_cutoff: float, energy: float, oversampling: float = 4.0) -> float:
+        """1 / (oversampling alpha_c / lambda 1e-3) [Angstrom] (ContrastTransferFunction.swift:129-135)."""
+        if not (math.isfinite(semiangle_cutoff) and semiangle_cutoff > 0):
+            raise SimulatoError("invalidAperture", f"invalid aperture: {semiangle_cutoff}")
+        if not (math.isfinite(oversampling) and oversampling > 0):
+            raise SimulatoError("invalidAperture", f"invalid oversampling: {oversampling}")
+        lam = ElectronBeam(energy).wavelength
+        return 1 / (oversampling * semiangle_cutoff / lam * 1e-3)
+
+
+def apply_ctf(waves: Waves, ctf: ContrastTransferFunction) -> Waves:
+    """The waves with the CTF applied, IFFT(FFT(psi) H) on the device (Waves.applying,
+    ContrastTransferFunction.swift:140-153); the intensity of the result is the HRTEM image."""
+    re = np.ascontiguousarray(waves.field.real)
+    im = np.ascontiguousarray(waves.field.imag)
+    ore, oim = np.empty_like(re), np.empty_like(im)
+    _check(N.lib().sk_hip_apply_ctf(C.byref(_grid_desc(waves.grid)), waves.beam.energy, C.byref(ctf.descriptor()), 1,
+                                    N.dptr(re), N.dptr(im), N.dptr(ore), N.dptr(oim)), "apply_ctf")
+    return Waves(ore + 1j * oim, waves.grid, waves.beam, waves.normalization, waves.tilt)
+
+
 class LineScan:
     """Probe positions along a straight line (abTEM's LineScan; LineScan.swift:26-198): `gpts` points from `start` to
     `end` [Angstrom], the last at `end` with `endpoint`. A given `sampling` wins over `gpts`: the count becomes
@@ -272,4 +356,5 @@ class LineScan:
         return np.stack([np.broadcast_to(xs, (self.gpts,)), np.broadcast_to(ys, (self.gpts,))], axis=-1)
 
 
-__all__ = ["Aberrations", "Aperture", "Probe", "GridScan", "LineScan", "ABERRATION_NAMES"]
+__all__ = ["Aberrations", "Aperture", "Probe", "GridScan", "LineScan", "ContrastTransferFunction", "apply_ctf",
+           "ABERRATION_NAMES"]

@@ -93,6 +93,12 @@ def main():
     meta = {k: v for k, v in js.items() if k != "arrays"}
     dump("waves", js, blob, names, meta)
 
+    # CTF kernels and HRTEM images of a SrTiO3 exit wave (tools/pins/ctf/pins.py)
+    js, blob = load("slice-3-pins.json")
+    names = sorted(js["arrays"]["index"].keys())
+    meta = {k: v for k, v in js.items() if k != "arrays"}
+    dump("slice3", js, blob, names, meta)
+
 
 if __name__ == "__main__":
     main()
