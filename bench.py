@@ -56,6 +56,11 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=128, help="positions of the float64 CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(8, cores), the reference's default")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="haadf", choices=["haadf", "frozen_phonons"],
+                    help="haadf (the headline scan) or frozen_phonons (16 configurations of uneven slice counts, "
+                         "pixelated 128 x 128 float32 crop + centre of mass)")
+    ap.add_argument("--fp-scan", type=int, default=128, help="frozen_phonons: scan points per axis")
+    ap.add_argument("--pipeline", type=int, default=-1, help="frozen_phonons: 1 lane pipeline, 0 one potential at a time")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="first run one device-to-device copy of BYTES (the reference kernel that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profile)")
@@ -80,6 +85,10 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+
+    if args.workload == "frozen_phonons":
+        print(json.dumps(frozen_phonon_bench(args, sk, torch)), flush=True)
+        return
 
     w = HaadfScan(batch=args.batch)
     st = w.structure
@@ -159,6 +168,51 @@ def main():
         result["cpu_baseline"], result["rel_err_vs_fp64"] = cpu_baseline(args, w, st, pot, positions, haadf, S)
 
     print(json.dumps(result), flush=True)
+
+
+def frozen_phonon_bench(args, sk, torch):
+    """4D-STEM with frozen phonons: 16 configurations of a seeded 40 x 40 A random cell (2000 atoms, Z from
+    {8, 14, 22, 38, 79}, sigma 0.08 A) cut to thicknesses uniform in 40-120 A, 512 x 512, 1 A slices, so the slice
+    counts differ; a fp_scan x fp_scan scan over the cell into a 128 x 128 float32 crop of the pattern and the
+    centre of mass, averaged over the configurations. One step = the whole scan through every configuration."""
+    import numpy as np
+    from simulatokit_amd.workloads import uneven_frozen_phonon_stack
+
+    structures, T = uneven_frozen_phonon_stack()
+    pots = [sk.Potential(st, 512, 512, slice_thickness=1.0) for st in structures]
+    probe = sk.Probe(sk.ElectronBeam(200e3), pots[0].grid, sk.Aperture(20.0))
+    t0 = time.perf_counter()
+    prepared = [p.prepare(probe.beam) for p in pots]
+    prepare_ms = (time.perf_counter() - t0) * 1e3
+    n = args.fp_scan
+    scan = sk.GridScan((0.0, 0.0), (40.0, 40.0), n, n)
+    dets = [sk.PixelatedDetector(crop=(128, 128), dtype="float32"), sk.CenterOfMassDetector("cutoff", "mrad")]
+    npos = n * n
+    out_pix = torch.empty((npos, 128, 128), dtype=torch.float32, device="cuda:0")
+    out_com = torch.empty((npos, 2), dtype=torch.float64, device="cuda:0")
+    opt = sk.ExecutionOptions(batch=args.batch, pipeline=args.pipeline)
+
+    def step():
+        sk.STEM.scan(probe, prepared, scan, dets, opt, device_outputs=[out_pix.data_ptr(), out_com.data_ptr()])
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    slices = [p.slice_count for p in prepared]
+    work = npos * sum(slices)
+    return {"metric": "probe-positions*slices/sec (frozen-phonon 4D-STEM, uneven slice counts)",
+            "value": work * args.steps / elapsed, "unit": "probe-positions*slices/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
+            "higher_is_better": True, "dtype": "c64", "data": "synthetic",
+            "config": {"workload": f"frozen_phonons_16cfg_512sq_{n}x{n}", "configurations": len(prepared),
+                       "slices": slices, "positions": npos, "batch": args.batch, "pipeline": args.pipeline,
+                       "detectors": "pixelated 128x128 float32 + centre of mass"},
+            "prepare_ms": prepare_ms, "scan_pos_slices": work}
 
 
 def copy_bandwidth(torch, nbytes: int = 1 << 30, reps: int = 5) -> float:

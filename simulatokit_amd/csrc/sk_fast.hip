@@ -482,18 +482,18 @@ constexpr int kPipePB = 8;  // positions per block in pipelined launches; lane_w
 //   FINAL:      FFT_x, * P, detectors                    (the item's last slice; the lane then idles)
 //   FINAL_INIT: FFT_x, * P, detectors, then the probe of the lane's next item generated into the same storage
 //   INIT:       the probe of the lane's first item
-template <int N>
-__global__ void __launch_bounds__(kColTile * (N / 8)) col_pipe_kernel(ColArgs a, const float2* __restrict__ tw) {
-  constexpr int T = N / 8, NT = kColTile * T;
-  __shared__ float2 lds[N * kColTile];  // [x][column]
-  const int tid = threadIdx.x, cl = tid % kColTile, i = tid / kColTile;
-  const int c = blockIdx.x * kColTile + cl;
+template <int N, int TC>
+__global__ void __launch_bounds__(TC * (N / 8)) col_pipe_kernel(ColArgs a, const float2* __restrict__ tw) {
+  constexpr int T = N / 8, NT = TC * T;
+  __shared__ float2 lds[N * TC];  // [x][column]
+  const int tid = threadIdx.x, cl = tid % TC, i = tid / TC;
+  const int c = blockIdx.x * TC + cl;
   const int b0 = blockIdx.y * kPipePB;
   const int lane = b0 / a.pipe.lane_waves, local0 = b0 - lane * a.pipe.lane_waves;
   const LaneDesc d = a.pipe.descs[(long long)(*a.pipe.step) * a.pipe.lanes + lane];
   if (d.mode == LANE_IDLE) return;  // uniform over the block
   const int iy = a.keep[c];
-  auto idx = [cl](int j) { return j * kColTile + cl; };
+  auto idx = [cl](int j) { return j * TC + cl; };
   const long long gw = (long long)a.nx * a.nc;
   LaneTwiddles<N> w;
   w.load(tw, i);
@@ -634,8 +634,8 @@ int fast_col_tile(int nx, int requested) {
 }
 
 int fast_col_blocks_per_wave(int nx, int nc, int pipeline) {
-  if (pipeline || nx != 512) return nc / kColTile;
-  return nc / 8;  // the 512-point detecting pass's tile
+  (void)pipeline;  // at 512 both the detecting pass and the lane pipeline's column pass use 8-column tiles
+  return nx == 512 ? nc / 8 : nc / kColTile;
 }
 
 void launch_row_fast(const RowArgs& a, int batch, hipStream_t s) {
@@ -673,9 +673,12 @@ void launch_row_pipe(const RowArgs& a, int batch, hipStream_t s) {
 
 void launch_col_pipe(const ColArgs& a, int batch, hipStream_t s) {
   const float2* tw = line_plan(a.nx, s).dev.roots;
-  dim3 grid(a.nc / kColTile, batch / kPipePB);
-  if (a.nx == 512) hipLaunchKernelGGL((col_pipe_kernel<512>), grid, dim3(kColTile * 64), 0, s, a, tw);
-  else hipLaunchKernelGGL((col_pipe_kernel<256>), grid, dim3(kColTile * 32), 0, s, a, tw);
+  // 512: 8-column blocks of 512 threads, whose 256-VGPR budget holds the fused detectors and probe generation
+  if (a.nx == 512)
+    hipLaunchKernelGGL((col_pipe_kernel<512, 8>), dim3(a.nc / 8, batch / kPipePB), dim3(8 * 64), 0, s, a, tw);
+  else
+    hipLaunchKernelGGL((col_pipe_kernel<256, kColTile>), dim3(a.nc / kColTile, batch / kPipePB), dim3(kColTile * 32),
+                       0, s, a, tw);
   SK_HIP(hipGetLastError());
 }
 
