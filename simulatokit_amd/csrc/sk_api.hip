@@ -1577,7 +1577,7 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
     int pipe = opt ? opt->pipeline : 0;
     // the library's choice: lanes when one potential's positions cannot fill a launch (measured at 512^2 with 16
     // configurations x 16,384 positions, one potential at a time runs 1.26 x faster than the lanes)
-    if (pipe < 0) pipe = (npots > 1 && npos < B) ? 1 : 0;
+    if (pipe < 0) pipe = (npots > 1 && npos < default_batch(p0, 1 << 30, opt)) ? 1 : 0;
     int W = 0, L = 0;
     if (pipe) {
       const int q = pipe_positions_per_block();
