@@ -49,7 +49,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=1024)
+    ap.add_argument("--batch", type=int, default=0,
+                    help="positions per launch; 0: the library's choice (sized to stay in the Infinity Cache)")
+    ap.add_argument("--compare-batch", type=int, default=1024,
+                    help="also time the scan at this many positions per launch (0: skip)")
     ap.add_argument("--graph", action="store_true", help="replay each batch's slice loop from a hipGraph")
     ap.add_argument("--generic", action="store_true", help="generic LDS passes instead of the fast passes")
     ap.add_argument("--col-tile", type=int, default=0, help="columns per 512-point column-pass block (8, 16)")
@@ -108,17 +111,27 @@ def main():
     out = torch.zeros(npos, dtype=torch.float64, device="cuda:0")
     positions = scan.positions
 
-    def step():
-        sk.STEM.scan(probe, [prepared], positions, [det], opt, device_outputs=[out.data_ptr()])
+    def step(o=opt):
+        sk.STEM.scan(probe, [prepared], positions, [det], o, device_outputs=[out.data_ptr()])
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    def timed(o):
+        for _ in range(args.warmup):
+            step(o)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step(o)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    compare = None
+    if args.compare_batch > 0 and args.compare_batch != args.batch:
+        o = sk.ExecutionOptions(batch=args.compare_batch, use_graph=args.graph, fast_path=not args.generic,
+                                col_tile=args.col_tile)
+        el = timed(o)
+        compare = {"batch": min(args.compare_batch, npos), "value": npos * S * args.steps / el,
+                   "ms_per_step": el / args.steps * 1e3}
+    elapsed = timed(opt)
     ms_per_step = elapsed / args.steps * 1e3
     value = npos * S * args.steps / elapsed
 
@@ -128,7 +141,7 @@ def main():
     sk.set_timing(False)
     tm = sk.last_timing()
     N = w.n * w.n
-    B = min(w.batch, npos)
+    B = int(tm["batch"])  # positions per launch of the timed scans
     alg_bytes = 16.0 * N * B  # per pass launch: the complex64 wave of every position read and written once
     col_ms, row_ms = tm["col_ms"], tm["row_ms"]  # the average launch of each pass
     dom = "column" if col_ms >= row_ms else "row"
@@ -163,6 +176,8 @@ def main():
         "prepare_ms": prepare_ms,
         "scan_pos_slices": npos * S,
     }
+    if compare is not None:
+        result["compare_batch"] = compare
 
     if not args.no_cpu_baseline:
         result["cpu_baseline"], result["rel_err_vs_fp64"] = cpu_baseline(args, w, st, pot, positions, haadf, S)
@@ -210,7 +225,7 @@ def frozen_phonon_bench(args, sk, torch):
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
             "higher_is_better": True, "dtype": "c64", "data": "synthetic",
             "config": {"workload": f"frozen_phonons_16cfg_512sq_{n}x{n}", "configurations": len(prepared),
-                       "slices": slices, "positions": npos, "batch": args.batch, "pipeline": args.pipeline,
+                       "slices": slices, "positions": npos, "batch": args.batch or "library", "pipeline": args.pipeline,
                        "detectors": "pixelated 128x128 float32 + centre of mass"},
             "prepare_ms": prepare_ms, "scan_pos_slices": work}
 

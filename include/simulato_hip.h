@@ -195,12 +195,13 @@ int sk_hip_detector_shape(const sk_grid* grid, double energy, const sk_detector*
 
 /* Device time of the dominant kernels of the last sk_hip_scan on the calling thread, measured with HIP events on
  * the library's stream: mean milliseconds per launch of the row pass and of the column pass, their launch counts,
- * and the bytes of the stored compact waves each launch reads and writes once. */
+ * the bytes of the stored compact waves each launch reads and writes once, and the positions per launch. */
 typedef struct sk_timing {
   double row_ms, col_ms;
   long long row_launches, col_launches;
   double row_bytes, col_bytes;
   double total_ms;
+  long long batch;
 } sk_timing;
 int sk_hip_last_timing(sk_timing* t);
 void sk_hip_set_timing(int enabled);

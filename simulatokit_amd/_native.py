@@ -40,7 +40,7 @@ class SkOptions(C.Structure):
 class SkTiming(C.Structure):
     _fields_ = [("row_ms", C.c_double), ("col_ms", C.c_double), ("row_launches", C.c_longlong),
                 ("col_launches", C.c_longlong), ("row_bytes", C.c_double), ("col_bytes", C.c_double),
-                ("total_ms", C.c_double)]
+                ("total_ms", C.c_double), ("batch", C.c_longlong)]
 
 
 class SkCtf(C.Structure):

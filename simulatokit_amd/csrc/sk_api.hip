@@ -496,14 +496,24 @@ LineFFT slice_plan(int n, hipStream_t s) {
   return line_plan(n, s).dev;
 }
 
+// Positions per launch. Every pass streams the batch's compact waves (nx x nc complex64 each) once in and once out,
+// so a batch whose waves stay in the device's last-level cache (kCachedWaveBytes) runs from that cache rather than
+// from HBM: at 512 x 512, 176 positions run 1.17 x faster per position than 1024, and 184 fall back to HBM rates. The default takes the largest cache-resident batch when it holds at least
+// kMinCachedBatch positions (enough to fill the GPU), else a 4 GiB budget of at most 1024 positions, and then
+// spreads `count` evenly over the batches, so no small remainder batch runs at a fraction of the GPU.
+constexpr int kMinCachedBatch = 64;
+
 int default_batch(const sk_potential& p, int count, const sk_options* opt) {
   int b = opt && opt->batch > 0 ? opt->batch : 0;
-  if (b <= 0) {
-    const size_t per = (size_t)p.grid.nx * p.nc * sizeof(float2);
-    const size_t budget = size_t(4) << 30;
-    b = (int)std::max<size_t>(1, std::min<size_t>(1024, budget / per));
-  }
-  return std::max(1, std::min(b, count));
+  if (b > 0) return std::max(1, std::min(b, count));
+  const size_t cached = kCachedWaveBytes / ((size_t)p.grid.nx * p.nc * sizeof(float2));
+  if (cached >= (size_t)kMinCachedBatch)
+    b = (int)std::min<size_t>(1024, cached);
+  else
+    b = (int)std::max<size_t>(1, std::min<size_t>(1024, (size_t(4) << 30) / ((size_t)p.grid.nx * p.nc * sizeof(float2))));
+  count = std::max(1, count);
+  const int batches = (count + b - 1) / b;
+  return std::max(1, (count + batches - 1) / batches);
 }
 
 RowArgs row_args(const sk_potential& p, float2* G) {
@@ -537,7 +547,7 @@ ColArgs col_args(const sk_potential& p, float2* G) {
   c.fx = slice_plan(p.grid.nx, ctx().stream);
   c.keep = p.keep_d.p;
   c.G = G;
-  c.col_tile = fast_col_tile(p.grid.nx, ctx().col_tile);
+  c.col_tile = ctx().col_tile;  // 0: chosen per launch from the batch's size
   return c;
 }
 
@@ -762,6 +772,9 @@ void run_pipeline(const sk_potential* const* pots, int npots, int npos, int W, i
 }
 
 }  // namespace
+
+int default_scan_batch(const sk_potential& p, int count) { return default_batch(p, count, nullptr); }
+
 }  // namespace sk
 
 using namespace sk;
@@ -1736,6 +1749,7 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
     if (timing.col_launches) timing.col_ms /= double(timing.col_launches);
     timing.row_bytes = 2.0 * sizeof(float2) * double(B) * nx * p0.nc;
     timing.col_bytes = timing.row_bytes;
+    timing.batch = B;
     c.last = timing;
   });
 }

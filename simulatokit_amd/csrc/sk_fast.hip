@@ -383,72 +383,74 @@ __global__ void __launch_bounds__(kColTile * (N / 8)) col_fast_kernel(ColArgs a,
   }
 }
 
-// Work-group order for tiles narrower than a 128-byte line: the hardware deals work-groups round-robin over the 8 XCDs
-// (each with its own L2), so consecutive block ids land on different XCDs. Renumbering XCD-major makes the blocks
-// that one XCD runs back to back take consecutive tiles, so the two 8-column tiles of a line share that XCD's L2.
-// A bijection of the grid whenever its size is a multiple of 8 (otherwise the order is left alone).
-__device__ __forceinline__ void xcd_pair_order(int& bx, int& by) {
-  const int gx = gridDim.x, total = gx * gridDim.y;
-  if (total % 8 != 0) return;
-  const int L = bx + gx * by, per = total / 8;
-  const int j = (L % 8) * per + L / 8;
-  bx = j % gx;
-  by = j / gx;
-}
-
 // Column pass of 512-point columns, one column per wavefront. The block's TC columns move between HBM and a
 // swizzled LDS tile in coalesced row segments of TC values (block phases); each wavefront then transforms its own
 // column with wavefront-local exchanges inside the tile (no block barrier), and the next position's loads are in
 // flight meanwhile. Three block barriers per position. With TC = 8 two blocks share a CU, so one block's barriers
 // overlap the other's work.
-template <int FLAGS, int PB, int TC>
-__global__ void __launch_bounds__(TC * 64) col512_kernel(ColArgs a, const float2* __restrict__ tw, int batch) {
+//
+// col512_run: work units [u0, u1), where unit u is position `b` of the tile of columns [TC tx, TC tx + TC) as
+// where(u, tx, b) gives them; consecutive units of one tile share its P, which is (re)loaded when the tile changes.
+// The twiddles are in LDS already, and every wavefront is past its last LDS access on entry and again on exit. The
+// unit's tile and position are recomputed where they are used rather than kept live across the transforms. ONE_TILE:
+// every unit is on one tile (its P is loaded once, before the loop).
+template <int FLAGS, int TC, bool ONE_TILE, class Where>
+__device__ __forceinline__ void col512_run(const ColArgs& a, int u0, int u1, Where where, float2* tile, float2* ptile,
+                                           const LdsTwiddles<512>& w, double* red) {
   constexpr int N = 512, T = 64, NT = TC * 64;
   constexpr bool HAS_P = (FLAGS & COL_MUL_P) != 0;
-  __shared__ float2 tile[N * TC];
-  __shared__ float2 ptile[HAS_P ? N * TC : 1];  // the block's columns of P (swizzled as the tile)
-  __shared__ float2 twt[LdsTwiddles<N>::kSize];
-  __shared__ double red[(NT / 64) * kMaxSlots];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int cb = tid % TC, xi = tid / TC;  // block phases: column cb of the tile, rows xi + 64 r
-  int bx = blockIdx.x, by = blockIdx.y;
-  if constexpr (TC == 8) xcd_pair_order(bx, by);
-  const int c0 = bx * TC;                  // the host launches nc / TC blocks along x: every column is in range
-  const int iyb = a.keep[c0 + cb];
-  const int iyw = a.keep[c0 + wave];       // the wavefront's column
-  const long long b0 = (long long)by * PB;
-  const long long blim = b0 + PB < batch ? b0 + PB : batch;
   const long long gw = (long long)a.nx * a.nc;
   auto tile_slot = [](int x, int c) { return sk::tile_slot<TC>(x, c); };
   auto widx = [wave, tile_slot](int j) { return tile_slot(j, wave); };
-  LdsTwiddles<N>::build(twt, tw, tid, NT);
-  const LdsTwiddles<N> w{twt, twt + LdsTwiddles<N>::R2 * LdsTwiddles<N>::L2, lane};
-  if constexpr (HAS_P) {
-#pragma unroll
-    for (int r = 0; r < 8; ++r) ptile[tile_slot(xi + T * r, cb)] = a.P[(long long)(xi + T * r) * a.nc + c0 + cb];
-  }
-  __syncthreads();
-  auto gload = [&](long long b, float2 (&g)[8]) {
+  // the block-phase load of unit u: column cb of its tile, rows xi + 64 r
+  auto gload = [&](int u, float2 (&g)[8]) {
+    int tx, b;
+    where(u, tx, b);
+    const int c = tx * TC + cb, iy = a.keep[c];
     if constexpr ((FLAGS & COL_IN_G) != 0) {
-      const float2* src = a.G + b * gw + c0 + cb;
+      const float2* src = a.G + b * gw + c;
 #pragma unroll
-      for (int r = 0; r < 8; ++r) g[r] = iyb >= 0 ? src[(long long)(xi + T * r) * a.nc] : make_float2(0.f, 0.f);
+      for (int r = 0; r < 8; ++r) g[r] = iy >= 0 ? src[(long long)(xi + T * r) * a.nc] : make_float2(0.f, 0.f);
     } else {
 #pragma unroll
       for (int r = 0; r < 8; ++r) {
         const int x = xi + T * r;
-        g[r] = iyb >= 0 ? cmul(cmul(a.spec[(long long)x * a.ny + iyb], a.rx[b * a.nx + x]), a.ry[b * a.ny + iyb])
-                        : make_float2(0.f, 0.f);
+        g[r] = iy >= 0 ? cmul(cmul(a.spec[(long long)x * a.ny + iy], a.rx[(long long)b * a.nx + x]),
+                              a.ry[(long long)b * a.ny + iy])
+                       : make_float2(0.f, 0.f);
       }
     }
   };
+  auto pload = [&](int tx) {
+    const float2* P = a.P + tx * TC + cb;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) ptile[tile_slot(xi + T * r, cb)] = P[(long long)(xi + T * r) * a.nc];
+  };
   float2 g[8];
-  if (b0 < blim) gload(b0, g);
-  for (long long b = b0; b < blim; ++b) {
+  int tcur = -1;
+  if (u0 < u1) {
+    gload(u0, g);
+    if constexpr (HAS_P && ONE_TILE) {
+      int b;
+      where(u0, tcur, b);
+      pload(tcur);
+    }
+  }
+  for (int u = u0; u < u1; ++u) {
+    if constexpr (HAS_P && !ONE_TILE) {
+      int tx, b;
+      where(u, tx, b);
+      if (tx != tcur) {  // uniform over the block; the previous unit's LDS reads ended before its last barrier
+        tcur = tx;
+        pload(tx);
+      }
+    }
 #pragma unroll
     for (int r = 0; r < 8; ++r) tile[tile_slot(xi + T * r, cb)] = g[r];
     __syncthreads();
-    if (b + 1 < blim) gload(b + 1, g);  // in flight during the transforms
+    if (u + 1 < u1) gload(u + 1, g);  // in flight during the transforms
     float2 v[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) v[r] = tile[tile_slot(lane + T * r, wave)];
@@ -458,21 +460,77 @@ __global__ void __launch_bounds__(TC * 64) col512_kernel(ColArgs a, const float2
       for (int r = 0; r < 8; ++r) v[r] = cmul(v[r], ptile[tile_slot(lane + T * r, wave)]);
     }
     if constexpr ((FLAGS & COL_DETECT) != 0) {
-      detect_block<N, NT>(v, a.det, first_position(a.pos0, a.pos_dev) + b, b, iyw, lane, tid,
-                          reinterpret_cast<float2*>(red), bx);
+      int tx, b;
+      where(u, tx, b);
+      detect_block<N, NT>(v, a.det, first_position(a.pos0, a.pos_dev) + b, b, a.keep[tx * TC + wave], lane, tid,
+                          reinterpret_cast<float2*>(red), tx);
       continue;  // detect_block ends with a barrier: the tile is free again
     }
     if constexpr ((FLAGS & COL_INV) != 0) fft_reg<N, +1, true>(v, tile, lane, w, widx);
 #pragma unroll
     for (int r = 0; r < 8; ++r) tile[tile_slot(lane + T * r, wave)] = v[r];
     __syncthreads();
-    float2* dst = a.G + b * gw + c0 + cb;
-    if (iyb >= 0) {  // padding columns stay as the row pass leaves them: never read, so never written
+    {
+      int tx, b;
+      where(u, tx, b);
+      const int c = tx * TC + cb;
+      float2* dst = a.G + b * gw + c;
+      if (a.keep[c] >= 0) {  // padding columns stay as the row pass leaves them: never read, so never written
 #pragma unroll
-      for (int r = 0; r < 8; ++r) dst[(long long)(xi + T * r) * a.nc] = tile[tile_slot(xi + T * r, cb)];
+        for (int r = 0; r < 8; ++r) dst[(long long)(xi + T * r) * a.nc] = tile[tile_slot(xi + T * r, cb)];
+      }
     }
     __syncthreads();
   }
+}
+
+// 2-D grid (16-column tiles): block (bx, by) runs tile bx over positions [PB by, PB by + PB).
+template <int FLAGS, int PB, int TC>
+__global__ void __launch_bounds__(TC * 64) col512_kernel(ColArgs a, const float2* __restrict__ tw, int batch) {
+  constexpr int N = 512, NT = TC * 64;
+  constexpr bool HAS_P = (FLAGS & COL_MUL_P) != 0;
+  __shared__ float2 tile[N * TC];
+  __shared__ float2 ptile[HAS_P ? N * TC : 1];  // the block's columns of P (swizzled as the tile)
+  __shared__ float2 twt[LdsTwiddles<N>::kSize];
+  __shared__ double red[(NT / 64) * kMaxSlots];
+  const int bx = blockIdx.x, by = blockIdx.y;  // the host launches nc / TC blocks along x: every column is in range
+  const int b0 = by * PB;
+  const int b1 = b0 + PB < batch ? b0 + PB : batch;
+  LdsTwiddles<N>::build(twt, tw, threadIdx.x, NT);
+  const LdsTwiddles<N> w{twt, twt + LdsTwiddles<N>::R2 * LdsTwiddles<N>::L2, (int)(threadIdx.x & 63)};
+  col512_run<FLAGS, TC, true>(a, b0, b1, [bx](int u, int& tx, int& b) { tx = bx; b = u; }, tile, ptile, w, red);
+}
+
+// Balanced 1-D grid of 8-column tiles, sized by the host to one round of resident blocks. The work is the list of
+// (tile pair m, position b) units in the order m * batch + b, cut into `ranges` contiguous spans of equal length (to
+// one unit); the blocks 2 r and 2 r + 1 run span r on tiles 2 m and 2 m + 1, whose 64-byte halves make up the same
+// 128-byte lines. Renumbered XCD-major (work-groups are dealt round-robin over the 8 XCDs), the two blocks of a span
+// run side by side on one XCD and share its L2. A span that crosses a pair boundary reloads P for its next tile.
+template <int FLAGS>
+__global__ void __launch_bounds__(8 * 64, (FLAGS & COL_DETECT) != 0 ? 1 : 4)
+    col512_span_kernel(ColArgs a, const float2* __restrict__ tw, int batch, int ranges) {
+  constexpr int N = 512, TC = 8, NT = TC * 64;
+  constexpr bool HAS_P = (FLAGS & COL_MUL_P) != 0;
+  __shared__ float2 tile[N * TC];
+  __shared__ float2 ptile[HAS_P ? N * TC : 1];
+  __shared__ float2 twt[LdsTwiddles<N>::kSize];
+  __shared__ double red[(NT / 64) * kMaxSlots];
+  const int G = gridDim.x;  // 2 * ranges
+  int j = blockIdx.x;
+  if (G % 8 == 0) j = (j % 8) * (G / 8) + j / 8;
+  const int h = j & 1, r = j >> 1;
+  const long long U = (long long)(a.nc / (2 * TC)) * batch;  // below 2^31: the host checks
+  const int u0 = (int)(U * r / ranges), u1 = (int)(U * (r + 1) / ranges);
+  LdsTwiddles<N>::build(twt, tw, threadIdx.x, NT);
+  const LdsTwiddles<N> w{twt, twt + LdsTwiddles<N>::R2 * LdsTwiddles<N>::L2, (int)(threadIdx.x & 63)};
+  col512_run<FLAGS, TC, false>(
+      a, u0, u1,
+      [batch, h](int u, int& tx, int& b) {
+        const int m = u / batch;
+        tx = 2 * m + h;
+        b = u - m * batch;
+      },
+      tile, ptile, w, red);
 }
 
 constexpr int kPipePB = 8;  // positions per block in pipelined launches; lane_waves is a multiple of it
@@ -581,7 +639,23 @@ void row_launch(const RowArgs& a, int batch, hipStream_t s, const float2* tw) {
   SK_HIP(hipGetLastError());
 }
 
-constexpr int kCol512Tile = 16;  // default columns per block of the 512-point column pass (8 measured slower)
+// Spans of the balanced 8-column pass of mode F over `units` (tile pair, position) units: one round of resident
+// blocks, two per span, and a multiple of 8 spans where there are enough units, so that the XCD-major renumbering
+// keeps both blocks of every span on one XCD.
+template <int F>
+int span_count(long long units) {
+  static int resident = 0;  // blocks of this kernel the device holds at once
+  if (resident == 0) {
+    int dev = 0, cus = 0, occ = 0;
+    SK_HIP(hipGetDevice(&dev));
+    SK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    SK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, col512_span_kernel<F>, 8 * 64, 0));
+    resident = std::max(2, cus * std::max(1, occ));
+  }
+  long long r = std::min<long long>(units, resident / 2);
+  if (r >= 8) r -= r % 8;
+  return (int)std::max<long long>(1, r);
+}
 
 template <int N, int F>
 void col_go(const ColArgs& a, int batch, hipStream_t s, const float2* tw) {
@@ -589,13 +663,14 @@ void col_go(const ColArgs& a, int batch, hipStream_t s, const float2* tw) {
   if constexpr (N == 512) {
     // the detecting pass always runs 8-column tiles: its reductions need more than the 128 VGPRs of a 1024-thread
     // block, and 512-thread blocks give it 256 without spilling
-    if constexpr (F == COL_FINAL) {
-      hipLaunchKernelGGL((col512_kernel<F, kColPB, 8>), dim3(a.nc / 8, by), dim3(8 * 64), 0, s, a, tw, batch);
-    } else {
-      if (fast_col_tile(512, a.col_tile) == 8)
-        hipLaunchKernelGGL((col512_kernel<F, kColPB, 8>), dim3(a.nc / 8, by), dim3(8 * 64), 0, s, a, tw, batch);
-      else
-        hipLaunchKernelGGL((col512_kernel<F, kColPB, 16>), dim3(a.nc / 16, by), dim3(16 * 64), 0, s, a, tw, batch);
+    const int tc = F == COL_FINAL ? 8 : fast_col_tile(512, a.col_tile, double(batch) * a.nx * a.nc * sizeof(float2));
+    if (tc == 8) {
+      const long long units = (long long)(a.nc / 16) * batch;
+      if (units >= (1LL << 31)) fail(SK_ERR_INVALID, "column pass: too many positions in one launch");
+      const int ranges = span_count<F>(units);
+      hipLaunchKernelGGL((col512_span_kernel<F>), dim3(2 * ranges), dim3(8 * 64), 0, s, a, tw, batch, ranges);
+    } else if constexpr (F != COL_FINAL) {
+      hipLaunchKernelGGL((col512_kernel<F, kColPB, 16>), dim3(a.nc / 16, by), dim3(16 * 64), 0, s, a, tw, batch);
     }
   } else {
     hipLaunchKernelGGL((col_fast_kernel<N, F, kColPB>), dim3(a.nc / kColTile, by), dim3(kColTile * (N / 8)), 0, s,
@@ -628,9 +703,12 @@ bool fast_col_supported(int nx, int nc, int flags) {
   return flags == COL_MID || flags == COL_INIT || flags == COL_FINAL;
 }
 
-int fast_col_tile(int nx, int requested) {
+int fast_col_tile(int nx, int requested, double wave_bytes) {
   if (nx != 512) return kColTile;
-  return requested == 8 || requested == 16 ? requested : kCol512Tile;
+  if (requested == 8 || requested == 16) return requested;
+  // 8-column spans when the batch stays in the last-level cache (they fill every CU for any batch); 16-column
+  // blocks when it streams from HBM, where their full 128-byte row segments measured faster
+  return wave_bytes <= (double)kCachedWaveBytes ? 8 : 16;
 }
 
 int fast_col_blocks_per_wave(int nx, int nc, int pipeline) {

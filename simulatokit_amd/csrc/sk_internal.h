@@ -250,7 +250,12 @@ bool fast_row_supported(int nx, int ny, int flags);
 bool fast_col_supported(int nx, int nc, int flags);
 void launch_row_fast(const RowArgs& a, int batch, hipStream_t s);
 void launch_col_fast(const ColArgs& a, int batch, hipStream_t s);
-int fast_col_tile(int nx, int requested);  // columns per block the fast column pass uses
+// Bytes of compact waves (batch x nx x nc complex64) a batch may hold and still stay in the last-level cache between
+// passes: the 256 MiB Infinity Cache of an MI355X less room for t, P and the partial sums (measured at 512 x 512:
+// 176 positions, 242 MiB, run from the cache; 184 positions, 253 MiB, at HBM rates).
+constexpr size_t kCachedWaveBytes = size_t(240) << 20;
+// columns per block the fast column pass uses for a request of 0, 8 or 16 and a batch of `wave_bytes` compact waves
+int fast_col_tile(int nx, int requested, double wave_bytes);
 int fast_col_blocks_per_wave(int nx, int nc, int pipeline);  // blocks of the detecting column pass per wave
 // lane pipeline launches (sk_fast.hip): the row pass and the column pass of one step, the partial-sum reduction of
 // the lanes that finished an item, and the step counter's increment
@@ -260,6 +265,8 @@ void launch_col_pipe(const ColArgs& a, int batch, hipStream_t s);
 void launch_reduce_pipe(const DetPlan& d, const PipeArgs& p, int batch, double* acc, hipStream_t s);
 void launch_step_increment(int* step, hipStream_t s);
 int pipe_positions_per_block();
+// positions per launch the scan takes for `count` positions through p when the caller names no batch (sk_api.hip)
+int default_scan_batch(const sk_potential& p, int count);
 
 // elementwise kernels (sk_prepare.hip)
 void launch_phase_to_complex(const double* V, double sigma, float2* out, size_t n, hipStream_t s);

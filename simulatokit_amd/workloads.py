@@ -29,7 +29,8 @@ def srtio3(nx: int, ny: int, nz: int) -> AtomicStructure:
 @dataclass
 class HaadfScan:
     """The HAADF scan workload: SrTiO3 tiled (tx, ty, tz), n x n grid, slices of `slice_thickness`, 200 keV probe with
-    a soft `semiangle` aperture, a scan_n x scan_n GridScan over one unit cell, annular detector inner..outer."""
+    a soft `semiangle` aperture, a scan_n x scan_n GridScan over one unit cell, annular detector inner..outer.
+    `batch`: positions per launch (0: the library's choice)."""
 
     tiles: Sequence[int] = (8, 8, 20)
     n: int = 512
@@ -39,7 +40,7 @@ class HaadfScan:
     scan_n: int = 64
     inner: float = 50.0
     outer: float = 200.0
-    batch: int = 1024
+    batch: int = 0
 
     @property
     def structure(self) -> AtomicStructure:

@@ -69,3 +69,17 @@ def test_scan_512_equals_the_oracle(scan_case, opt):
     for m, wv in zip(got, want):
         wv = wv.reshape(-1)
         assert np.abs(m.values - wv).max() <= ACC.detector_bound(np.abs(wv).max())
+
+
+@pytest.mark.parametrize("batch", [1, 5, 7, 12])
+def test_scan_512_balanced_spans_equal_the_oracle(scan_case, batch):
+    """The 8-column pass cuts the (tile pair, position) units of a launch into equal spans, one per pair of blocks;
+    batches of 5 and 7 give spans that cross from one tile pair to the next (P reloaded mid-span), batch 1 gives
+    spans of one unit. The 16-column blocks run the same launches as the control."""
+    prepared, probe, scan, dets, want = scan_case
+    got = sk.STEM.scan(probe, [prepared], scan, dets, sk.ExecutionOptions(batch=batch, col_tile=8))
+    ctl = sk.STEM.scan(probe, [prepared], scan, dets, sk.ExecutionOptions(batch=batch, col_tile=16))
+    for m, c, wv in zip(got, ctl, want):
+        wv = wv.reshape(-1)
+        assert np.abs(m.values - wv).max() <= ACC.detector_bound(np.abs(wv).max())
+        assert np.abs(m.values - c.values).max() <= 1e-6 * np.abs(wv).max()
