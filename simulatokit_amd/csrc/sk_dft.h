@@ -46,6 +46,31 @@ __device__ inline void dft8(float2 (&u)[8], int sign) {
   u[7] = csub(e3, w3);
 }
 
+// dft8 of a vector whose samples 3 and 4 are zero (not read): the two DFT4s drop the four additions with a zero
+// operand. Equal to dft8 on such input up to the sign of exact zeros.
+__device__ inline void dft8_z34(float2 (&u)[8], int sign) {
+  // even samples (u0, u2, 0, u6)
+  const float2 et2 = cadd(u[2], u[6]), et3 = mul_si(csub(u[2], u[6]), sign);
+  const float2 e0 = cadd(u[0], et2), e1 = cadd(u[0], et3), e2 = csub(u[0], et2), e3 = csub(u[0], et3);
+  // odd samples (u1, 0, u5, u7)
+  const float2 ot0 = cadd(u[1], u[5]), ot1 = csub(u[1], u[5]);
+  const float2 ot3 = mul_si(make_float2(-u[7].x, -u[7].y), sign);
+  const float2 o0 = cadd(ot0, u[7]), o1 = cadd(ot1, ot3), o2 = csub(ot0, u[7]), o3 = csub(ot1, ot3);
+  const float h = 0.70710678118654752f;
+  const float s = float(sign);
+  const float2 w1 = make_float2((o1.x - s * o1.y) * h, (o1.y + s * o1.x) * h);
+  const float2 w2 = mul_si(o2, sign);
+  const float2 w3 = make_float2((-o3.x - s * o3.y) * h, (-o3.y + s * o3.x) * h);
+  u[0] = cadd(e0, o0);
+  u[4] = csub(e0, o0);
+  u[1] = cadd(e1, w1);
+  u[5] = csub(e1, w1);
+  u[2] = cadd(e2, w2);
+  u[6] = csub(e2, w2);
+  u[3] = cadd(e3, w3);
+  u[7] = csub(e3, w3);
+}
+
 // One pixel of the final pass: intensity I at (ix, iy) into the scalar slots and the pixelated accumulators
 // (AnnularDetector, CenterOfMassDetector, PixelatedDetector; Detectors.swift:119-205).
 __device__ inline void detect_pixel(const DetPlan& d, int ix, int iy, double I, long long pos,

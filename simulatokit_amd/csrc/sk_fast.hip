@@ -129,9 +129,12 @@ __device__ __forceinline__ int tile_slot(int x, int c) {
 
 // One Stockham stage of radix R at sub-length L (see the file comment); W: the lane's factors for this stage
 // (ignored when L = 1). FROM_REG: inputs are the registers; TO_REG: outputs go to the registers. WAVE: the line
-// lives in one wavefront, so the exchanges need no block barrier.
-template <int N, int R, int L, int SIGN, bool FROM_REG, bool TO_REG, bool WAVE = false, class Idx, class Tw>
+// lives in one wavefront, so the exchanges need no block barrier. Z34: the first radix-8 stage of a line whose
+// registers 3 and 4 are zero in every lane (they are not read).
+template <int N, int R, int L, int SIGN, bool FROM_REG, bool TO_REG, bool WAVE = false, bool Z34 = false, class Idx,
+          class Tw>
 __device__ __forceinline__ void stage(float2 (&v)[8], float2* __restrict__ buf, int i, Tw W, Idx idx) {
+  static_assert(!Z34 || (R == 8 && L == 1 && FROM_REG), "Z34: the first stage of radix 8, from the registers");
   constexpr int T = N / 8;
   constexpr int NB = 8 / R;
   constexpr int STRIDE = N / R;
@@ -157,7 +160,8 @@ __device__ __forceinline__ void stage(float2 (&v)[8], float2* __restrict__ buf, 
         u[q][j] = SIGN < 0 ? cmul(u[q][j], wf) : cmulc(u[q][j], wf);
       }
     }
-    dft_pow2<R>(u[q], SIGN);
+    if constexpr (Z34) dft8_z34(u[q], SIGN);
+    else dft_pow2<R>(u[q], SIGN);
     if constexpr (TO_REG) {
 #pragma unroll
       for (int j = 0; j < R; ++j) v[q + NB * j] = u[q][j];
@@ -170,14 +174,15 @@ __device__ __forceinline__ void stage(float2 (&v)[8], float2* __restrict__ buf, 
   if constexpr (!TO_REG) exchange_sync<WAVE>();
 }
 
-// Unscaled transform of the line in registers (SIGN -1 forward, +1 inverse).
-template <int N, int SIGN, bool WAVE = false, class Idx, class TW>
+// Unscaled transform of the line in registers (SIGN -1 forward, +1 inverse). Z34: registers 3 and 4 hold zeros in
+// every lane (elements 3T..5T-1 of the line).
+template <int N, int SIGN, bool WAVE = false, bool Z34 = false, class Idx, class TW>
 __device__ __forceinline__ void fft_reg(float2 (&v)[8], float2* buf, int i, const TW& w, Idx idx) {
   using P = FastPlan<N>;
   auto none = [](int, int) { return make_float2(1.f, 0.f); };
   auto f2 = [&w](int q, int j) { return w.s2(q, j); };
   auto f3 = [&w](int q, int j) { return w.s3(q, j); };
-  stage<N, P::R1, 1, SIGN, true, false, WAVE>(v, buf, i, none, idx);
+  stage<N, P::R1, 1, SIGN, true, false, WAVE, Z34>(v, buf, i, none, idx);
   stage<N, P::R2, P::R1, SIGN, false, false, WAVE>(v, buf, i, f2, idx);
   stage<N, P::R3, P::R1 * P::R2, SIGN, false, true, WAVE>(v, buf, i, f3, idx);
 }
@@ -201,8 +206,14 @@ __device__ __forceinline__ int compact_of(const RowArgs& a, int iy) {
 
 // Row pass: block = one wavefront = 64 / T rows x of positions [b0, b0 + PB). PIPE: the positions belong to one lane
 // of the pipeline, whose descriptor for the current step gives the transmission function and the position count.
-template <int N, int FLAGS, int PB, bool PIPE = false>
+// BAND (compact input and output): the kept set has the shape of a 2/3 bandlimit, 2T-1 <= kept_hi < 3T and
+// 5T <= kept_lo <= 6T (the host checks), so of the lane's elements i + T r those of r = 0, 1, 6, 7 are always kept,
+// r = 3, 4 never (the inverse transform starts from zeros there, the forward one's values there are dropped), and
+// only r = 2 and 5 depend on the lane.
+template <int N, int FLAGS, int PB, bool PIPE = false, bool BAND = false>
 __global__ void __launch_bounds__(64) row_fast_kernel(RowArgs a, const float2* __restrict__ tw, int batch) {
+  static_assert(!BAND || (FLAGS & (ROW_IN_COMPACT | ROW_OUT_COMPACT | ROW_INV)) ==
+                             (ROW_IN_COMPACT | ROW_OUT_COMPACT | ROW_INV), "BAND: compact in and out");
   constexpr int T = N / 8, ROWS = 64 / T, LD = N;
   __shared__ float2 lds[ROWS * LD];
   const int tid = threadIdx.x, row = tid / T, i = tid - row * T;
@@ -235,7 +246,16 @@ __global__ void __launch_bounds__(64) row_fast_kernel(RowArgs a, const float2* _
     const long long b = b0 + p;
     if (b >= limit) break;  // uniform over the block
     float2 v[8];
-    if constexpr ((FLAGS & ROW_IN_COMPACT) != 0) {
+    if constexpr (BAND) {
+      const float2* g = a.G + b * gw + (long long)x * a.nc;
+      v[0] = g[i];
+      v[1] = g[i + T];
+      v[2] = cidx[2] >= 0 ? g[cidx[2]] : make_float2(0.f, 0.f);
+      v[3] = v[4] = make_float2(0.f, 0.f);
+      v[5] = cidx[5] >= 0 ? g[cidx[5]] : make_float2(0.f, 0.f);
+      v[6] = g[cidx[6]];
+      v[7] = g[cidx[7]];
+    } else if constexpr ((FLAGS & ROW_IN_COMPACT) != 0) {
       const float2* g = a.G + b * gw + (long long)x * a.nc;
 #pragma unroll
       for (int r = 0; r < 8; ++r) v[r] = cidx[r] >= 0 ? g[cidx[r]] : make_float2(0.f, 0.f);
@@ -244,7 +264,7 @@ __global__ void __launch_bounds__(64) row_fast_kernel(RowArgs a, const float2* _
 #pragma unroll
       for (int r = 0; r < 8; ++r) v[r] = g[i + T * r];
     }
-    if constexpr ((FLAGS & ROW_INV) != 0) fft_reg<N, +1, true>(v, buf, i, w, idx);
+    if constexpr ((FLAGS & ROW_INV) != 0) fft_reg<N, +1, true, BAND>(v, buf, i, w, idx);
     if constexpr ((FLAGS & ROW_WRITE_PLANE) != 0) {
       float2* pl = a.plane + b * a.plane_stride + (long long)x * N;
 #pragma unroll
@@ -255,7 +275,15 @@ __global__ void __launch_bounds__(64) row_fast_kernel(RowArgs a, const float2* _
       for (int r = 0; r < 8; ++r) v[r] = cmul(v[r], t[r]);
     }
     if constexpr ((FLAGS & ROW_FWD) != 0) fft_reg<N, -1, true>(v, buf, i, w, idx);
-    if constexpr ((FLAGS & ROW_OUT_COMPACT) != 0) {
+    if constexpr (BAND) {
+      float2* g = a.G + b * gw + (long long)x * a.nc;
+      g[i] = v[0];
+      g[i + T] = v[1];
+      if (cidx[2] >= 0) g[cidx[2]] = v[2];
+      if (cidx[5] >= 0) g[cidx[5]] = v[5];
+      g[cidx[6]] = v[6];
+      g[cidx[7]] = v[7];
+    } else if constexpr ((FLAGS & ROW_OUT_COMPACT) != 0) {
       float2* g = a.G + b * gw + (long long)x * a.nc;
 #pragma unroll
       for (int r = 0; r < 8; ++r)
@@ -620,10 +648,22 @@ bool is_fast_length(int n) { return n == 256 || n == 512; }
 constexpr int kRowPB = 8;  // positions per row-pass block
 constexpr int kColPB = 8;  // positions per column-pass block (P and the twiddles loaded once per 8 positions)
 
+// The kept set has the shape the BAND row kernels assume (a 2/3 bandlimit of an N-point line, T = N / 8).
+bool band_shape(const RowArgs& a, int N) {
+  const int T = N / 8;
+  return a.kept_hi >= 2 * T - 1 && a.kept_hi < 3 * T && a.kept_lo >= 5 * T && a.kept_lo <= 6 * T;
+}
+
 template <int N, int F>
 void row_go(const RowArgs& a, int batch, hipStream_t s, const float2* tw) {
   constexpr int ROWS = 64 / (N / 8);
   dim3 grid(a.nx / ROWS, (batch + kRowPB - 1) / kRowPB);
+  if constexpr (F == ROW_MID) {
+    if (band_shape(a, N)) {
+      hipLaunchKernelGGL((row_fast_kernel<N, F, kRowPB, false, true>), grid, dim3(64), 0, s, a, tw, batch);
+      return;
+    }
+  }
   hipLaunchKernelGGL((row_fast_kernel<N, F, kRowPB>), grid, dim3(64), 0, s, a, tw, batch);
 }
 
@@ -739,12 +779,15 @@ int pipe_positions_per_block() { return kPipePB; }
 void launch_row_pipe(const RowArgs& a, int batch, hipStream_t s) {
   const float2* tw = line_plan(a.ny, s).dev.roots;
   constexpr int F = ROW_MID;
+  const bool band = band_shape(a, a.ny);
   if (a.ny == 512) {
     dim3 grid(a.nx / 1, batch / kPipePB);
-    hipLaunchKernelGGL((row_fast_kernel<512, F, kPipePB, true>), grid, dim3(64), 0, s, a, tw, batch);
+    if (band) hipLaunchKernelGGL((row_fast_kernel<512, F, kPipePB, true, true>), grid, dim3(64), 0, s, a, tw, batch);
+    else hipLaunchKernelGGL((row_fast_kernel<512, F, kPipePB, true>), grid, dim3(64), 0, s, a, tw, batch);
   } else {
     dim3 grid(a.nx / 2, batch / kPipePB);
-    hipLaunchKernelGGL((row_fast_kernel<256, F, kPipePB, true>), grid, dim3(64), 0, s, a, tw, batch);
+    if (band) hipLaunchKernelGGL((row_fast_kernel<256, F, kPipePB, true, true>), grid, dim3(64), 0, s, a, tw, batch);
+    else hipLaunchKernelGGL((row_fast_kernel<256, F, kPipePB, true>), grid, dim3(64), 0, s, a, tw, batch);
   }
   SK_HIP(hipGetLastError());
 }
