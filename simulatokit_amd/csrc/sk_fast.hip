@@ -432,15 +432,22 @@ __device__ __forceinline__ void col512_run(const ColArgs& a, int u0, int u1, Whe
   const long long gw = (long long)a.nx * a.nc;
   auto tile_slot = [](int x, int c) { return sk::tile_slot<TC>(x, c); };
   auto widx = [wave, tile_slot](int j) { return tile_slot(j, wave); };
+  // Block-phase element r of the lane in the [x][nc] rows of the tile that starts at t0: row xi + T r, column cb.
+  // Tile starts are uniform over the block, so the accesses take a scalar base plus the lane's 32-bit byte offset.
+  const unsigned loff = (unsigned)(xi * a.nc + cb) * (unsigned)sizeof(float2);
+  const long long rstep = (long long)T * a.nc * (long long)sizeof(float2);
+  auto elem = [loff, rstep](const float2* t0, int r) {
+    return reinterpret_cast<const float2*>(reinterpret_cast<const char*>(t0) + r * rstep + loff);
+  };
   // the block-phase load of unit u: column cb of its tile, rows xi + 64 r
   auto gload = [&](int u, float2 (&g)[8]) {
     int tx, b;
     where(u, tx, b);
     const int c = tx * TC + cb, iy = a.keep[c];
     if constexpr ((FLAGS & COL_IN_G) != 0) {
-      const float2* src = a.G + b * gw + c;
+      const float2* t0 = a.G + b * gw + tx * TC;
 #pragma unroll
-      for (int r = 0; r < 8; ++r) g[r] = iy >= 0 ? src[(long long)(xi + T * r) * a.nc] : make_float2(0.f, 0.f);
+      for (int r = 0; r < 8; ++r) g[r] = iy >= 0 ? *elem(t0, r) : make_float2(0.f, 0.f);
     } else {
 #pragma unroll
       for (int r = 0; r < 8; ++r) {
@@ -451,10 +458,13 @@ __device__ __forceinline__ void col512_run(const ColArgs& a, int u0, int u1, Whe
       }
     }
   };
+  // P is zero on the rows x = 3T .. 5T-1 (r = 3, 4 of every lane): there |kx| >= 0.375 / sx exceeds the antialias
+  // cutoff 0.3333333 / max(sx, sy), and the propagator carries that aperture. Those rows are neither loaded nor
+  // multiplied, and the inverse transform starts from zeros there.
   auto pload = [&](int tx) {
-    const float2* P = a.P + tx * TC + cb;
 #pragma unroll
-    for (int r = 0; r < 8; ++r) ptile[tile_slot(xi + T * r, cb)] = P[(long long)(xi + T * r) * a.nc];
+    for (int r = 0; r < 8; ++r)
+      if (r != 3 && r != 4) ptile[tile_slot(xi + T * r, cb)] = *elem(a.P + tx * TC, r);
   };
   float2 g[8];
   int tcur = -1;
@@ -485,7 +495,8 @@ __device__ __forceinline__ void col512_run(const ColArgs& a, int u0, int u1, Whe
     if constexpr ((FLAGS & COL_FWD) != 0) fft_reg<N, -1, true>(v, tile, lane, w, widx);
     if constexpr (HAS_P) {
 #pragma unroll
-      for (int r = 0; r < 8; ++r) v[r] = cmul(v[r], ptile[tile_slot(lane + T * r, wave)]);
+      for (int r = 0; r < 8; ++r)
+        v[r] = (r == 3 || r == 4) ? make_float2(0.f, 0.f) : cmul(v[r], ptile[tile_slot(lane + T * r, wave)]);
     }
     if constexpr ((FLAGS & COL_DETECT) != 0) {
       int tx, b;
@@ -494,18 +505,17 @@ __device__ __forceinline__ void col512_run(const ColArgs& a, int u0, int u1, Whe
                           reinterpret_cast<float2*>(red), tx);
       continue;  // detect_block ends with a barrier: the tile is free again
     }
-    if constexpr ((FLAGS & COL_INV) != 0) fft_reg<N, +1, true>(v, tile, lane, w, widx);
+    if constexpr ((FLAGS & COL_INV) != 0) fft_reg<N, +1, true, HAS_P>(v, tile, lane, w, widx);
 #pragma unroll
     for (int r = 0; r < 8; ++r) tile[tile_slot(lane + T * r, wave)] = v[r];
     __syncthreads();
     {
       int tx, b;
       where(u, tx, b);
-      const int c = tx * TC + cb;
-      float2* dst = a.G + b * gw + c;
-      if (a.keep[c] >= 0) {  // padding columns stay as the row pass leaves them: never read, so never written
+      const float2* t0 = a.G + b * gw + tx * TC;
+      if (a.keep[tx * TC + cb] >= 0) {  // padding columns stay as the row pass leaves them: never read or written
 #pragma unroll
-        for (int r = 0; r < 8; ++r) dst[(long long)(xi + T * r) * a.nc] = tile[tile_slot(xi + T * r, cb)];
+        for (int r = 0; r < 8; ++r) *const_cast<float2*>(elem(t0, r)) = tile[tile_slot(xi + T * r, cb)];
       }
     }
     __syncthreads();
