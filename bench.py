@@ -2,11 +2,13 @@
 
 Workload (the headline HAADF configuration): SrTiO3 tiled 8 x 8 x 20 (31.24 x 31.24 x 78.1 Angstrom, 6,400 atoms),
 512 x 512 grid, 40 slices of 1.9525 Angstrom, 200 keV probe with a 20 mrad soft aperture (C10 = C30 = 0), a 64 x 64
-GridScan over one 3.905 Angstrom cell, annular detector 50-200 mrad, complex64 waves, 1024 positions per launch.
+GridScan over one 3.905 Angstrom cell, annular detector 50-200 mrad, complex64 waves.
 
 One step = one full 64 x 64 scan (4,096 positions x 40 slices = 163,840 position-slices) with the prepared
 potential resident in HBM and the HAADF image left in HBM. `value` = position-slices per second over the K timed
-steps. The roofline entry times the dominant kernel with HIP events on the library's stream; the CPU baseline runs
+steps at the library's positions per launch: the largest batch whose waves stay in the 256 MiB Infinity Cache,
+spread evenly over the scan (171 here). The same scan at 1024 positions per launch (`--compare-batch`) is timed
+first and reported beside it as `compare_batch`. The roofline entry times the dominant kernel with HIP events on the library's stream; the CPU baseline runs
 the float64 oracle on a bounded sample of positions on the host's cores, and the same sample gives the HIP path's
 error against float64.
 
@@ -29,8 +31,9 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "column_pass_haadf_512_traffic.md")
 
 
-def measured_traffic(kernel: str):
-    """The `traffic` block of the committed counter summary for `kernel` at this workload, or None."""
+def measured_traffic(kernel: str, batch: int):
+    """The `traffic` block of the committed counter summary for `kernel` at this workload and positions per launch,
+    or None."""
     try:
         text = open(TRAFFIC_SUMMARY).read()
     except OSError:
@@ -41,7 +44,9 @@ def measured_traffic(kernel: str):
     end = text.find("```", start + 7)
     data = json.loads(text[start + 7:end])
     entry = data.get(kernel)
-    return None if entry is None else float(entry["hbm_bytes_per_launch"])
+    if entry is None or int(entry.get("batch", -1)) != batch:
+        return None
+    return float(entry["hbm_bytes_per_launch"])
 
 
 def main():
@@ -169,7 +174,7 @@ def main():
                    "semiangle_mrad": w.semiangle, "fast_path": not args.generic, "graph": bool(args.graph),
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(f"{dom} pass"),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(f"{dom} pass", B),
                      "kernel": f"{dom} pass", "kernel_ms": dom_ms, "row_ms": row_ms, "col_ms": col_ms,
                      "algorithmic_bytes_per_launch": alg_bytes, "stored_bytes_per_launch": stored_bytes,
                      "copy_GBs": copy_gbs},
