@@ -1588,9 +1588,12 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
     int B = default_batch(p0, npos, opt);
     // lane pipeline: lanes of W positions, each on its own potential and slice at every step
     int pipe = opt ? opt->pipeline : 0;
-    // the library's choice: lanes when one potential's positions cannot fill a launch (measured at 512^2 with 16
-    // configurations x 16,384 positions, one potential at a time runs 1.26 x faster than the lanes)
-    if (pipe < 0) pipe = (npots > 1 && npos < default_batch(p0, 1 << 30, opt)) ? 1 : 0;
+    // The library's choice is one potential at a time. A lane owns its chunks of positions, so the lanes never
+    // hold more positions than one potential's scan, and their per-step bookkeeping then costs more than the uneven
+    // slice counts save. Measured at 512^2 over 16 configurations of 43-115 slices, one potential at a time was
+    // faster at every scan size: 16 positions 1.60 x, 64 positions 1.55 x, 144 positions 1.43 x, 16,384 positions
+    // 1.35 x.
+    if (pipe < 0) pipe = 0;
     int W = 0, L = 0;
     if (pipe) {
       const int q = pipe_positions_per_block();

@@ -83,3 +83,24 @@ def test_scan_512_balanced_spans_equal_the_oracle(scan_case, batch):
         wv = wv.reshape(-1)
         assert np.abs(m.values - wv).max() <= ACC.detector_bound(np.abs(wv).max())
         assert np.abs(m.values - c.values).max() <= 1e-6 * np.abs(wv).max()
+
+
+@pytest.mark.parametrize("tiles", [(4, 2, 2), (2, 4, 2)], ids=["x_long", "y_long"])
+def test_scan_512_anisotropic_sampling_equals_the_oracle(tiles):
+    """Unequal sampling in x and y. With the cell twice as long in x, the antialias cutoff follows the coarser x
+    sampling, so only |ky| up to a third of the y Nyquist frequency is kept: the kept set is narrower than the 2/3
+    band the band-pruned row kernel assumes, and the plain row kernel runs. With the cell longer in y, the kept set
+    is the full 2/3 band in y and the column pass's zero rows of P reach further in x. Both run through the column
+    pass's P-band pruning."""
+    st = srtio3(*tiles)
+    pot = sk.Potential(st, 512, 512, slice_thickness=2.0)
+    V, _ = O.potential(512, 512, st.numbers, st.positions, st.box, pot.slice_thicknesses)
+    probe = sk.Probe(sk.ElectronBeam(200e3), pot.grid, sk.Aperture(20.0))
+    scan = sk.GridScan((0, 0), (SRTIO3_A, SRTIO3_A), 2, 2)
+    dets = [sk.AnnularDetector(50, 200), sk.PixelatedDetector("cutoff")]
+    want = O.stem_scan([V], [pot.slice_thicknesses], (st.box[0], st.box[1]), 200e3, 20.0, scan.positions,
+                       [O.det_annular(50, 200), O.det_pixelated("cutoff")])
+    got = sk.STEM.scan(probe, [pot.prepare(probe.beam)], scan, dets)
+    for m, wv in zip(got, want):
+        wv = wv.reshape(-1)
+        assert np.abs(m.values - wv).max() <= ACC.detector_bound(np.abs(wv).max())
