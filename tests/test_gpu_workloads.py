@@ -76,3 +76,22 @@ def test_full_haadf_scan_equals_the_oracle_at_sample_positions(haadf_scan):
     got = haadf.values[pick]
     print("HAADF rel err at samples", np.abs(got - want) / np.abs(want))
     assert rel_max(got, want) <= TOL
+
+
+def test_default_batch_is_sized_to_the_infinity_cache(haadf_scan):
+    """The library's positions per launch: the largest batch whose compact waves (nx x 352 complex64 at 512^2) take at
+    most 240 MiB, spread evenly over the scan: 4,096 positions -> 24 launches of at most 171. A fixed batch is
+    honoured as given, and the chunking changes no number the scan computes."""
+    w, st, pot, scan, haadf, _ = haadf_scan
+    probe = sk.Probe(sk.ElectronBeam(w.energy), pot.grid, sk.Aperture(w.semiangle))
+    prepared = pot.prepare(probe.beam)
+    det = [sk.AnnularDetector(w.inner, w.outer)]
+    sk.set_timing(True)
+    try:
+        sk.STEM.scan(probe, [prepared], scan, det)
+        assert sk.last_timing()["batch"] == 171
+        fixed = sk.STEM.scan(probe, [prepared], scan, det, sk.ExecutionOptions(batch=1024))[0]
+        assert sk.last_timing()["batch"] == 1024
+    finally:
+        sk.set_timing(False)
+    assert rel_max(fixed.values, haadf.values) <= 1e-12
