@@ -209,7 +209,8 @@ class ExecutionOptions:
     use_graph: bool = False  # replay the slice loop from captured hipGraphs
     fast_path: bool = True   # register-resident FFT passes where the grid length has one
     pipeline: int = -1       # scan: 1 lanes of positions each on its own potential and slice per launch, 0 one
-                             # potential at a time, -1 the library's choice (one potential at a time)
+                             # potential at a time, -1 the library's choice (lanes for a few positions over several
+                             # potentials, else one potential at a time)
     lane_waves: int = 0      # positions per lane (0 = the library's choice)
     col_tile: int = 0        # 512-point columns: columns per column-pass block, 8 or 16 (0 = the library's choice)
     precision: str = "float32"  # the slice loop's transforms: "float32", or "split_bf16" (matrix cores; 256, 512)

@@ -472,6 +472,8 @@ struct Workspace {
   DevBuf<LaneDesc> descs;          // lane pipeline: [steps + 1][lanes]
   DevBuf<int> step;                // lane pipeline: step counter
   DevBuf<float2> rx_all, ry_all;   // lane pipeline: ramps of every position
+  DevBuf<double> acc_copies;       // lane pipeline with shared chunks: [lanes][npos][slots]
+  std::vector<DevBuf<double>> pix_copies;  // and [lanes][npos][mx][my] per pixelated detector
   std::vector<DevBuf<int>> maps;
   std::vector<DevBuf<double>> weights;
 };
@@ -502,6 +504,8 @@ LineFFT slice_plan(int n, hipStream_t s) {
 // kMinCachedBatch positions (enough to fill the GPU), else a 4 GiB budget of at most 1024 positions, and then
 // spreads `count` evenly over the batches, so no small remainder batch runs at a fraction of the GPU.
 constexpr int kMinCachedBatch = 64;
+// device memory the lane pipeline may take for per-lane detector accumulators when lanes share positions
+constexpr size_t kLaneCopyBytes = size_t(2) << 30;
 
 int default_batch(const sk_potential& p, int count, const sk_options* opt) {
   int b = opt && opt->batch > 0 ? opt->batch : 0;
@@ -668,10 +672,30 @@ void run_pipeline(const sk_potential* const* pots, int npots, int npos, int W, i
   };
   std::vector<std::vector<Item>> lane_items(L);
   const int chunks = (npos + W - 1) / W;
-  for (int q = 0; q < chunks; ++q) {
-    const int lane = q % L;
+  // With no more lanes than chunks, every lane owns its chunks and runs all potentials over them (rotated, so that
+  // the lanes hold different potentials at a time): no two lanes ever accumulate into one position. With more lanes
+  // than chunks (a few positions over many potentials), the (chunk, potential) items go to the lanes longest first,
+  // each to the lane with the fewest slices so far, and every lane accumulates into its own copy of the detector
+  // accumulators; the copies are summed in lane order at the end.
+  const bool copies = L > chunks;
+  if (!copies) {
+    for (int q = 0; q < chunks; ++q) {
+      const int lane = q % L;
+      for (int k = 0; k < npots; ++k)
+        lane_items[lane].push_back({q * W, std::min(W, npos - q * W), (k + lane) % npots});
+    }
+  } else {
+    std::vector<Item> items;
     for (int k = 0; k < npots; ++k)
-      lane_items[lane].push_back({q * W, std::min(W, npos - q * W), (k + lane) % npots});
+      for (int q = 0; q < chunks; ++q) items.push_back({q * W, std::min(W, npos - q * W), k});
+    std::stable_sort(items.begin(), items.end(),
+                     [pots](const Item& a, const Item& b) { return pots[a.pot]->nslices > pots[b.pot]->nslices; });
+    std::vector<long long> load(L, 0);
+    for (const Item& it : items) {
+      const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      lane_items[l].push_back(it);
+      load[l] += pots[it.pot]->nslices;
+    }
   }
   int nsteps = 0;
   for (auto& li : lane_items) {
@@ -683,10 +707,12 @@ void run_pipeline(const sk_potential* const* pots, int npots, int npos, int W, i
   for (int l = 0; l < L; ++l) {
     const auto& li = lane_items[l];
     if (li.empty()) continue;
+    const int acc_off = copies ? l * npos : 0;
     LaneDesc& pro = descs[l];
     pro.mode = LANE_INIT;
     pro.next_pos = li[0].pos;
     pro.next_count = li[0].count;
+    pro.acc_off = acc_off;
     int st = 1;
     for (size_t m = 0; m < li.size(); ++m) {
       const sk_potential& P = *pots[li[m].pot];
@@ -696,6 +722,7 @@ void run_pipeline(const sk_potential* const* pots, int npots, int npos, int W, i
         d.P = P.props.p + (size_t)P.prop_of_slice[j] * nx * P.nc;
         d.pos = li[m].pos;
         d.count = li[m].count;
+        d.acc_off = acc_off;
         if (j < P.nslices - 1) {
           d.mode = LANE_MID;
         } else if (m + 1 < li.size()) {
@@ -718,13 +745,29 @@ void run_pipeline(const sk_potential* const* pots, int npots, int npos, int W, i
     launch_ramps(nx, ny, g.extent_x / nx, g.extent_y / ny, w.positions.p, q0, nullptr, cnt,
                  w.rx_all.p + (size_t)q0 * nx, w.ry_all.p + (size_t)q0 * ny, s);
   }
+  DetPlan dpl = dp;  // where the lanes accumulate
+  double* acc = w.acc.p;
+  const int nslots = std::max(1, dp.nslots);
+  if (copies) {
+    const size_t n = (size_t)npos * nslots;
+    w.acc_copies.ensure(n * L);
+    SK_HIP(hipMemsetAsync(w.acc_copies.p, 0, sizeof(double) * n * L, s));
+    acc = w.acc_copies.p;
+    w.pix_copies.resize(std::max<size_t>(w.pix_copies.size(), (size_t)dp.npix));
+    for (int q = 0; q < dp.npix; ++q) {
+      const size_t m = (size_t)npos * dp.pix_stride[q];
+      w.pix_copies[q].ensure(m * L);
+      SK_HIP(hipMemsetAsync(w.pix_copies[q].p, 0, sizeof(double) * m * L, s));
+      dpl.pix_acc[q] = w.pix_copies[q].p;
+    }
+  }
   PipeArgs pa{w.descs.p, w.step.p, L, W, w.rx_all.p, w.ry_all.p};
   RowArgs r = row_args(p0, w.G.p);
   r.flags = ROW_IN_COMPACT | ROW_INV | ROW_MUL_T | ROW_FWD | ROW_OUT_COMPACT;
   r.pipe = pa;
   ColArgs cc = col_args(p0, w.G.p);
   cc.spec = w.spec.p;
-  cc.det = dp;
+  cc.det = dpl;
   cc.pipe = pa;
   auto step_fn = [&](Timer& t) {
     t.mark_start(s, 0);
@@ -733,8 +776,15 @@ void run_pipeline(const sk_potential* const* pots, int npots, int npos, int W, i
     t.mark_start(s, 1);
     launch_col_pipe(cc, B, s);
     t.mark_stop(s);
-    launch_reduce_pipe(dp, pa, B, w.acc.p, s);
+    launch_reduce_pipe(dpl, pa, B, acc, s);
     launch_step_increment(w.step.p, s);
+  };
+  // the lanes' copies into the scan's accumulators, in lane order
+  auto fold_copies = [&] {
+    if (!copies) return;
+    launch_sum_copies(w.acc_copies.p, (long long)npos * nslots, L, w.acc.p, s);
+    for (int q = 0; q < dp.npix; ++q)
+      launch_sum_copies(w.pix_copies[q].p, (long long)npos * dp.pix_stride[q], L, dp.pix_acc[q], s);
   };
   // prologue: the probe of every lane's first item (descriptor row 0)
   launch_col_pipe(cc, B, s);
@@ -745,6 +795,7 @@ void run_pipeline(const sk_potential* const* pots, int npots, int npos, int W, i
       if ((st & 31) == 31) tm.harvest(timing);
     }
     tm.harvest(timing);
+    fold_copies();
     return;
   }
   // hipGraph: K steps captured once; every replay reads its steps from the device counter
@@ -767,6 +818,7 @@ void run_pipeline(const sk_potential* const* pots, int npots, int npos, int W, i
   int st = 0;
   for (; st + K <= nsteps; st += K) SK_HIP(hipGraphLaunch(ge, s));
   for (; st < nsteps; ++st) step_fn(off);
+  fold_copies();
   SK_HIP(hipStreamSynchronize(s));
   SK_HIP(hipGraphExecDestroy(ge));
 }
@@ -1588,19 +1640,29 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
     int B = default_batch(p0, npos, opt);
     // lane pipeline: lanes of W positions, each on its own potential and slice at every step
     int pipe = opt ? opt->pipeline : 0;
-    // The library's choice is one potential at a time. A lane owns its chunks of positions, so the lanes never
-    // hold more positions than one potential's scan, and their per-step bookkeeping then costs more than the uneven
-    // slice counts save. Measured at 512^2 over 16 configurations of 43-115 slices, one potential at a time was
-    // faster at every scan size: 16 positions 1.60 x, 64 positions 1.55 x, 144 positions 1.43 x, 16,384 positions
-    // 1.35 x.
-    if (pipe < 0) pipe = 0;
+    // The library's choice: lanes when there are several potentials and one potential's positions fill at most an
+    // eighth of a launch. Lanes holding the same positions for different potentials each accumulate into their own
+    // copy, so a launch carries up to a cache-sized batch of (position, potential) pairs. Measured at 512^2 over 16
+    // configurations of 43-115 slices: 16 positions ran 1.46 x faster in lanes than one potential at a time, while
+    // 64 positions ran 1.21 x slower, since the lanes' column pass is slower per position than the balanced one.
+    if (pipe < 0) pipe = (npots > 1 && 8 * npos <= default_batch(p0, 1 << 30, opt)) ? 1 : 0;
     int W = 0, L = 0;
     if (pipe) {
       const int q = pipe_positions_per_block();
       W = opt && opt->lane_waves > 0 ? opt->lane_waves
                                      : std::max(q, std::min(128, ((npos + 7) / 8 + q - 1) / q * q));
       const int chunks = (npos + W - 1) / W;
-      L = std::max(1, std::min(chunks, std::max(B, W) / W));
+      // lanes: as many as a launch of the cache-sized batch holds, up to one per (chunk, potential) item; lanes
+      // beyond the chunk count share chunks and need their own accumulator copies, within kLaneCopyBytes
+      const int Bp = default_batch(p0, 1 << 30, opt);
+      L = (int)std::max<long long>(1, std::min<long long>((long long)chunks * npots, std::max(Bp, W) / W));
+      if (L > chunks) {
+        double per_lane = (double)npos * std::max(1, dp.nslots);
+        for (int k = 0; k < dp.npix; ++k) per_lane += (double)npos * dp.pix_stride[k];
+        per_lane *= sizeof(double);
+        const int fit = (int)std::min<double>(L, (double)kLaneCopyBytes / per_lane);
+        L = std::max(std::min(L, chunks), fit);
+      }
       if (!(fast && fits && pipeline_supported(nx, ny, p0.nc, W))) {
         if (opt && opt->pipeline > 0)
           fail(SK_ERR_INVALID, "the lane pipeline needs the fast passes (grid lengths 256 or 512), lanes of a "

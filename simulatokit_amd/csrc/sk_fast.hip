@@ -616,7 +616,7 @@ __global__ void __launch_bounds__(TC * (N / 8)) col_pipe_kernel(ColArgs a, const
         for (int r = 0; r < 8; ++r) g[(long long)(i + T * r) * a.nc] = v[r];
         continue;
       }
-      detect_block<N, NT>(v, a.det, (long long)d.pos + local, b, iy, i, tid, lds, blockIdx.x);
+      detect_block<N, NT>(v, a.det, (long long)d.pos + d.acc_off + local, b, iy, i, tid, lds, blockIdx.x);
     }
     if ((d.mode == LANE_FINAL_INIT || d.mode == LANE_INIT) && local < d.next_count) {
       const long long q = (long long)d.next_pos + local;
@@ -640,13 +640,22 @@ __global__ void reduce_pipe_kernel(DetPlan d, PipeArgs pa, double* acc) {
   const int lane = int(b / pa.lane_waves), local = int(b - (long long)lane * pa.lane_waves);
   const LaneDesc ld = pa.descs[(long long)(*pa.step) * pa.lanes + lane];
   if (!(ld.mode == LANE_FINAL || ld.mode == LANE_FINAL_INIT) || local >= ld.count) return;
-  const long long pos = (long long)ld.pos + local;
+  const long long pos = (long long)ld.pos + ld.acc_off + local;
   for (int q = threadIdx.x; q < d.nslots; q += blockDim.x) {
     const double* p = d.partial + (b * d.nslots + q) * d.blocks_per_wave;
     double s = 0.0;
     for (int k = 0; k < d.blocks_per_wave; ++k) s += p[k];
     acc[pos * d.nslots + q] += s;
   }
+}
+
+// The lanes' accumulator copies into the scan's accumulators, in the fixed order of the copies.
+__global__ void sum_copies_kernel(const double* __restrict__ in, long long n, int copies, double* __restrict__ out) {
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double s = out[e];
+  for (int c = 0; c < copies; ++c) s += in[c * n + e];
+  out[e] = s;
 }
 
 __global__ void step_increment_kernel(int* step) {
@@ -816,6 +825,12 @@ void launch_col_pipe(const ColArgs& a, int batch, hipStream_t s) {
 void launch_reduce_pipe(const DetPlan& d, const PipeArgs& p, int batch, double* acc, hipStream_t s) {
   if (d.nslots == 0) return;
   hipLaunchKernelGGL(reduce_pipe_kernel, dim3(batch), dim3(64), 0, s, d, p, acc);
+  SK_HIP(hipGetLastError());
+}
+
+void launch_sum_copies(const double* in, long long n, int copies, double* out, hipStream_t s) {
+  if (n <= 0 || copies <= 0) return;
+  hipLaunchKernelGGL(sum_copies_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in, n, copies, out);
   SK_HIP(hipGetLastError());
 }
 

@@ -136,7 +136,7 @@ struct LaneDesc {
   int count;        // positions of the current item (<= lane_waves)
   int next_pos;     // FINAL_INIT / INIT: first position of the lane's next item
   int next_count;   // its positions
-  int pad;
+  int acc_off;      // added to positions where the lane accumulates its detectors (its own accumulator copy)
 };
 
 struct PipeArgs {
@@ -264,6 +264,8 @@ void launch_row_pipe(const RowArgs& a, int batch, hipStream_t s);
 void launch_col_pipe(const ColArgs& a, int batch, hipStream_t s);
 void launch_reduce_pipe(const DetPlan& d, const PipeArgs& p, int batch, double* acc, hipStream_t s);
 void launch_step_increment(int* step, hipStream_t s);
+// out[e] += sum over c = 0 .. copies-1, in that order, of in[c * n + e]
+void launch_sum_copies(const double* in, long long n, int copies, double* out, hipStream_t s);
 int pipe_positions_per_block();
 // positions per launch the scan takes for `count` positions through p when the caller names no batch (sk_api.hip)
 int default_scan_batch(const sk_potential& p, int count);

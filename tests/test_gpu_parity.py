@@ -272,3 +272,28 @@ def test_scan_is_deterministic_and_batch_independent():
         g = sk.STEM.scan(probe, [prepared], scan, dets, sk.ExecutionOptions(batch=batch, use_graph=True))
         for x, y in zip(a, g):
             assert np.array_equal(x.values, y.values), batch
+
+
+def test_lane_pipeline_single_position_over_many_potentials():
+    """A CBED-style scan: one probe position over five potentials of different slice counts. The lanes hold the same
+    position for different potentials at once, each accumulating into its own copy of the detector sums, which are
+    added in lane order at the end; the per-potential waves are the same float32 numbers as one potential at a
+    time, so only the float64 order of the sum over potentials may differ."""
+    a = 3.905
+    cell = sk.AtomicStructure((a, a, a), [38, 22, 8, 8, 8],
+                              [[0, 0, 0], [a / 2, a / 2, a / 2], [a / 2, a / 2, 0], [a / 2, 0, a / 2], [0, a / 2, a / 2]])
+    pots = [sk.Potential(cell.repeated(3, 3, nz), 256, 256, slice_thickness=2.0) for nz in (1, 2, 3, 4, 6)]
+    probe = sk.Probe(sk.ElectronBeam(200e3), pots[0].grid, sk.Aperture(20.0))
+    positions = np.array([[a / 3, a / 4]])
+    dets = [sk.AnnularDetector(50, 150), sk.CenterOfMassDetector("cutoff", "mrad"), sk.PixelatedDetector("valid")]
+    prepared = [p.prepare(probe.beam) for p in pots]
+    seq = sk.STEM.scan(probe, prepared, positions, dets, sk.ExecutionOptions(pipeline=0))
+    for opt in (sk.ExecutionOptions(pipeline=1), sk.ExecutionOptions(pipeline=-1),
+                sk.ExecutionOptions(pipeline=1, use_graph=True)):
+        lanes = sk.STEM.scan(probe, prepared, positions, dets, opt)
+        for x, y in zip(seq, lanes):
+            assert rel_max(y.values, x.values) <= 1e-12
+    # the same twice in a row: the accumulator copies start from zero on every scan
+    again = sk.STEM.scan(probe, prepared, positions, dets, sk.ExecutionOptions(pipeline=1))
+    for x, y in zip(lanes, again):
+        assert np.array_equal(x.values, y.values)
