@@ -1643,8 +1643,9 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
     // The library's choice: lanes when there are several potentials and one potential's positions fill at most an
     // eighth of a launch. Lanes holding the same positions for different potentials each accumulate into their own
     // copy, so a launch carries up to a cache-sized batch of (position, potential) pairs. Measured at 512^2 over 16
-    // configurations of 43-115 slices: 16 positions ran 1.46 x faster in lanes than one potential at a time, while
-    // 64 positions ran 1.21 x slower, since the lanes' column pass is slower per position than the balanced one.
+    // configurations of 43-115 slices: 16 positions ran 1.45 x faster in lanes than one potential at a time, while
+    // 64 positions ran 1.17 x slower: the lanes' per-step bookkeeping and their skipped units cost more than the
+    // fuller launches save once one potential's positions fill a third of a launch.
     if (pipe < 0) pipe = (npots > 1 && 8 * npos <= default_batch(p0, 1 << 30, opt)) ? 1 : 0;
     int W = 0, L = 0;
     if (pipe) {

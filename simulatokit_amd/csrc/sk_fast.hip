@@ -417,12 +417,21 @@ __global__ void __launch_bounds__(kColTile * (N / 8)) col_fast_kernel(ColArgs a,
 // flight meanwhile. Three block barriers per position. With TC = 8 two blocks share a CU, so one block's barriers
 // overlap the other's work.
 //
-// col512_run: work units [u0, u1), where unit u is position `b` of the tile of columns [TC tx, TC tx + TC) as
-// where(u, tx, b) gives them; consecutive units of one tile share its P, which is (re)loaded when the tile changes.
-// The twiddles are in LDS already, and every wavefront is past its last LDS access on entry and again on exit. The
-// unit's tile and position are recomputed where they are used rather than kept live across the transforms. ONE_TILE:
-// every unit is on one tile (its P is loaded once, before the loop).
-template <int FLAGS, int TC, bool ONE_TILE, class Where>
+// One work unit of the 512-point column pass: wave `b` of the batch on the tile of columns [TC tx, TC tx + TC).
+struct ColUnit {
+  int tx, b;
+  bool on;            // false: nothing to do for this unit (uniform over the block)
+  const float2* P;    // the wave's propagator [nx][nc]
+  long long pos;      // detectors: where the wave accumulates
+  int ramp;           // generated probes: the row of rx and ry
+};
+
+// col512_run: work units [u0, u1) as where(u) describes them; consecutive units of one tile and propagator share
+// its P, which is (re)loaded when either changes. The twiddles are in LDS already, and every wavefront is past its
+// last LDS access on entry and again on exit. A unit's description is recomputed where it is used rather than kept
+// live across the transforms. ONE_TILE: every unit is on one tile with one P (loaded once, before the loop). SKIPS:
+// some units may be off (otherwise every unit is taken to be on, and the next unit is always prefetched).
+template <int FLAGS, int TC, bool ONE_TILE, bool SKIPS, class Where>
 __device__ __forceinline__ void col512_run(const ColArgs& a, int u0, int u1, Where where, float2* tile, float2* ptile,
                                            const LdsTwiddles<512>& w, double* red) {
   constexpr int N = 512, T = 64, NT = TC * 64;
@@ -441,19 +450,18 @@ __device__ __forceinline__ void col512_run(const ColArgs& a, int u0, int u1, Whe
   };
   // the block-phase load of unit u: column cb of its tile, rows xi + 64 r
   auto gload = [&](int u, float2 (&g)[8]) {
-    int tx, b;
-    where(u, tx, b);
-    const int c = tx * TC + cb, iy = a.keep[c];
+    const ColUnit U = where(u);
+    const int c = U.tx * TC + cb, iy = a.keep[c];
     if constexpr ((FLAGS & COL_IN_G) != 0) {
-      const float2* t0 = a.G + b * gw + tx * TC;
+      const float2* t0 = a.G + U.b * gw + U.tx * TC;
 #pragma unroll
       for (int r = 0; r < 8; ++r) g[r] = iy >= 0 ? *elem(t0, r) : make_float2(0.f, 0.f);
     } else {
 #pragma unroll
       for (int r = 0; r < 8; ++r) {
         const int x = xi + T * r;
-        g[r] = iy >= 0 ? cmul(cmul(a.spec[(long long)x * a.ny + iy], a.rx[(long long)b * a.nx + x]),
-                              a.ry[(long long)b * a.ny + iy])
+        g[r] = iy >= 0 ? cmul(cmul(a.spec[(long long)x * a.ny + iy], a.rx[(long long)U.ramp * a.nx + x]),
+                              a.ry[(long long)U.ramp * a.ny + iy])
                        : make_float2(0.f, 0.f);
       }
     }
@@ -461,34 +469,50 @@ __device__ __forceinline__ void col512_run(const ColArgs& a, int u0, int u1, Whe
   // P is zero on the rows x = 3T .. 5T-1 (r = 3, 4 of every lane): there |kx| >= 0.375 / sx exceeds the antialias
   // cutoff 0.3333333 / max(sx, sy), and the propagator carries that aperture. Those rows are neither loaded nor
   // multiplied, and the inverse transform starts from zeros there.
-  auto pload = [&](int tx) {
+  auto pload = [&](int tx, const float2* P) {
 #pragma unroll
     for (int r = 0; r < 8; ++r)
-      if (r != 3 && r != 4) ptile[tile_slot(xi + T * r, cb)] = *elem(a.P + tx * TC, r);
+      if (r != 3 && r != 4) ptile[tile_slot(xi + T * r, cb)] = *elem(P + tx * TC, r);
   };
   float2 g[8];
+  int gu = -1;  // the unit whose block-phase values g holds
   int tcur = -1;
+  const float2* pcur = nullptr;
   if (u0 < u1) {
-    gload(u0, g);
+    const ColUnit U = where(u0);
+    if (!SKIPS || U.on) {
+      gload(u0, g);
+      gu = u0;
+    }
     if constexpr (HAS_P && ONE_TILE) {
-      int b;
-      where(u0, tcur, b);
-      pload(tcur);
+      tcur = U.tx;
+      pload(tcur, U.P);
     }
   }
   for (int u = u0; u < u1; ++u) {
+    if constexpr (SKIPS) {
+      if (!where(u).on) continue;  // uniform over the block
+    }
     if constexpr (HAS_P && !ONE_TILE) {
-      int tx, b;
-      where(u, tx, b);
-      if (tx != tcur) {  // uniform over the block; the previous unit's LDS reads ended before its last barrier
-        tcur = tx;
-        pload(tx);
+      const ColUnit U = where(u);
+      // uniform; the previous unit's LDS reads ended before its last barrier. P changes between units only where
+      // units belong to different lanes (SKIPS); elsewhere it is the launch's one propagator.
+      if (U.tx != tcur || (SKIPS && U.P != pcur)) {
+        tcur = U.tx;
+        pcur = U.P;
+        pload(U.tx, U.P);
       }
+    }
+    if constexpr (SKIPS) {
+      if (gu != u) gload(u, g);
     }
 #pragma unroll
     for (int r = 0; r < 8; ++r) tile[tile_slot(xi + T * r, cb)] = g[r];
     __syncthreads();
-    if (u + 1 < u1) gload(u + 1, g);  // in flight during the transforms
+    if (u + 1 < u1 && (!SKIPS || where(u + 1).on)) {  // in flight during the transforms
+      gload(u + 1, g);
+      gu = u + 1;
+    }
     float2 v[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) v[r] = tile[tile_slot(lane + T * r, wave)];
@@ -499,10 +523,9 @@ __device__ __forceinline__ void col512_run(const ColArgs& a, int u0, int u1, Whe
         v[r] = (r == 3 || r == 4) ? make_float2(0.f, 0.f) : cmul(v[r], ptile[tile_slot(lane + T * r, wave)]);
     }
     if constexpr ((FLAGS & COL_DETECT) != 0) {
-      int tx, b;
-      where(u, tx, b);
-      detect_block<N, NT>(v, a.det, first_position(a.pos0, a.pos_dev) + b, b, a.keep[tx * TC + wave], lane, tid,
-                          reinterpret_cast<float2*>(red), tx);
+      const ColUnit U = where(u);
+      detect_block<N, NT>(v, a.det, U.pos, U.b, a.keep[U.tx * TC + wave], lane, tid, reinterpret_cast<float2*>(red),
+                          U.tx);
       continue;  // detect_block ends with a barrier: the tile is free again
     }
     if constexpr ((FLAGS & COL_INV) != 0) fft_reg<N, +1, true, HAS_P>(v, tile, lane, w, widx);
@@ -510,10 +533,9 @@ __device__ __forceinline__ void col512_run(const ColArgs& a, int u0, int u1, Whe
     for (int r = 0; r < 8; ++r) tile[tile_slot(lane + T * r, wave)] = v[r];
     __syncthreads();
     {
-      int tx, b;
-      where(u, tx, b);
-      const float2* t0 = a.G + b * gw + tx * TC;
-      if (a.keep[tx * TC + cb] >= 0) {  // padding columns stay as the row pass leaves them: never read or written
+      const ColUnit U = where(u);
+      const float2* t0 = a.G + U.b * gw + U.tx * TC;
+      if (a.keep[U.tx * TC + cb] >= 0) {  // padding columns stay as the row pass leaves them: never read or written
 #pragma unroll
         for (int r = 0; r < 8; ++r) *const_cast<float2*>(elem(t0, r)) = tile[tile_slot(xi + T * r, cb)];
       }
@@ -536,7 +558,10 @@ __global__ void __launch_bounds__(TC * 64) col512_kernel(ColArgs a, const float2
   const int b1 = b0 + PB < batch ? b0 + PB : batch;
   LdsTwiddles<N>::build(twt, tw, threadIdx.x, NT);
   const LdsTwiddles<N> w{twt, twt + LdsTwiddles<N>::R2 * LdsTwiddles<N>::L2, (int)(threadIdx.x & 63)};
-  col512_run<FLAGS, TC, true>(a, b0, b1, [bx](int u, int& tx, int& b) { tx = bx; b = u; }, tile, ptile, w, red);
+  const long long first = first_position(a.pos0, a.pos_dev);
+  col512_run<FLAGS, TC, true, false>(
+      a, b0, b1, [&a, bx, first](int u) { return ColUnit{bx, u, true, a.P, first + u, u}; }, tile,
+      ptile, w, red);
 }
 
 // Balanced 1-D grid of 8-column tiles, sized by the host to one round of resident blocks. The work is the list of
@@ -561,12 +586,52 @@ __global__ void __launch_bounds__(8 * 64, (FLAGS & COL_DETECT) != 0 ? 1 : 4)
   const int u0 = (int)(U * r / ranges), u1 = (int)(U * (r + 1) / ranges);
   LdsTwiddles<N>::build(twt, tw, threadIdx.x, NT);
   const LdsTwiddles<N> w{twt, twt + LdsTwiddles<N>::R2 * LdsTwiddles<N>::L2, (int)(threadIdx.x & 63)};
-  col512_run<FLAGS, TC, false>(
+  const long long first = first_position(a.pos0, a.pos_dev);
+  col512_run<FLAGS, TC, false, false>(
       a, u0, u1,
-      [batch, h](int u, int& tx, int& b) {
-        const int m = u / batch;
-        tx = 2 * m + h;
-        b = u - m * batch;
+      [&a, batch, h, first](int u) {
+        const int m = u / batch, b = u - m * batch;
+        return ColUnit{2 * m + h, b, true, a.P, first + b, b};
+      },
+      tile, ptile, w, red);
+}
+
+// Lane pipeline, 512-point columns: the balanced spans of col512_span_kernel over the pipeline's whole batch, each
+// unit taking the descriptor of its lane for the current step. FLAGS selects the lanes served: COL_MID the lanes in
+// a mid slice, COL_FINAL those finishing an item (FINAL, FINAL_INIT), COL_INIT those starting one (INIT, FINAL_INIT:
+// launched after COL_FINAL, into the same storage). Units of other lanes, and waves past a lane's count, are skipped.
+template <int FLAGS>
+__global__ void __launch_bounds__(8 * 64, (FLAGS & COL_DETECT) != 0 ? 1 : 4)
+    col512_lanes_kernel(ColArgs a, const float2* __restrict__ tw, int batch, int ranges) {
+  constexpr int N = 512, TC = 8, NT = TC * 64;
+  constexpr bool HAS_P = (FLAGS & COL_MUL_P) != 0;
+  __shared__ float2 tile[N * TC];
+  __shared__ float2 ptile[HAS_P ? N * TC : 1];
+  __shared__ float2 twt[LdsTwiddles<N>::kSize];
+  __shared__ double red[(NT / 64) * kMaxSlots];
+  const int G = gridDim.x;  // 2 * ranges
+  int j = blockIdx.x;
+  if (G % 8 == 0) j = (j % 8) * (G / 8) + j / 8;
+  const int h = j & 1, r = j >> 1;
+  const long long U = (long long)(a.nc / (2 * TC)) * batch;  // below 2^31: the host checks
+  const int u0 = (int)(U * r / ranges), u1 = (int)(U * (r + 1) / ranges);
+  LdsTwiddles<N>::build(twt, tw, threadIdx.x, NT);
+  const LdsTwiddles<N> w{twt, twt + LdsTwiddles<N>::R2 * LdsTwiddles<N>::L2, (int)(threadIdx.x & 63)};
+  const LaneDesc* descs = a.pipe.descs + (long long)(*a.pipe.step) * a.pipe.lanes;
+  const int W = a.pipe.lane_waves;
+  col512_run<FLAGS, TC, false, true>(
+      a, u0, u1,
+      [descs, batch, h, W](int u) {
+        const int m = u / batch, b = u - m * batch, ln = b / W, local = b - ln * W;
+        const LaneDesc& d = descs[ln];
+        ColUnit c{2 * m + h, b, false, d.P, (long long)d.pos + d.acc_off + local, d.next_pos + local};
+        if constexpr ((FLAGS & COL_DETECT) != 0)
+          c.on = (d.mode == LANE_FINAL || d.mode == LANE_FINAL_INIT) && local < d.count;
+        else if constexpr ((FLAGS & COL_GENERATE) != 0)
+          c.on = (d.mode == LANE_INIT || d.mode == LANE_FINAL_INIT) && local < d.next_count;
+        else
+          c.on = d.mode == LANE_MID && local < d.count;
+        return c;
       },
       tile, ptile, w, red);
 }
@@ -701,14 +766,17 @@ void row_launch(const RowArgs& a, int batch, hipStream_t s, const float2* tw) {
 // Spans of the balanced 8-column pass of mode F over `units` (tile pair, position) units: one round of resident
 // blocks, two per span, and a multiple of 8 spans where there are enough units, so that the XCD-major renumbering
 // keeps both blocks of every span on one XCD.
-template <int F>
+template <int F, bool LANES = false>
 int span_count(long long units) {
   static int resident = 0;  // blocks of this kernel the device holds at once
   if (resident == 0) {
     int dev = 0, cus = 0, occ = 0;
     SK_HIP(hipGetDevice(&dev));
     SK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    SK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, col512_span_kernel<F>, 8 * 64, 0));
+    if constexpr (LANES)
+      SK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, col512_lanes_kernel<F>, 8 * 64, 0));
+    else
+      SK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, col512_span_kernel<F>, 8 * 64, 0));
     resident = std::max(2, cus * std::max(1, occ));
   }
   long long r = std::min<long long>(units, resident / 2);
@@ -813,12 +881,24 @@ void launch_row_pipe(const RowArgs& a, int batch, hipStream_t s) {
 
 void launch_col_pipe(const ColArgs& a, int batch, hipStream_t s) {
   const float2* tw = line_plan(a.nx, s).dev.roots;
-  // 512: 8-column blocks of 512 threads, whose 256-VGPR budget holds the fused detectors and probe generation
-  if (a.nx == 512)
-    hipLaunchKernelGGL((col_pipe_kernel<512, 8>), dim3(a.nc / 8, batch / kPipePB), dim3(8 * 64), 0, s, a, tw);
-  else
+  if (a.nx == 512) {
+    // balanced spans, one launch per mode: mid slices, items finishing (detectors), items starting (probes); the
+    // probes of FINAL_INIT lanes overwrite the waves their detectors have just read
+    const long long units = (long long)(a.nc / 16) * batch;
+    if (units >= (1LL << 31)) fail(SK_ERR_INVALID, "column pass: too many positions in one launch");
+    int ranges = span_count<COL_MID, true>(units);
+    hipLaunchKernelGGL((col512_lanes_kernel<COL_MID>), dim3(2 * ranges), dim3(8 * 64), 0, s, a, tw, batch, ranges);
+    ranges = span_count<COL_FINAL, true>(units);
+    hipLaunchKernelGGL((col512_lanes_kernel<COL_FINAL>), dim3(2 * ranges), dim3(8 * 64), 0, s, a, tw, batch, ranges);
+    ColArgs g = a;  // generated probes: the ramps of every position, indexed by position
+    g.rx = a.pipe.rx_all;
+    g.ry = a.pipe.ry_all;
+    ranges = span_count<COL_INIT, true>(units);
+    hipLaunchKernelGGL((col512_lanes_kernel<COL_INIT>), dim3(2 * ranges), dim3(8 * 64), 0, s, g, tw, batch, ranges);
+  } else {
     hipLaunchKernelGGL((col_pipe_kernel<256, kColTile>), dim3(a.nc / kColTile, batch / kPipePB), dim3(kColTile * 32),
                        0, s, a, tw);
+  }
   SK_HIP(hipGetLastError());
 }
 
