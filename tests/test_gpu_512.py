@@ -104,3 +104,24 @@ def test_scan_512_anisotropic_sampling_equals_the_oracle(tiles):
     for m, wv in zip(got, want):
         wv = wv.reshape(-1)
         assert np.abs(m.values - wv).max() <= ACC.detector_bound(np.abs(wv).max())
+
+
+@pytest.mark.parametrize("npos, nz", [(200, (1, 2, 3)), (24, (1, 2, 3, 1, 2, 3, 2, 1))],
+                         ids=["lanes_own_chunks", "lanes_share_chunks"])
+def test_lanes_at_512_equal_one_potential_at_a_time(npos, nz):
+    """The lanes' 512-point column launches (mid slices, items finishing, items starting) over several potentials of
+    uneven slice counts, where lanes run several items in a row, so a lane's next probe is generated into the
+    storage its detectors have just read. 200 positions: lanes of 32 own their chunks. 24 positions over 8
+    potentials: 24 items on 21 lanes of 8 that share chunks, each with its own accumulators. Each (position,
+    potential) wave is the same float32 computation as one potential at a time, so only the float64 order of the
+    sum over potentials may differ."""
+    pots = [sk.Potential(srtio3(4, 4, k), 512, 512, slice_thickness=2.0) for k in nz]
+    probe = sk.Probe(sk.ElectronBeam(200e3), pots[0].grid, sk.Aperture(20.0))
+    rng = np.random.default_rng(7)
+    positions = rng.uniform(0.0, 4 * SRTIO3_A, (npos, 2))
+    dets = [sk.AnnularDetector(50, 200), sk.CenterOfMassDetector("cutoff", "mrad"), sk.PixelatedDetector("cutoff")]
+    prepared = [p.prepare(probe.beam) for p in pots]
+    one = sk.STEM.scan(probe, prepared, positions, dets, sk.ExecutionOptions(pipeline=0))
+    lanes = sk.STEM.scan(probe, prepared, positions, dets, sk.ExecutionOptions(pipeline=1))
+    for x, y in zip(one, lanes):
+        assert rel_max(y.values, x.values) <= 1e-12
