@@ -54,6 +54,7 @@ struct Context {
   int status = SK_ERR_UNAVAILABLE;
   std::string why;
   hipStream_t stream = nullptr;
+  bool big_llc = false;   // the device has a 256 MiB last-level cache (MI300 / MI355 class: gfx94x, gfx950)
   DevBuf<double> lobato;  // 103 x 10
   // timing
   bool timing = false;
@@ -103,6 +104,11 @@ Context& require() {
         c.why = std::string("HIP initialisation failed: ") + hipGetErrorString(e);
       } else {
         c.status = SK_OK;
+        hipDeviceProp_t prop{};
+        if (hipGetDeviceProperties(&prop, 0) == hipSuccess) {
+          const std::string arch = prop.gcnArchName;
+          c.big_llc = arch.rfind("gfx94", 0) == 0 || arch.rfind("gfx950", 0) == 0;
+        }
         std::vector<double> tab(103 * 10);
         for (int z = 0; z < 103; ++z)
           for (int k = 0; k < 5; ++k) {
@@ -500,8 +506,9 @@ LineFFT slice_plan(int n, hipStream_t s) {
 
 // Positions per launch. Every pass streams the batch's compact waves (nx x nc complex64 each) once in and once out,
 // so a batch whose waves stay in the device's last-level cache (kCachedWaveBytes) runs from that cache rather than
-// from HBM: at 512 x 512, 176 positions run 1.17 x faster per position than 1024, and 184 fall back to HBM rates. The default takes the largest cache-resident batch when it holds at least
-// kMinCachedBatch positions (enough to fill the GPU), else a 4 GiB budget of at most 1024 positions, and then
+// from HBM: at 512 x 512, 176 positions run 1.17 x faster per position than 1024, and 184 fall back to HBM rates.
+// On a device with such a cache, the default takes the largest cache-resident batch when it holds at least
+// kMinCachedBatch positions (enough to fill the GPU); otherwise a 4 GiB budget of at most 1024 positions. It then
 // spreads `count` evenly over the batches, so no small remainder batch runs at a fraction of the GPU.
 constexpr int kMinCachedBatch = 64;
 // device memory the lane pipeline may take for per-lane detector accumulators when lanes share positions
@@ -510,7 +517,7 @@ constexpr size_t kLaneCopyBytes = size_t(2) << 30;
 int default_batch(const sk_potential& p, int count, const sk_options* opt) {
   int b = opt && opt->batch > 0 ? opt->batch : 0;
   if (b > 0) return std::max(1, std::min(b, count));
-  const size_t cached = kCachedWaveBytes / ((size_t)p.grid.nx * p.nc * sizeof(float2));
+  const size_t cached = ctx().big_llc ? kCachedWaveBytes / ((size_t)p.grid.nx * p.nc * sizeof(float2)) : 0;
   if (cached >= (size_t)kMinCachedBatch)
     b = (int)std::min<size_t>(1024, cached);
   else

@@ -835,7 +835,7 @@ int fast_col_tile(int nx, int requested, double wave_bytes) {
   if (requested == 8 || requested == 16) return requested;
   // 8-column spans when the batch stays in the last-level cache (they fill every CU for any batch); 16-column
   // blocks when it streams from HBM, where their full 128-byte row segments measured faster
-  return wave_bytes <= (double)kCachedWaveBytes ? 8 : 16;
+  return wave_bytes <= (double)kCachedWaveBytes ? 8 : 16;  // the 8-column spans also fill every CU at any batch
 }
 
 int fast_col_blocks_per_wave(int nx, int nc, int pipeline) {
