@@ -110,19 +110,21 @@ __device__ __forceinline__ void exchange_sync() {
 // group of 16 and of 32 lanes.
 __device__ __forceinline__ int row_slot(int j) { return j ^ ((j >> 3) & 31); }
 
-// LDS slot of (x, column) in a 512 x TC column tile: conflict-free both for the block's coalesced phases (lanes that
-// share an x take consecutive columns) and for each column's own exchanges (the row_slot patterns in x), in groups of
-// 16 and of 32 lanes.
+// LDS slot of (x, column) in a 512 x TC column tile. Found by search among shift-XOR swizzles to be free of bank
+// conflicts, in groups of 16 and of 32 lanes, for every access of the column pass: the block phases' rows
+// (lanes that share an x take consecutive columns, x = xi + 64 r), the block phases' first-stage writes
+// (x = 8 xi + j), and each column wavefront's exchanges (reads x = l + 64 j, writes 8 l + j and
+// 64 (l / 8) + l mod 8 + 8 j).
 template <int TC>
 __device__ __forceinline__ int tile_slot(int x, int c) {
   if constexpr (TC == 16) {
-    const int h = ((x >> 4) ^ (x >> 7)) & 1;
-    const int g = (x ^ (x >> 3)) & 15;
+    const int h = ((x >> 3) ^ (x >> 7)) & 1;
+    const int g = (x ^ (x >> 2) ^ (x >> 3)) & 15;
     return (x ^ h) * 16 + (c ^ g);
   } else {
     static_assert(TC == 8, "column tiles of 8 or 16");
-    const int h = ((x >> 1) ^ (x >> 6)) & 3;
-    const int g = (x ^ (x >> 3)) & 7;
+    const int h = ((x >> 3) ^ (x >> 6)) & 3;
+    const int g = (x ^ (x >> 4)) & 7;
     return (x ^ h) * 8 + (c ^ g);
   }
 }
@@ -430,8 +432,10 @@ struct ColUnit {
 // its P, which is (re)loaded when either changes. The twiddles are in LDS already, and every wavefront is past its
 // last LDS access on entry and again on exit. A unit's description is recomputed where it is used rather than kept
 // live across the transforms. ONE_TILE: every unit is on one tile with one P (loaded once, before the loop). SKIPS:
-// some units may be off (otherwise every unit is taken to be on, and the next unit is always prefetched).
-template <int FLAGS, int TC, bool ONE_TILE, bool SKIPS, class Where>
+// some units may be off (otherwise every unit is taken to be on, and the next unit is always prefetched). FUSE: the
+// block phases run the first and last stages of the transforms (below); otherwise they only move the tile between
+// HBM and LDS, and each column wavefront runs whole transforms (fewer registers, for 1024-thread blocks).
+template <int FLAGS, int TC, bool ONE_TILE, bool SKIPS, bool FUSE, class Where>
 __device__ __forceinline__ void col512_run(const ColArgs& a, int u0, int u1, Where where, float2* tile, float2* ptile,
                                            const LdsTwiddles<512>& w, double* red) {
   constexpr int N = 512, T = 64, NT = TC * 64;
@@ -506,17 +510,36 @@ __device__ __forceinline__ void col512_run(const ColArgs& a, int u0, int u1, Whe
     if constexpr (SKIPS) {
       if (gu != u) gload(u, g);
     }
+    // The first stage of the line's transform combines the eight elements i + 64 r of a column, which a block-phase
+    // thread holds for i = xi, column cb: the block phases run it from their loaded values and write its outputs to
+    // the tile (then a block barrier); the column wavefronts run the middle stages with wavefront-local exchanges;
+    // the block phases run the last stage of the inverse from the tile into the values they store.
+    auto none = [](int, int) { return make_float2(1.f, 0.f); };
+    auto f2 = [&w](int q, int j) { return w.s2(q, j); };
+    auto f3 = [&w](int q, int j) { return w.s3(q, j); };
+    auto bidx = [cb, tile_slot](int j) { return tile_slot(j, cb); };
+    if constexpr (!FUSE) {
 #pragma unroll
-    for (int r = 0; r < 8; ++r) tile[tile_slot(xi + T * r, cb)] = g[r];
-    __syncthreads();
+      for (int r = 0; r < 8; ++r) tile[tile_slot(xi + T * r, cb)] = g[r];
+      __syncthreads();
+    } else if constexpr ((FLAGS & COL_FWD) != 0) {
+      stage<N, 8, 1, -1, true, false, false>(g, tile, xi, none, bidx);
+    } else {
+      stage<N, 8, 1, +1, true, false, false>(g, tile, xi, none, bidx);
+    }
     if (u + 1 < u1 && (!SKIPS || where(u + 1).on)) {  // in flight during the transforms
       gload(u + 1, g);
       gu = u + 1;
     }
     float2 v[8];
+    if constexpr (!FUSE) {
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = tile[tile_slot(lane + T * r, wave)];
-    if constexpr ((FLAGS & COL_FWD) != 0) fft_reg<N, -1, true>(v, tile, lane, w, widx);
+      for (int r = 0; r < 8; ++r) v[r] = tile[tile_slot(lane + T * r, wave)];
+      if constexpr ((FLAGS & COL_FWD) != 0) fft_reg<N, -1, true>(v, tile, lane, w, widx);
+    } else if constexpr ((FLAGS & COL_FWD) != 0) {
+      stage<N, 8, 8, -1, false, false, true>(v, tile, lane, f2, widx);
+      stage<N, 8, 64, -1, false, true, true>(v, tile, lane, f3, widx);
+    }
     if constexpr (HAS_P) {
 #pragma unroll
       for (int r = 0; r < 8; ++r)
@@ -528,19 +551,37 @@ __device__ __forceinline__ void col512_run(const ColArgs& a, int u0, int u1, Whe
                           U.tx);
       continue;  // detect_block ends with a barrier: the tile is free again
     }
-    if constexpr ((FLAGS & COL_INV) != 0) fft_reg<N, +1, true, HAS_P>(v, tile, lane, w, widx);
+    if constexpr (!FUSE) {
+      if constexpr ((FLAGS & COL_INV) != 0) fft_reg<N, +1, true, HAS_P>(v, tile, lane, w, widx);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) tile[tile_slot(lane + T * r, wave)] = v[r];
-    __syncthreads();
-    {
+      for (int r = 0; r < 8; ++r) tile[tile_slot(lane + T * r, wave)] = v[r];
+      __syncthreads();
       const ColUnit U = where(u);
       const float2* t0 = a.G + U.b * gw + U.tx * TC;
       if (a.keep[U.tx * TC + cb] >= 0) {  // padding columns stay as the row pass leaves them: never read or written
 #pragma unroll
         for (int r = 0; r < 8; ++r) *const_cast<float2*>(elem(t0, r)) = tile[tile_slot(xi + T * r, cb)];
       }
+      __syncthreads();
+      continue;
     }
+    // the inverse: its first stage from the registers (after a forward transform), its second in the wavefront
+    if constexpr ((FLAGS & COL_FWD) != 0) stage<N, 8, 1, +1, true, false, true, HAS_P>(v, tile, lane, none, widx);
+    stage<N, 8, 8, +1, false, false, true>(v, tile, lane, f2, widx);
     __syncthreads();
+    {
+      // last stage in the block phases; its barrier between the reads and the next unit's writes frees the tile
+      const LdsTwiddles<N> wb{w.t2, w.t3, xi};
+      auto f3b = [&wb](int q, int j) { return wb.s3(q, j); };
+      float2 o[8];
+      stage<N, 8, 64, +1, false, true, false>(o, tile, xi, f3b, bidx);
+      const ColUnit U = where(u);
+      const float2* t0 = a.G + U.b * gw + U.tx * TC;
+      if (a.keep[U.tx * TC + cb] >= 0) {  // padding columns stay as the row pass leaves them: never read or written
+#pragma unroll
+        for (int r = 0; r < 8; ++r) *const_cast<float2*>(elem(t0, r)) = o[r];
+      }
+    }
   }
 }
 
@@ -559,7 +600,7 @@ __global__ void __launch_bounds__(TC * 64) col512_kernel(ColArgs a, const float2
   LdsTwiddles<N>::build(twt, tw, threadIdx.x, NT);
   const LdsTwiddles<N> w{twt, twt + LdsTwiddles<N>::R2 * LdsTwiddles<N>::L2, (int)(threadIdx.x & 63)};
   const long long first = first_position(a.pos0, a.pos_dev);
-  col512_run<FLAGS, TC, true, false>(
+  col512_run<FLAGS, TC, true, false, false>(
       a, b0, b1, [&a, bx, first](int u) { return ColUnit{bx, u, true, a.P, first + u, u}; }, tile,
       ptile, w, red);
 }
@@ -587,7 +628,7 @@ __global__ void __launch_bounds__(8 * 64, (FLAGS & COL_DETECT) != 0 ? 1 : 4)
   LdsTwiddles<N>::build(twt, tw, threadIdx.x, NT);
   const LdsTwiddles<N> w{twt, twt + LdsTwiddles<N>::R2 * LdsTwiddles<N>::L2, (int)(threadIdx.x & 63)};
   const long long first = first_position(a.pos0, a.pos_dev);
-  col512_run<FLAGS, TC, false, false>(
+  col512_run<FLAGS, TC, false, false, true>(
       a, u0, u1,
       [&a, batch, h, first](int u) {
         const int m = u / batch, b = u - m * batch;
@@ -619,7 +660,7 @@ __global__ void __launch_bounds__(8 * 64, (FLAGS & COL_DETECT) != 0 ? 1 : 4)
   const LdsTwiddles<N> w{twt, twt + LdsTwiddles<N>::R2 * LdsTwiddles<N>::L2, (int)(threadIdx.x & 63)};
   const LaneDesc* descs = a.pipe.descs + (long long)(*a.pipe.step) * a.pipe.lanes;
   const int W = a.pipe.lane_waves;
-  col512_run<FLAGS, TC, false, true>(
+  col512_run<FLAGS, TC, false, true, true>(
       a, u0, u1,
       [descs, batch, h, W](int u) {
         const int m = u / batch, b = u - m * batch, ln = b / W, local = b - ln * W;
