@@ -743,6 +743,18 @@ void run_pipeline(const sk_potential* const* pots, int npots, int npos, int W, i
     }
   }
   w.descs.upload(descs, s);
+  // per descriptor row: the column launches it needs (1 mid slices, 2 items finishing, 4 items starting) and whether
+  // any lane runs a slice (the row pass)
+  std::vector<int> row_modes(nsteps + 1, 0);
+  std::vector<char> row_busy(nsteps + 1, 0);
+  for (int st = 0; st <= nsteps; ++st)
+    for (int l = 0; l < L; ++l) {
+      const LaneDesc& d = descs[(size_t)st * L + l];
+      if (d.t) row_busy[st] = 1;
+      if (d.mode == LANE_MID) row_modes[st] |= 1;
+      if (d.mode == LANE_FINAL || d.mode == LANE_FINAL_INIT) row_modes[st] |= 2;
+      if (d.mode == LANE_INIT || d.mode == LANE_FINAL_INIT) row_modes[st] |= 4;
+    }
   w.step.ensure(1);
   SK_HIP(hipMemsetAsync(w.step.p, 0, sizeof(int), s));
   w.rx_all.ensure((size_t)npos * nx);
@@ -768,7 +780,7 @@ void run_pipeline(const sk_potential* const* pots, int npots, int npos, int W, i
       dpl.pix_acc[q] = w.pix_copies[q].p;
     }
   }
-  PipeArgs pa{w.descs.p, w.step.p, L, W, w.rx_all.p, w.ry_all.p};
+  PipeArgs pa{w.descs.p, w.step.p, L, W, w.rx_all.p, w.ry_all.p, 0};
   RowArgs r = row_args(p0, w.G.p);
   r.flags = ROW_IN_COMPACT | ROW_INV | ROW_MUL_T | ROW_FWD | ROW_OUT_COMPACT;
   r.pipe = pa;
@@ -793,18 +805,30 @@ void run_pipeline(const sk_potential* const* pots, int npots, int npos, int W, i
     for (int q = 0; q < dp.npix; ++q)
       launch_sum_copies(w.pix_copies[q].p, (long long)npos * dp.pix_stride[q], L, dp.pix_acc[q], s);
   };
-  // prologue: the probe of every lane's first item (descriptor row 0)
-  launch_col_pipe(cc, B, s);
-  launch_step_increment(w.step.p, s);
   if (!graph) {
-    for (int st = 0; st < nsteps; ++st) {
-      step_fn(tm);
+    // the step goes to every launch by value, and each step launches only the kernels its lanes need
+    r.pipe.step = cc.pipe.step = pa.step = nullptr;
+    for (int st = 0; st <= nsteps; ++st) {  // row 0: the prologue, the probe of every lane's first item
+      r.pipe.step_now = cc.pipe.step_now = pa.step_now = st;
+      if (st > 0 && row_busy[st]) {
+        tm.mark_start(s, 0);
+        launch_row_pipe(r, B, s);
+        tm.mark_stop(s);
+      }
+      const bool timed = (row_modes[st] & 1) != 0;  // the column time is that of steps with mid slices
+      if (timed) tm.mark_start(s, 1);
+      launch_col_pipe(cc, B, s, row_modes[st]);
+      if (timed) tm.mark_stop(s);
+      if (row_modes[st] & 2) launch_reduce_pipe(dpl, pa, B, acc, s);
       if ((st & 31) == 31) tm.harvest(timing);
     }
     tm.harvest(timing);
     fold_copies();
     return;
   }
+  // prologue: the probe of every lane's first item (descriptor row 0)
+  launch_col_pipe(cc, B, s);
+  launch_step_increment(w.step.p, s);
   // hipGraph: K steps captured once; every replay reads its steps from the device counter
   const int K = std::min(nsteps, 32);
   Timer off;

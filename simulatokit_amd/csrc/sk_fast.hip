@@ -225,7 +225,7 @@ __global__ void __launch_bounds__(64) row_fast_kernel(RowArgs a, const float2* _
   long long limit = batch;
   if constexpr (PIPE) {
     const int lane = b0 / a.pipe.lane_waves;  // lane_waves is a multiple of PB: the block stays in one lane
-    const LaneDesc d = a.pipe.descs[(long long)(*a.pipe.step) * a.pipe.lanes + lane];
+    const LaneDesc d = a.pipe.descs[(long long)pipe_step(a.pipe) * a.pipe.lanes + lane];
     if (d.t == nullptr) return;  // idle lane (uniform over the block)
     tsrc = d.t;
     limit = (long long)lane * a.pipe.lane_waves + d.count;
@@ -658,7 +658,7 @@ __global__ void __launch_bounds__(8 * 64, (FLAGS & COL_DETECT) != 0 ? 1 : 4)
   const int u0 = (int)(U * r / ranges), u1 = (int)(U * (r + 1) / ranges);
   LdsTwiddles<N>::build(twt, tw, threadIdx.x, NT);
   const LdsTwiddles<N> w{twt, twt + LdsTwiddles<N>::R2 * LdsTwiddles<N>::L2, (int)(threadIdx.x & 63)};
-  const LaneDesc* descs = a.pipe.descs + (long long)(*a.pipe.step) * a.pipe.lanes;
+  const LaneDesc* descs = a.pipe.descs + (long long)pipe_step(a.pipe) * a.pipe.lanes;
   const int W = a.pipe.lane_waves;
   col512_run<FLAGS, TC, false, true, true>(
       a, u0, u1,
@@ -692,7 +692,7 @@ __global__ void __launch_bounds__(TC * (N / 8)) col_pipe_kernel(ColArgs a, const
   const int c = blockIdx.x * TC + cl;
   const int b0 = blockIdx.y * kPipePB;
   const int lane = b0 / a.pipe.lane_waves, local0 = b0 - lane * a.pipe.lane_waves;
-  const LaneDesc d = a.pipe.descs[(long long)(*a.pipe.step) * a.pipe.lanes + lane];
+  const LaneDesc d = a.pipe.descs[(long long)pipe_step(a.pipe) * a.pipe.lanes + lane];
   if (d.mode == LANE_IDLE) return;  // uniform over the block
   const int iy = a.keep[c];
   auto idx = [cl](int j) { return j * TC + cl; };
@@ -744,7 +744,7 @@ __global__ void __launch_bounds__(TC * (N / 8)) col_pipe_kernel(ColArgs a, const
 __global__ void reduce_pipe_kernel(DetPlan d, PipeArgs pa, double* acc) {
   const long long b = blockIdx.x;
   const int lane = int(b / pa.lane_waves), local = int(b - (long long)lane * pa.lane_waves);
-  const LaneDesc ld = pa.descs[(long long)(*pa.step) * pa.lanes + lane];
+  const LaneDesc ld = pa.descs[(long long)pipe_step(pa) * pa.lanes + lane];
   if (!(ld.mode == LANE_FINAL || ld.mode == LANE_FINAL_INIT) || local >= ld.count) return;
   const long long pos = (long long)ld.pos + ld.acc_off + local;
   for (int q = threadIdx.x; q < d.nslots; q += blockDim.x) {
@@ -920,22 +920,30 @@ void launch_row_pipe(const RowArgs& a, int batch, hipStream_t s) {
   SK_HIP(hipGetLastError());
 }
 
-void launch_col_pipe(const ColArgs& a, int batch, hipStream_t s) {
+void launch_col_pipe(const ColArgs& a, int batch, hipStream_t s, int modes) {
+  if (modes == 0) return;
   const float2* tw = line_plan(a.nx, s).dev.roots;
   if (a.nx == 512) {
     // balanced spans, one launch per mode: mid slices, items finishing (detectors), items starting (probes); the
     // probes of FINAL_INIT lanes overwrite the waves their detectors have just read
     const long long units = (long long)(a.nc / 16) * batch;
     if (units >= (1LL << 31)) fail(SK_ERR_INVALID, "column pass: too many positions in one launch");
-    int ranges = span_count<COL_MID, true>(units);
-    hipLaunchKernelGGL((col512_lanes_kernel<COL_MID>), dim3(2 * ranges), dim3(8 * 64), 0, s, a, tw, batch, ranges);
-    ranges = span_count<COL_FINAL, true>(units);
-    hipLaunchKernelGGL((col512_lanes_kernel<COL_FINAL>), dim3(2 * ranges), dim3(8 * 64), 0, s, a, tw, batch, ranges);
-    ColArgs g = a;  // generated probes: the ramps of every position, indexed by position
-    g.rx = a.pipe.rx_all;
-    g.ry = a.pipe.ry_all;
-    ranges = span_count<COL_INIT, true>(units);
-    hipLaunchKernelGGL((col512_lanes_kernel<COL_INIT>), dim3(2 * ranges), dim3(8 * 64), 0, s, g, tw, batch, ranges);
+    if (modes & 1) {
+      const int ranges = span_count<COL_MID, true>(units);
+      hipLaunchKernelGGL((col512_lanes_kernel<COL_MID>), dim3(2 * ranges), dim3(8 * 64), 0, s, a, tw, batch, ranges);
+    }
+    if (modes & 2) {
+      const int ranges = span_count<COL_FINAL, true>(units);
+      hipLaunchKernelGGL((col512_lanes_kernel<COL_FINAL>), dim3(2 * ranges), dim3(8 * 64), 0, s, a, tw, batch,
+                         ranges);
+    }
+    if (modes & 4) {
+      ColArgs g = a;  // generated probes: the ramps of every position, indexed by position
+      g.rx = a.pipe.rx_all;
+      g.ry = a.pipe.ry_all;
+      const int ranges = span_count<COL_INIT, true>(units);
+      hipLaunchKernelGGL((col512_lanes_kernel<COL_INIT>), dim3(2 * ranges), dim3(8 * 64), 0, s, g, tw, batch, ranges);
+    }
   } else {
     hipLaunchKernelGGL((col_pipe_kernel<256, kColTile>), dim3(a.nc / kColTile, batch / kPipePB), dim3(kColTile * 32),
                        0, s, a, tw);

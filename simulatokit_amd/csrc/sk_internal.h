@@ -141,11 +141,15 @@ struct LaneDesc {
 
 struct PipeArgs {
   const LaneDesc* descs;  // [steps][lanes]; null: not a pipelined launch
-  const int* step;        // device step counter
+  const int* step;        // device step counter (replayed graphs); null: the step is step_now
   int lanes, lane_waves;
   const float2* rx_all;   // [npos][nx] shift ramps of every position
   const float2* ry_all;   // [npos][ny]
+  int step_now;           // the step of this launch when there is no device counter
 };
+
+// the descriptor row of a pipelined launch
+__device__ __forceinline__ int pipe_step(const PipeArgs& p) { return p.step ? *p.step : p.step_now; }
 
 // ---- pass descriptions --------------------------------------------------------------------------------------------
 // Row pass (along y, contiguous): flags
@@ -261,7 +265,8 @@ int fast_col_blocks_per_wave(int nx, int nc, int pipeline);  // blocks of the de
 // the lanes that finished an item, and the step counter's increment
 bool pipeline_supported(int nx, int ny, int nc, int lane_waves);
 void launch_row_pipe(const RowArgs& a, int batch, hipStream_t s);
-void launch_col_pipe(const ColArgs& a, int batch, hipStream_t s);
+// modes: a mask of the lanes served at 512 (1 mid slices, 2 items finishing, 4 items starting); 0 launches nothing
+void launch_col_pipe(const ColArgs& a, int batch, hipStream_t s, int modes = 7);
 void launch_reduce_pipe(const DetPlan& d, const PipeArgs& p, int batch, double* acc, hipStream_t s);
 void launch_step_increment(int* step, hipStream_t s);
 // out[e] += sum over c = 0 .. copies-1, in that order, of in[c * n + e]
