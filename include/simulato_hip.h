@@ -76,8 +76,8 @@ typedef struct sk_options {
   int fast_path;    /* 1 (default): register-resident FFT passes where the length has one; 0: generic LDS passes */
   int pipeline;     /* scan: 1 = lanes of positions advance one slice per launch, each through its own potential
                        and slice (potentials of different slice counts share every launch); 0 = one potential at a
-                       time; -1 = the library's choice (lanes when npots > 1 and the positions fill at most an
-                       eighth of a launch, else one potential at a time) */
+                       time; -1 = the library's choice (lanes when npots > 1 and the positions fill at most a
+                       quarter of a launch, else one potential at a time) */
   int lane_waves;   /* pipeline: positions per lane (0 = the library's choice) */
   int col_tile;     /* 512-point columns: columns per block of the column pass, 8 or 16 (0 = the library's choice) */
   int precision;    /* transforms of the slice loop: 0 = float32 (default); 1 = split-BF16 on the matrix cores

@@ -1671,13 +1671,12 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
     int B = default_batch(p0, npos, opt);
     // lane pipeline: lanes of W positions, each on its own potential and slice at every step
     int pipe = opt ? opt->pipeline : 0;
-    // The library's choice: lanes when there are several potentials and one potential's positions fill at most an
-    // eighth of a launch. Lanes holding the same positions for different potentials each accumulate into their own
+    // The library's choice: lanes when there are several potentials and one potential's positions fill at most a
+    // quarter of a launch. Lanes holding the same positions for different potentials each accumulate into their own
     // copy, so a launch carries up to a cache-sized batch of (position, potential) pairs. Measured at 512^2 over 16
-    // configurations of 43-115 slices: 16 positions ran 1.45 x faster in lanes than one potential at a time, while
-    // 64 positions ran 1.17 x slower: the lanes' per-step bookkeeping and their skipped units cost more than the
-    // fuller launches save once one potential's positions fill a third of a launch.
-    if (pipe < 0) pipe = (npots > 1 && 8 * npos <= default_batch(p0, 1 << 30, opt)) ? 1 : 0;
+    // configurations of 43-115 slices, lanes against one potential at a time: 1 position 3.44 x, 8 positions 2.50 x,
+    // 16 positions 1.62 x, 32 positions 1.04 x, 48 positions 1.01 x, 64 positions 0.92 x, 144 positions 0.92 x.
+    if (pipe < 0) pipe = (npots > 1 && 4 * npos <= default_batch(p0, 1 << 30, opt)) ? 1 : 0;
     int W = 0, L = 0;
     if (pipe) {
       const int q = pipe_positions_per_block();
