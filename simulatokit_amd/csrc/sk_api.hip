@@ -12,6 +12,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "sk_internal.h"
@@ -350,14 +351,26 @@ const int* identity_map(int n) {
   return p;
 }
 
+// A pass launch; its ev_start / ev_stop (when set) time it: the fast passes attach them to the kernel's dispatch, the
+// generic passes record them on the stream around their launches.
 void row_pass(const RowArgs& a, int batch, bool fast, hipStream_t s) {
-  if (fast && fast_row_supported(a.nx, a.ny, a.flags)) launch_row_fast(a, batch, s);
-  else launch_row_generic(a, batch, s);
+  if (fast && fast_row_supported(a.nx, a.ny, a.flags)) {
+    launch_row_fast(a, batch, s);
+    return;
+  }
+  if (a.ev_start) SK_HIP(hipEventRecord(a.ev_start, s));
+  launch_row_generic(a, batch, s);
+  if (a.ev_stop) SK_HIP(hipEventRecord(a.ev_stop, s));
 }
 
 void col_pass(const ColArgs& a, int batch, bool fast, hipStream_t s) {
-  if (fast && fast_col_supported(a.nx, a.nc, a.flags)) launch_col_fast(a, batch, s);
-  else launch_col_generic(a, batch, s);
+  if (fast && fast_col_supported(a.nx, a.nc, a.flags)) {
+    launch_col_fast(a, batch, s);
+    return;
+  }
+  if (a.ev_start) SK_HIP(hipEventRecord(a.ev_start, s));
+  launch_col_generic(a, batch, s);
+  if (a.ev_stop) SK_HIP(hipEventRecord(a.ev_stop, s));
 }
 
 // Full 2-D transform of `batch` waves in place (buf: batch x nx x ny): forward unscaled (sign -1) or inverse with
@@ -587,6 +600,21 @@ struct Timer {
     SK_HIP(hipEventRecord(ev[used + 1], s));
     used += 2;
   }
+  // the events that time the next launch of kind k, recorded by that launch itself (null, null when off)
+  std::pair<hipEvent_t, hipEvent_t> take(int k) {
+    if (!on) return {nullptr, nullptr};
+    if (used + 2 > ev.size()) {
+      for (int i = 0; i < 64; ++i) {
+        hipEvent_t e;
+        SK_HIP(hipEventCreate(&e));
+        ev.push_back(e);
+      }
+      kind.resize(ev.size() / 2 + 1);
+    }
+    kind[used / 2] = k;
+    used += 2;
+    return {ev[used - 2], ev[used - 1]};
+  }
   void harvest(sk_timing& t) {
     if (!on || !used) return;
     SK_HIP(hipEventSynchronize(ev[used - 1]));
@@ -630,9 +658,9 @@ void slice_loop(const sk_potential& p, int batch, float2* G, bool entry_full, fl
         r.plane = planes + (size_t)plane_out[j - 1] * batch * N;
       }
     }
-    tm.mark_start(s, 0);
+    std::tie(r.ev_start, r.ev_stop) = tm.take(0);
     row_pass(r, batch, fast, s);
-    tm.mark_stop(s);
+    r.ev_start = r.ev_stop = nullptr;
     // column pass of slice j
     c.P = p.props.p + (size_t)p.prop_of_slice[j] * p.grid.nx * p.nc;
     if (j == p.nslices - 1 && final_col) {
@@ -641,9 +669,9 @@ void slice_loop(const sk_potential& p, int batch, float2* G, bool entry_full, fl
       col_pass(f, batch, fast, s);
     } else {
       c.flags = COL_IN_G | COL_FWD | COL_MUL_P | COL_INV | COL_OUT;
-      tm.mark_start(s, 1);
+      std::tie(c.ev_start, c.ev_stop) = tm.take(1);
       col_pass(c, batch, fast, s);
-      tm.mark_stop(s);
+      c.ev_start = c.ev_stop = nullptr;
     }
     r.flags = 0;
     r.plane = nullptr;

@@ -18,6 +18,7 @@
 // share an x read 128 contiguous bytes. Twiddles come from the line's float64-derived root table.
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include "sk_dft.h"
 #include "sk_internal.h"
@@ -770,6 +771,14 @@ __global__ void step_increment_kernel(int* step) {
 
 bool is_fast_length(int n) { return n == 256 || n == 512; }
 
+// A launch timed by the events of its own dispatch when the caller asks for timing (e0 not null): the events then
+// bracket the kernel alone, not the stream around it.
+template <class K, class... A>
+void launch_timed(K kernel, dim3 grid, dim3 block, hipStream_t s, hipEvent_t e0, hipEvent_t e1, A... args) {
+  if (e0) hipExtLaunchKernelGGL(kernel, grid, block, 0, s, e0, e1, 0, args...);
+  else hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
+}
+
 constexpr int kRowPB = 8;  // positions per row-pass block
 constexpr int kColPB = 8;  // positions per column-pass block (P and the twiddles loaded once per 8 positions)
 
@@ -785,11 +794,12 @@ void row_go(const RowArgs& a, int batch, hipStream_t s, const float2* tw) {
   dim3 grid(a.nx / ROWS, (batch + kRowPB - 1) / kRowPB);
   if constexpr (F == ROW_MID) {
     if (band_shape(a, N)) {
-      hipLaunchKernelGGL((row_fast_kernel<N, F, kRowPB, false, true>), grid, dim3(64), 0, s, a, tw, batch);
+      launch_timed(row_fast_kernel<N, F, kRowPB, false, true>, grid, dim3(64), s, a.ev_start, a.ev_stop, a, tw,
+                   batch);
       return;
     }
   }
-  hipLaunchKernelGGL((row_fast_kernel<N, F, kRowPB>), grid, dim3(64), 0, s, a, tw, batch);
+  launch_timed(row_fast_kernel<N, F, kRowPB>, grid, dim3(64), s, a.ev_start, a.ev_stop, a, tw, batch);
 }
 
 template <int N>
@@ -836,13 +846,15 @@ void col_go(const ColArgs& a, int batch, hipStream_t s, const float2* tw) {
       const long long units = (long long)(a.nc / 16) * batch;
       if (units >= (1LL << 31)) fail(SK_ERR_INVALID, "column pass: too many positions in one launch");
       const int ranges = span_count<F>(units);
-      hipLaunchKernelGGL((col512_span_kernel<F>), dim3(2 * ranges), dim3(8 * 64), 0, s, a, tw, batch, ranges);
+      launch_timed(col512_span_kernel<F>, dim3(2 * ranges), dim3(8 * 64), s, a.ev_start, a.ev_stop, a, tw, batch,
+                   ranges);
     } else if constexpr (F != COL_FINAL) {
-      hipLaunchKernelGGL((col512_kernel<F, kColPB, 16>), dim3(a.nc / 16, by), dim3(16 * 64), 0, s, a, tw, batch);
+      launch_timed(col512_kernel<F, kColPB, 16>, dim3(a.nc / 16, by), dim3(16 * 64), s, a.ev_start, a.ev_stop, a, tw,
+                   batch);
     }
   } else {
-    hipLaunchKernelGGL((col_fast_kernel<N, F, kColPB>), dim3(a.nc / kColTile, by), dim3(kColTile * (N / 8)), 0, s,
-                       a, tw, batch);
+    launch_timed(col_fast_kernel<N, F, kColPB>, dim3(a.nc / kColTile, by), dim3(kColTile * (N / 8)), s, a.ev_start,
+                 a.ev_stop, a, tw, batch);
   }
 }
 

@@ -179,6 +179,7 @@ struct RowArgs {
   // the kept set as two ranges: iy in [0, kept_hi] -> c = iy; iy in [kept_lo, ny) -> c = iy - kept_shift
   int kept_hi, kept_lo, kept_shift;
   PipeArgs pipe;          // lane pipeline (fast passes only); pipe.descs null otherwise
+  hipEvent_t ev_start, ev_stop;  // host side: events that time this launch (null: untimed)
 };
 
 enum ColFlags : int {
@@ -237,6 +238,7 @@ struct ColArgs {
   int col_tile;           // 512-point fast column pass: columns per block, 8 or 16 (0 = the default)
   DetPlan det;
   PipeArgs pipe;          // lane pipeline (fast passes only); pipe.descs null otherwise
+  hipEvent_t ev_start, ev_stop;  // host side: events that time this launch (null: untimed)
 };
 
 __device__ __forceinline__ long long first_position(int pos0, const int* pos_dev) {
