@@ -612,7 +612,7 @@ __global__ void __launch_bounds__(TC * 64) col512_kernel(ColArgs a, const float2
 // 128-byte lines. Renumbered XCD-major (work-groups are dealt round-robin over the 8 XCDs), the two blocks of a span
 // run side by side on one XCD and share its L2. A span that crosses a pair boundary reloads P for its next tile.
 template <int FLAGS>
-__global__ void __launch_bounds__(8 * 64, (FLAGS & COL_DETECT) != 0 ? 1 : 4)
+__global__ void __launch_bounds__(8 * 64, 4)
     col512_span_kernel(ColArgs a, const float2* __restrict__ tw, int batch, int ranges) {
   constexpr int N = 512, TC = 8, NT = TC * 64;
   constexpr bool HAS_P = (FLAGS & COL_MUL_P) != 0;
