@@ -7,15 +7,17 @@
 // multiply in the row pass, the propagator multiply in the column pass) and the detector reduction of the final
 // pass happen on the registers between the transforms, so every pass reads and writes each stored value once.
 //
-//   row pass:    one wavefront per block and row x; IFFT_y -> * t(x, y) -> FFT_y for PB positions in turn
-//   column pass: 16 compact columns per block;      FFT_x -> * P(kx, ky) -> IFFT_x | detectors, PB positions
+//   row pass:    one wavefront per block and row x; IFFT_y -> * t(x, y) -> FFT_y for 8 positions in turn, with the
+//                zero band of the 2/3 bandlimit pruned (row_fast_kernel)
+//   column pass: at 512, blocks of 8 compact columns, one column per wavefront, running balanced spans of
+//                (tile, position) units; the block phases move 64-byte row segments between HBM and an LDS tile and
+//                run the first forward and last inverse stage, the wavefronts the middle stages and * P(kx, ky) or
+//                the detectors (col512_span_kernel; col512_lanes_kernel for the lane pipeline; col512_kernel with
+//                16-column blocks for batches that stream from HBM). At 256, 16 columns per block (col_fast_kernel).
 //
-// A block keeps its spatial tile fixed and walks PB positions of the batch, so the lane's twiddle factors and its
-// share of t (row pass) or P (column pass) are loaded into registers once and reused PB times.
-//
-// LDS layouts are chosen so that each group of 16 lanes touches 16 distinct 8-byte bank pairs: a row keeps one pad
-// slot per eight values (index j + j/8); a column tile is stored transposed, [x][16 columns], so the 16 lanes that
-// share an x read 128 contiguous bytes. Twiddles come from the line's float64-derived root table.
+// A block keeps the lane's twiddle factors (registers or LDS) and its share of t (row pass) or P (column pass) for
+// all the positions it runs. LDS layouts are XOR swizzles, each found by search to be free of bank conflicts for
+// every access pattern of its kernel. Twiddles come from the line's float64-derived root table.
 
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
@@ -680,7 +682,8 @@ __global__ void __launch_bounds__(8 * 64, (FLAGS & COL_DETECT) != 0 ? 1 : 4)
 
 constexpr int kPipePB = 8;  // positions per block in pipelined launches; lane_waves is a multiple of it
 
-// Pipelined column pass: the lane of the block's positions gives the mode of this step —
+// Pipelined column pass of the 256-point grids (512 uses col512_lanes_kernel): the lane of the block's positions gives
+// the mode of this step —
 //   MID:        FFT_x, * P, IFFT_x                       (a slice inside the item)
 //   FINAL:      FFT_x, * P, detectors                    (the item's last slice; the lane then idles)
 //   FINAL_INIT: FFT_x, * P, detectors, then the probe of the lane's next item generated into the same storage
