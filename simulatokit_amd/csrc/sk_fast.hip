@@ -16,8 +16,9 @@
 //                16-column blocks for batches that stream from HBM). At 256, 16 columns per block (col_fast_kernel).
 //
 // A block keeps the lane's twiddle factors (registers or LDS) and its share of t (row pass) or P (column pass) for
-// all the positions it runs. LDS layouts are XOR swizzles, each found by search to be free of bank conflicts for
-// every access pattern of its kernel. Twiddles come from the line's float64-derived root table.
+// all the positions it runs. The row lines and the 512-point column tiles use XOR-swizzled LDS layouts, each found by
+// search to be free of bank conflicts for every access pattern of its kernel; the 256-point column pass keeps its
+// tile as [x][16 columns]. Twiddles come from the line's float64-derived root table.
 
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
