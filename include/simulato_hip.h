@@ -81,7 +81,9 @@ typedef struct sk_options {
   int lane_waves;   /* pipeline: positions per lane (0 = the library's choice) */
   int col_tile;     /* 512-point columns: columns per block of the column pass, 8 or 16 (0 = the library's choice) */
   int precision;    /* transforms of the slice loop: 0 = float32 (default); 1 = split-BF16 on the matrix cores
-                       (bfloat16 hi/lo pairs, three products, float32 accumulation; grid lengths 256 and 512) */
+                       (bfloat16 hi/lo pairs, three products, float32 accumulation; grid lengths 256,
+                       384, 512 and 1024, and the Bluestein lengths 65-512 that have a prime factor
+                       above 13) */
   int reserved[1];
 } sk_options;
 

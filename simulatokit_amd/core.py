@@ -213,7 +213,7 @@ class ExecutionOptions:
                              # potentials, else one potential at a time)
     lane_waves: int = 0      # positions per lane (0 = the library's choice)
     col_tile: int = 0        # 512-point columns: columns per column-pass block, 8 or 16 (0 = the library's choice)
-    precision: str = "float32"  # the slice loop's transforms: "float32", or "split_bf16" (matrix cores; 256, 512)
+    precision: str = "float32"  # the slice loop's transforms: "float32", or "split_bf16" (matrix cores)
 
 
 @dataclass(frozen=True)

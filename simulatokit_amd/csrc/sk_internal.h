@@ -103,11 +103,12 @@ struct LineFFT {
   const float2* roots;  // m entries, e^{-2 pi i k / m}
   const float2* chirp;  // n entries, e^{-pi i k^2 / n} (Bluestein)
   const float2* khat;   // m entries, FFT_m of the conjugated chirp, times 1/m (Bluestein)
-  // split-BF16 MFMA line transform (n = 256 or 512 only; sk_mfma.h): 0 = the float32 Stockham stages above
+  // split-BF16 MFMA line transform of length m (m = 256, 384, 512 or 1024; sk_mfma.h): 0 = the float32 Stockham
+  // stages above
   int use_mfma;
   const void* mtab16;   // DFT-16 block matrices in MFMA B-operand order, bfloat16 hi/lo
-  const void* mtabn2;   // DFT-(n/16) block matrices, same order
-  const float2* mtw;    // e^{-2 pi i j2 k1 / n}, [k1 16][j2 n/16]
+  const void* mtabn2;   // DFT-(m/16) block matrices, same order
+  const float2* mtw;    // e^{-2 pi i j2 k1 / m}, [k1 16][j2 m/16]
 };
 
 // Host-owned tables of one length.
@@ -118,7 +119,8 @@ struct LinePlan {
 
 // Returns the shared plan of length n (built once per process and device).
 const LinePlan& line_plan(int n, hipStream_t s);
-// The plan of length n with the split-BF16 MFMA line transform (n = 256 or 512), or fails.
+// The plan of length n with the split-BF16 MFMA line transform, or fails: n = 256, 384, 512 or 1024, or a length
+// done by Bluestein's algorithm whose power-of-two length m is 256, 512 or 1024 (65 <= n <= 512).
 const LineFFT& mfma_line_plan(int n, hipStream_t s);
 bool mfma_supported(int n);
 

@@ -109,18 +109,25 @@ __device__ float2* lds_stockham(float2* a, float2* b, int lines, int ls, const L
   return a;
 }
 
+// Unscaled transform of length p.m of every line: MF = 0, the float32 Stockham stages; MF = n2 > 0, the split-BF16
+// matrix-core stages of length p.m = 16 n2 (the plan carries their tables). Returns the buffer holding the result.
+template <int MF>
+__device__ float2* lds_length_m(float2* a, float2* b, int lines, int ls, const LineFFT& p, int sign, int tid, int T) {
+  if constexpr (MF == 0) {
+    return lds_stockham(a, b, lines, ls, p, sign, tid, T);
+  } else {
+    return mfma_lines<MF>(a, b, lines, ls, static_cast<const bf16x8*>(p.mtab16), static_cast<const bf16x8*>(p.mtabn2),
+                          p.mtw, sign, tid);
+  }
+}
+
 // Unscaled transform of logical length p.n of every line (entries [0, n) of each line of stride ls).
 // Bluestein (1970): X_k = c_k sum_j (x_j c_j) conj(c_{k-j}), c_k = e^{sign pi i k^2 / n}, the convolution done with
 // two power-of-two transforms of length m.
+template <int MF>
 __device__ float2* lds_line_transform(float2* a, float2* b, int lines, int ls, const LineFFT& p, int sign, int tid,
                                       int T) {
-  if (p.use_mfma) {
-    const bf16x8* t16 = static_cast<const bf16x8*>(p.mtab16);
-    const bf16x8* tn2 = static_cast<const bf16x8*>(p.mtabn2);
-    return p.n == 256 ? mfma_lines<16>(a, b, lines, ls, t16, tn2, p.mtw, sign, tid)
-                      : mfma_lines<32>(a, b, lines, ls, t16, tn2, p.mtw, sign, tid);
-  }
-  if (!p.blue) return lds_stockham(a, b, lines, ls, p, sign, tid, T);
+  if (!p.blue) return lds_length_m<MF>(a, b, lines, ls, p, sign, tid, T);
   const int n = p.n, m = p.m;
   for (int idx = tid; idx < lines * m; idx += T) {
     const int l = idx / m, k = idx - l * m;
@@ -133,7 +140,7 @@ __device__ float2* lds_line_transform(float2* a, float2* b, int lines, int ls, c
     a[l * ls + k] = v;
   }
   __syncthreads();
-  float2* r = lds_stockham(a, b, lines, ls, p, -1, tid, T);
+  float2* r = lds_length_m<MF>(a, b, lines, ls, p, -1, tid, T);
   for (int idx = tid; idx < lines * m; idx += T) {
     const int l = idx / m, k = idx - l * m;
     float2 kh = p.khat[k];
@@ -142,7 +149,7 @@ __device__ float2* lds_line_transform(float2* a, float2* b, int lines, int ls, c
   }
   __syncthreads();
   float2* o = (r == a) ? b : a;
-  r = lds_stockham(r, o, lines, ls, p, +1, tid, T);
+  r = lds_length_m<MF>(r, o, lines, ls, p, +1, tid, T);
   for (int idx = tid; idx < lines * n; idx += T) {
     const int l = idx / n, k = idx - l * n;
     float2 c = p.chirp[k];
@@ -154,7 +161,8 @@ __device__ float2* lds_line_transform(float2* a, float2* b, int lines, int ls, c
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// Generic row pass: block = `rows` lines along y of one wave (blockIdx.y).
+// Generic row pass: block = `rows` lines along y of one wave (blockIdx.y). MF selects the line transform (0 float32
+// Stockham, else the split-BF16 stages of n2 = MF), one kernel each so the float32 kernel keeps its registers.
 // ---------------------------------------------------------------------------------------------------------------
 constexpr int kGenericThreads = 256;
 constexpr int kGenericLdsComplex = 2048;  // complex values per LDS buffer (16 KiB); two buffers
@@ -162,6 +170,7 @@ constexpr int kGenericLdsComplex = 2048;  // complex values per LDS buffer (16 K
 inline int row_lines(int m) { return m >= kGenericLdsComplex ? 1 : kGenericLdsComplex / m; }
 int col_generic_lines(int m) { return m >= kGenericLdsComplex ? 1 : kGenericLdsComplex / m; }
 
+template <int MF>
 __global__ void __launch_bounds__(kGenericThreads) row_generic_kernel(RowArgs a, int rows) {
   extern __shared__ float2 lds[];
   const int m = a.fy.m, ls = m;
@@ -190,7 +199,7 @@ __global__ void __launch_bounds__(kGenericThreads) row_generic_kernel(RowArgs a,
   float2* cur = A;
   float2* oth = B;
   if (f & ROW_INV) {
-    cur = lds_line_transform(cur, oth, rows, ls, a.fy, +1, tid, T);
+    cur = lds_line_transform<MF>(cur, oth, rows, ls, a.fy, +1, tid, T);
     oth = (cur == A) ? B : A;
   }
   if (f & (ROW_WRITE_PLANE | ROW_MUL_T)) {
@@ -203,7 +212,7 @@ __global__ void __launch_bounds__(kGenericThreads) row_generic_kernel(RowArgs a,
     }
     __syncthreads();
   }
-  if (f & ROW_FWD) cur = lds_line_transform(cur, oth, rows, ls, a.fy, -1, tid, T);
+  if (f & ROW_FWD) cur = lds_line_transform<MF>(cur, oth, rows, ls, a.fy, -1, tid, T);
   if (f & ROW_OUT_COMPACT) {
     for (int idx = tid; idx < rows * a.nc; idx += T) {
       const int l = idx / a.nc, c = idx - l * a.nc, x = x0 + l;
@@ -240,8 +249,9 @@ __device__ inline void block_reduce_slots(double (&s)[kMaxSlots], int nslots, do
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// Generic column pass: block = `cols` compact columns (lines along x) of one wave.
+// Generic column pass: block = `cols` compact columns (lines along x) of one wave; MF as in the row pass.
 // ---------------------------------------------------------------------------------------------------------------
+template <int MF>
 __global__ void __launch_bounds__(kGenericThreads) col_generic_kernel(ColArgs a, int cols) {
   extern __shared__ float2 lds[];
   const int m = a.fx.m, ls = m;
@@ -271,7 +281,7 @@ __global__ void __launch_bounds__(kGenericThreads) col_generic_kernel(ColArgs a,
   float2* cur = A;
   float2* oth = B;
   if (f & COL_FWD) {
-    cur = lds_line_transform(cur, oth, cols, ls, a.fx, -1, tid, T);
+    cur = lds_line_transform<MF>(cur, oth, cols, ls, a.fx, -1, tid, T);
     oth = (cur == A) ? B : A;
   }
   if (f & COL_MUL_P) {
@@ -302,7 +312,7 @@ __global__ void __launch_bounds__(kGenericThreads) col_generic_kernel(ColArgs a,
                        a.det.blocks_per_wave);
     return;
   }
-  if (f & COL_INV) cur = lds_line_transform(cur, oth, cols, ls, a.fx, +1, tid, T);
+  if (f & COL_INV) cur = lds_line_transform<MF>(cur, oth, cols, ls, a.fx, +1, tid, T);
   if (f & COL_OUT) {
     for (int idx = tid; idx < cols * a.nx; idx += T) {
       const int k = idx / cols, l = idx - k * cols, c = c0 + l;
@@ -326,11 +336,25 @@ __global__ void reduce_partials_kernel(DetPlan d, int pos0, const int* pos_dev, 
 // ---------------------------------------------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------------------------------------------
+// The kernel variant of a plan's line transform: 0 float32, else n2 = m / 16 of the split-BF16 stages.
+int transform_kind(const LineFFT& p) {
+  if (!p.use_mfma) return 0;
+  if (p.m != 256 && p.m != 384 && p.m != 512 && p.m != 1024)
+    fail(SK_ERR_INVALID, "split-BF16 plan of unsupported length " + std::to_string(p.m));
+  return p.m / 16;
+}
+
 void launch_row_generic(const RowArgs& a, int batch, hipStream_t s) {
   const int rows = row_lines(a.fy.m);
   dim3 grid((a.nx + rows - 1) / rows, batch);
   const size_t lds = 2 * sizeof(float2) * rows * a.fy.m;
-  hipLaunchKernelGGL(row_generic_kernel, grid, dim3(kGenericThreads), lds, s, a, rows);
+  switch (transform_kind(a.fy)) {
+    case 0: hipLaunchKernelGGL(row_generic_kernel<0>, grid, dim3(kGenericThreads), lds, s, a, rows); break;
+    case 16: hipLaunchKernelGGL(row_generic_kernel<16>, grid, dim3(kGenericThreads), lds, s, a, rows); break;
+    case 24: hipLaunchKernelGGL(row_generic_kernel<24>, grid, dim3(kGenericThreads), lds, s, a, rows); break;
+    case 32: hipLaunchKernelGGL(row_generic_kernel<32>, grid, dim3(kGenericThreads), lds, s, a, rows); break;
+    default: hipLaunchKernelGGL(row_generic_kernel<64>, grid, dim3(kGenericThreads), lds, s, a, rows); break;
+  }
   SK_HIP(hipGetLastError());
 }
 
@@ -339,7 +363,13 @@ void launch_col_generic(const ColArgs& a, int batch, hipStream_t s) {
   dim3 grid((a.nc + cols - 1) / cols, batch);
   size_t lds = 2 * sizeof(float2) * cols * a.fx.m;
   if (a.flags & COL_DETECT) lds = std::max(lds, sizeof(float2) * cols * a.fx.m + sizeof(double) * kGenericThreads);
-  hipLaunchKernelGGL(col_generic_kernel, grid, dim3(kGenericThreads), lds, s, a, cols);
+  switch (transform_kind(a.fx)) {
+    case 0: hipLaunchKernelGGL(col_generic_kernel<0>, grid, dim3(kGenericThreads), lds, s, a, cols); break;
+    case 16: hipLaunchKernelGGL(col_generic_kernel<16>, grid, dim3(kGenericThreads), lds, s, a, cols); break;
+    case 24: hipLaunchKernelGGL(col_generic_kernel<24>, grid, dim3(kGenericThreads), lds, s, a, cols); break;
+    case 32: hipLaunchKernelGGL(col_generic_kernel<32>, grid, dim3(kGenericThreads), lds, s, a, cols); break;
+    default: hipLaunchKernelGGL(col_generic_kernel<64>, grid, dim3(kGenericThreads), lds, s, a, cols); break;
+  }
   SK_HIP(hipGetLastError());
 }
 
@@ -397,6 +427,15 @@ std::vector<float2> to_f2(const std::vector<cd>& v) {
   return o;
 }
 
+// The length of the device transform of a line of n points: n when n factors into the Stockham radices, else the
+// Bluestein length, the smallest power of two (at least 16) holding the 2 n - 1 point convolution.
+int device_length(int n) {
+  if (n <= 1 || !smooth_radices(n).empty()) return n;
+  int m = 16;
+  while (m < 2 * n - 1) m <<= 1;
+  return m;
+}
+
 std::map<int, std::unique_ptr<LinePlan>>& plans() {
   static auto* p = new std::map<int, std::unique_ptr<LinePlan>>;  // process lifetime: never freed at exit
   return *p;
@@ -412,16 +451,9 @@ const LinePlan& line_plan(int n, hipStream_t s) {
   auto lp = std::make_unique<LinePlan>();
   LineFFT& d = lp->dev;
   d.n = n;
-  auto r = smooth_radices(n);
-  int m = n;
-  if (r.empty() && n > 1) {
-    d.blue = 1;
-    m = 16;
-    while (m < 2 * n - 1) m <<= 1;
-    r = smooth_radices(m);
-  } else {
-    d.blue = 0;
-  }
+  const int m = device_length(n);
+  d.blue = m != n;
+  auto r = smooth_radices(m);
   if (m > 8192) fail(SK_ERR_INVALID, "grid length " + std::to_string(n) + " exceeds the device FFT's LDS tile");
   if ((int)r.size() > kMaxStages) fail(SK_ERR_INVALID, "too many FFT stages");
   d.m = m;
@@ -457,7 +489,10 @@ const LinePlan& line_plan(int n, hipStream_t s) {
   return ref;
 }
 
-bool mfma_supported(int n) { return n == 256 || n == 512; }
+bool mfma_supported(int n) {
+  const int m = device_length(n);
+  return m == 256 || m == 384 || m == 512 || m == 1024;
+}
 
 namespace {
 // float -> bfloat16 bits, round to nearest even (finite inputs)
@@ -474,9 +509,10 @@ float bf16_value(uint16_t b) {
   return f;
 }
 // The real block matrix of the length-L DFT in MFMA B-operand order [sign][k-step][tile][hi/lo][lane][8]:
-// row k = 16 s + 8 (lane / 32) + j (input j1 = k / 2, re/im k % 2), column 32 t + lane % 32 (output k1, re/im).
+// row k = 16 s + 8 (lane / 32) + j (input j1 = k / 2, re/im k % 2), column 32 t + lane % 32 (output k1, re/im);
+// the columns past 2 L (the padding of the last tile when L is not a multiple of 16) are zero.
 std::vector<uint16_t> dft_block_table(int L) {
-  const int ksteps = L / 8, ntiles = L / 16;
+  const int ksteps = L / 8, ntiles = (L + 15) / 16;
   std::vector<uint16_t> out((size_t)2 * ksteps * ntiles * 2 * 64 * 8);
   for (int si = 0; si < 2; ++si)
     for (int s = 0; s < ksteps; ++s)
@@ -487,7 +523,7 @@ std::vector<uint16_t> dft_block_table(int L) {
             const int j1 = k >> 1, p = k & 1, k1 = nn >> 1, q = nn & 1;
             const double ang = (si ? 2.0 : -2.0) * M_PI * double((long long)j1 * k1 % L) / double(L);
             const double wr = std::cos(ang), wi = std::sin(ang);
-            const double R = p == 0 ? (q == 0 ? wr : wi) : (q == 0 ? -wi : wr);
+            const double R = k1 >= L ? 0.0 : p == 0 ? (q == 0 ? wr : wi) : (q == 0 ? -wi : wr);
             const uint16_t hi = bf16_bits((float)R);
             const uint16_t lo = bf16_bits((float)(R - (double)bf16_value(hi)));
             const size_t base = ((((size_t)si * ksteps + s) * ntiles + t) * 2) * 64 * 8;
@@ -507,15 +543,17 @@ const LineFFT& mfma_line_plan(int n, hipStream_t s) {
   static std::map<int, std::unique_ptr<MfmaPlan>>* plans = new std::map<int, std::unique_ptr<MfmaPlan>>;
   auto it = plans->find(n);
   if (it != plans->end()) return it->second->dev;
-  if (!mfma_supported(n)) fail(SK_ERR_INVALID, "the split-BF16 MFMA transforms support grid lengths 256 and 512");
+  if (!mfma_supported(n))
+    fail(SK_ERR_INVALID, "the split-BF16 MFMA transforms support grid lengths 256, 384, 512 and 1024, and lengths "
+                         "done by Bluestein's algorithm from 65 to 512 (grid length " + std::to_string(n) + ")");
   auto mp = std::make_unique<MfmaPlan>();
   mp->dev = line_plan(n, s).dev;
-  const int n2 = n / 16;
+  const int m = mp->dev.m, n2 = m / 16;  // the matrix-core stages transform the device length m (Bluestein's, or n)
   mp->t16.upload(dft_block_table(16), s);
   mp->tn2.upload(dft_block_table(n2), s);
   std::vector<cd> tw((size_t)16 * n2);
   for (int k1 = 0; k1 < 16; ++k1)
-    for (int j2 = 0; j2 < n2; ++j2) tw[(size_t)k1 * n2 + j2] = root((long long)k1 * j2 % n, n);
+    for (int j2 = 0; j2 < n2; ++j2) tw[(size_t)k1 * n2 + j2] = root((long long)k1 * j2 % m, m);
   mp->tw.upload(to_f2(tw), s);
   mp->dev.use_mfma = 1;
   mp->dev.mtab16 = mp->t16.p;
