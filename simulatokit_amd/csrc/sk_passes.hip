@@ -26,6 +26,31 @@
 namespace sk {
 
 // ---------------------------------------------------------------------------------------------------------------
+// Quotient and remainder by a divisor the compiler does not know: n / d and n mod d for 0 <= n < 65536 and
+// 0 < d <= 16384, from the float reciprocal of d. The truncated product n * (1 / d) lies within one of the quotient
+// (relative error below 2^-22), and one correction step either way makes it exact; an exhaustive check over that
+// range, also with 1 / d off by one ulp either way, finds no error. About 6 VALU instructions, against some 30 for
+// a 32-bit integer division.
+// ---------------------------------------------------------------------------------------------------------------
+struct Div {
+  int d;
+  float inv;
+  __device__ explicit Div(int dd) : d(dd), inv(1.0f / float(dd)) {}
+  __device__ __forceinline__ int quot(int n, int& r) const {
+    int q = int(float(n) * inv);
+    r = n - q * d;
+    if (r < 0) {
+      --q;
+      r += d;
+    } else if (r >= d) {
+      ++q;
+      r -= d;
+    }
+    return q;
+  }
+};
+
+// ---------------------------------------------------------------------------------------------------------------
 // Small DFTs (sign -1 forward, +1 inverse, unscaled)
 // ---------------------------------------------------------------------------------------------------------------
 template <int R>
@@ -52,6 +77,23 @@ __device__ inline void dft_small(float2 (&u)[R], int sign, const float2* roots, 
   }
 }
 
+constexpr int kGenericThreads = 256;
+constexpr int kGenericLdsComplex = 2048;  // unpadded complex values per LDS buffer (16 KiB); two buffers
+
+// LDS layout of the float32 line transforms: entry q = line * ls + k of the unpadded lines lives at slot
+// q + (q / 16), one pad slot per 16 entries. The Stockham stages read and write at power-of-two strides (the first
+// stage of radix 8 writes at stride 8, the column loads step by the line stride); the pad spreads those strides
+// over the LDS banks (a bank model of the 384-point stages: 2.3 -> 1.3 times the conflict-free time). The
+// split-BF16 stages keep their lines unpadded.
+__device__ __forceinline__ int pad16(int q) { return q + (q >> 4); }
+template <int MF>
+__device__ __forceinline__ int at(int q) {
+  if constexpr (MF == 0) return pad16(q);
+  else return q;
+}
+// entries of one LDS buffer holding `q` unpadded entries
+__host__ __device__ inline int lds_entries(int q, bool padded) { return padded ? q + (q >> 4) : q; }
+
 // One Stockham pass of radix R over `lines` lines of length m (line stride ls): for butterfly i in [0, m/R),
 // k = i mod L, u_j = x[i + j m/R] * w_{L R}^{j k}, DFT_R(u), y[(i - k) R + k + j L] = u_j.
 template <int R>
@@ -60,27 +102,27 @@ __device__ void stockham_stage(const float2* __restrict__ src, float2* __restric
   const int stride = m / R;
   const int step = m / (L * R);
   const int total = lines * stride;
+  const Div dstride(stride), dL(L);
   for (int w = tid; w < total; w += T) {
-    const int line = w / stride;
-    const int i = w - line * stride;
-    const float2* x = src + line * ls;
-    float2* y = dst + line * ls;
-    const int k = i % L;
+    int i, k;
+    const int line = dstride.quot(w, i);
+    const int lb = line * ls;
+    (void)dL.quot(i, k);
     float2 u[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-      float2 v = x[i + j * stride];
+      float2 v = src[pad16(lb + i + j * stride)];
       if (j > 0 && k > 0) {
-        float2 tw = roots[(j * k * step) % m];
+        float2 tw = roots[j * k * step];  // j k step < R L step = m
         if (sign > 0) tw.y = -tw.y;
         v = cmul(v, tw);
       }
       u[j] = v;
     }
     dft_small<R>(u, sign, roots, m);
-    const int base = (i - k) * R + k;
+    const int base = lb + (i - k) * R + k;
 #pragma unroll
-    for (int j = 0; j < R; ++j) y[base + j * L] = u[j];
+    for (int j = 0; j < R; ++j) dst[pad16(base + j * L)] = u[j];
   }
 }
 
@@ -109,11 +151,74 @@ __device__ float2* lds_stockham(float2* a, float2* b, int lines, int ls, const L
   return a;
 }
 
-// Unscaled transform of length p.m of every line: MF = 0, the float32 Stockham stages; MF = n2 > 0, the split-BF16
-// matrix-core stages of length p.m = 16 n2 (the plan carries their tables). Returns the buffer holding the result.
-template <int MF>
+// The same pass with the radix, sub-length, length and line count fixed at compile time (line stride M), so every
+// index folds to shifts and constant multiplies.
+template <int R, int L, int M, int LINES>
+__device__ __forceinline__ void stockham_stage_fixed(const float2* __restrict__ src, float2* __restrict__ dst,
+                                                     const float2* __restrict__ roots, int sign, int tid) {
+  constexpr int stride = M / R, step = M / (L * R), total = LINES * stride;
+  for (int w = tid; w < total; w += kGenericThreads) {
+    const int line = w / stride, i = w - line * stride;
+    const int lb = line * M;
+    const int k = i % L;
+    float2 u[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      float2 v = src[pad16(lb + i + j * stride)];
+      if constexpr (L > 1) {
+        if (j > 0) {
+          float2 tw = roots[j * k * step];  // roots[0] = 1 where k = 0
+          if (sign > 0) tw.y = -tw.y;
+          v = cmul(v, tw);
+        }
+      }
+      u[j] = v;
+    }
+    dft_small<R>(u, sign, roots, M);
+    const int base = lb + (i - k) * R + k;
+#pragma unroll
+    for (int j = 0; j < R; ++j) dst[pad16(base + j * L)] = u[j];
+  }
+}
+
+// Compile-time stage lists of the fixed lengths: 384 = 8 8 2 3 (a grid length of the precision study) and 1024 =
+// 8 8 8 2 (Bluestein's length for grid lengths 257..512, 509 among them, and the grid length 1024). The plan's
+// radices for these lengths are the same.
+template <int M>
+__device__ float2* lds_stockham_fixed(float2* a, float2* b, const float2* roots, int sign, int tid) {
+  constexpr int LINES = kGenericLdsComplex / M;
+  if constexpr (M == 384) {
+    stockham_stage_fixed<8, 1, M, LINES>(a, b, roots, sign, tid);
+    __syncthreads();
+    stockham_stage_fixed<8, 8, M, LINES>(b, a, roots, sign, tid);
+    __syncthreads();
+    stockham_stage_fixed<2, 64, M, LINES>(a, b, roots, sign, tid);
+    __syncthreads();
+    stockham_stage_fixed<3, 128, M, LINES>(b, a, roots, sign, tid);
+    __syncthreads();
+    return a;
+  } else {
+    static_assert(M == 1024, "fixed lengths 384 and 1024");
+    stockham_stage_fixed<8, 1, M, LINES>(a, b, roots, sign, tid);
+    __syncthreads();
+    stockham_stage_fixed<8, 8, M, LINES>(b, a, roots, sign, tid);
+    __syncthreads();
+    stockham_stage_fixed<8, 64, M, LINES>(a, b, roots, sign, tid);
+    __syncthreads();
+    stockham_stage_fixed<2, 512, M, LINES>(b, a, roots, sign, tid);
+    __syncthreads();
+    return a;
+  }
+}
+
+// Unscaled transform of length p.m of every line: MF = 0, the float32 Stockham stages (FM > 0: of the length FM
+// fixed at compile time, lines = kGenericLdsComplex / FM); MF = n2 > 0, the split-BF16 matrix-core stages of length
+// p.m = 16 n2 (the plan carries their tables). Returns the buffer holding the result.
+template <int MF, int FM>
 __device__ float2* lds_length_m(float2* a, float2* b, int lines, int ls, const LineFFT& p, int sign, int tid, int T) {
-  if constexpr (MF == 0) {
+  if constexpr (MF == 0 && FM > 0) {
+    return lds_stockham_fixed<FM>(a, b, p.roots, sign, tid);
+  } else if constexpr (MF == 0) {
     return lds_stockham(a, b, lines, ls, p, sign, tid, T);
   } else {
     return mfma_lines<MF>(a, b, lines, ls, static_cast<const bf16x8*>(p.mtab16), static_cast<const bf16x8*>(p.mtabn2),
@@ -124,66 +229,70 @@ __device__ float2* lds_length_m(float2* a, float2* b, int lines, int ls, const L
 // Unscaled transform of logical length p.n of every line (entries [0, n) of each line of stride ls).
 // Bluestein (1970): X_k = c_k sum_j (x_j c_j) conj(c_{k-j}), c_k = e^{sign pi i k^2 / n}, the convolution done with
 // two power-of-two transforms of length m.
-template <int MF>
+template <int MF, int FM>
 __device__ float2* lds_line_transform(float2* a, float2* b, int lines, int ls, const LineFFT& p, int sign, int tid,
                                       int T) {
-  if (!p.blue) return lds_length_m<MF>(a, b, lines, ls, p, sign, tid, T);
+  if (!p.blue) return lds_length_m<MF, FM>(a, b, lines, ls, p, sign, tid, T);
   const int n = p.n, m = p.m;
+  const Div dm(m), dn(n);
   for (int idx = tid; idx < lines * m; idx += T) {
-    const int l = idx / m, k = idx - l * m;
+    int k;
+    const int l = dm.quot(idx, k);
     float2 v = make_float2(0.f, 0.f);
     if (k < n) {
       float2 c = p.chirp[k];
       if (sign > 0) c.y = -c.y;
-      v = cmul(a[l * ls + k], c);
+      v = cmul(a[at<MF>(l * ls + k)], c);
     }
-    a[l * ls + k] = v;
+    a[at<MF>(l * ls + k)] = v;
   }
   __syncthreads();
-  float2* r = lds_length_m<MF>(a, b, lines, ls, p, -1, tid, T);
+  float2* r = lds_length_m<MF, FM>(a, b, lines, ls, p, -1, tid, T);
   for (int idx = tid; idx < lines * m; idx += T) {
-    const int l = idx / m, k = idx - l * m;
+    int k;
+    const int l = dm.quot(idx, k);
     float2 kh = p.khat[k];
     if (sign > 0) kh.y = -kh.y;
-    r[l * ls + k] = cmul(r[l * ls + k], kh);
+    r[at<MF>(l * ls + k)] = cmul(r[at<MF>(l * ls + k)], kh);
   }
   __syncthreads();
   float2* o = (r == a) ? b : a;
-  r = lds_length_m<MF>(r, o, lines, ls, p, +1, tid, T);
+  r = lds_length_m<MF, FM>(r, o, lines, ls, p, +1, tid, T);
   for (int idx = tid; idx < lines * n; idx += T) {
-    const int l = idx / n, k = idx - l * n;
+    int k;
+    const int l = dn.quot(idx, k);
     float2 c = p.chirp[k];
     if (sign > 0) c.y = -c.y;
-    r[l * ls + k] = cmul(r[l * ls + k], c);
+    r[at<MF>(l * ls + k)] = cmul(r[at<MF>(l * ls + k)], c);
   }
   __syncthreads();
   return r;
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// Generic row pass: block = `rows` lines along y of one wave (blockIdx.y). MF selects the line transform (0 float32
-// Stockham, else the split-BF16 stages of n2 = MF), one kernel each so the float32 kernel keeps its registers.
+// Generic row pass: block = `rows` lines along y of one wave (blockIdx.y). MF and FM select the line transform (MF = 0:
+// float32 Stockham, of the fixed length FM when FM > 0; else the split-BF16 stages of n2 = MF), one kernel each so
+// the float32 kernels keep their registers.
 // ---------------------------------------------------------------------------------------------------------------
-constexpr int kGenericThreads = 256;
-constexpr int kGenericLdsComplex = 2048;  // complex values per LDS buffer (16 KiB); two buffers
-
 inline int row_lines(int m) { return m >= kGenericLdsComplex ? 1 : kGenericLdsComplex / m; }
 int col_generic_lines(int m) { return m >= kGenericLdsComplex ? 1 : kGenericLdsComplex / m; }
 
-template <int MF>
+template <int MF, int FM>
 __global__ void __launch_bounds__(kGenericThreads) row_generic_kernel(RowArgs a, int rows) {
   extern __shared__ float2 lds[];
-  const int m = a.fy.m, ls = m;
+  const int m = FM > 0 ? FM : a.fy.m, ls = m;
   float2* A = lds;
-  float2* B = lds + rows * ls;
+  float2* B = lds + lds_entries(rows * ls, MF == 0);
   const int tid = threadIdx.x, T = blockDim.x;
   const int x0 = blockIdx.x * rows;
   const long long b = blockIdx.y;
   const int f = a.flags;
   const long long gw = (long long)a.nx * a.nc;
+  const Div dm(m), dny(a.ny), dnc(a.nc);
 
   for (int idx = tid; idx < rows * m; idx += T) {
-    const int l = idx / m, k = idx - l * m, x = x0 + l;
+    int k;
+    const int l = dm.quot(idx, k), x = x0 + l;
     float2 v = make_float2(0.f, 0.f);
     if (x < a.nx && k < a.ny) {
       if (f & ROW_IN_COMPACT) {
@@ -193,39 +302,42 @@ __global__ void __launch_bounds__(kGenericThreads) row_generic_kernel(RowArgs a,
         v = a.in_full[b * a.in_full_stride + (long long)x * a.ny + k];
       }
     }
-    A[l * ls + k] = v;
+    A[at<MF>(l * ls + k)] = v;
   }
   __syncthreads();
   float2* cur = A;
   float2* oth = B;
   if (f & ROW_INV) {
-    cur = lds_line_transform<MF>(cur, oth, rows, ls, a.fy, +1, tid, T);
+    cur = lds_line_transform<MF, FM>(cur, oth, rows, ls, a.fy, +1, tid, T);
     oth = (cur == A) ? B : A;
   }
   if (f & (ROW_WRITE_PLANE | ROW_MUL_T)) {
     for (int idx = tid; idx < rows * a.ny; idx += T) {
-      const int l = idx / a.ny, k = idx - l * a.ny, x = x0 + l;
+      int k;
+      const int l = dny.quot(idx, k), x = x0 + l;
       if (x >= a.nx) continue;
-      float2 v = cur[l * ls + k];
+      float2 v = cur[at<MF>(l * ls + k)];
       if (f & ROW_WRITE_PLANE) a.plane[b * a.plane_stride + (long long)x * a.ny + k] = cscale(v, a.out_scale);
-      if (f & ROW_MUL_T) cur[l * ls + k] = cmul(v, a.t[(long long)x * a.ny + k]);
+      if (f & ROW_MUL_T) cur[at<MF>(l * ls + k)] = cmul(v, a.t[(long long)x * a.ny + k]);
     }
     __syncthreads();
   }
-  if (f & ROW_FWD) cur = lds_line_transform<MF>(cur, oth, rows, ls, a.fy, -1, tid, T);
+  if (f & ROW_FWD) cur = lds_line_transform<MF, FM>(cur, oth, rows, ls, a.fy, -1, tid, T);
   if (f & ROW_OUT_COMPACT) {
     for (int idx = tid; idx < rows * a.nc; idx += T) {
-      const int l = idx / a.nc, c = idx - l * a.nc, x = x0 + l;
+      int c;
+      const int l = dnc.quot(idx, c), x = x0 + l;
       if (x >= a.nx) continue;
       const int iy = a.keep[c];
-      a.G[b * gw + (long long)x * a.nc + c] = iy >= 0 ? cur[l * ls + iy] : make_float2(0.f, 0.f);
+      a.G[b * gw + (long long)x * a.nc + c] = iy >= 0 ? cur[at<MF>(l * ls + iy)] : make_float2(0.f, 0.f);
     }
   }
   if (f & ROW_OUT_FULL) {
     for (int idx = tid; idx < rows * a.ny; idx += T) {
-      const int l = idx / a.ny, k = idx - l * a.ny, x = x0 + l;
+      int k;
+      const int l = dny.quot(idx, k), x = x0 + l;
       if (x >= a.nx) continue;
-      a.out_full[b * a.out_full_stride + (long long)x * a.ny + k] = cscale(cur[l * ls + k], a.out_scale);
+      a.out_full[b * a.out_full_stride + (long long)x * a.ny + k] = cscale(cur[at<MF>(l * ls + k)], a.out_scale);
     }
   }
 }
@@ -249,22 +361,24 @@ __device__ inline void block_reduce_slots(double (&s)[kMaxSlots], int nslots, do
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// Generic column pass: block = `cols` compact columns (lines along x) of one wave; MF as in the row pass.
+// Generic column pass: block = `cols` compact columns (lines along x) of one wave; MF and FM as in the row pass.
 // ---------------------------------------------------------------------------------------------------------------
-template <int MF>
+template <int MF, int FM>
 __global__ void __launch_bounds__(kGenericThreads) col_generic_kernel(ColArgs a, int cols) {
   extern __shared__ float2 lds[];
-  const int m = a.fx.m, ls = m;
+  const int m = FM > 0 ? FM : a.fx.m, ls = m;
   float2* A = lds;
-  float2* B = lds + cols * ls;
+  float2* B = lds + lds_entries(cols * ls, MF == 0);
   const int tid = threadIdx.x, T = blockDim.x;
   const int c0 = blockIdx.x * cols;
   const long long b = blockIdx.y;
   const int f = a.flags;
   const long long gw = (long long)a.nx * a.nc;
+  const Div dcols(cols);
 
   for (int idx = tid; idx < cols * m; idx += T) {
-    const int k = idx / cols, l = idx - k * cols, c = c0 + l;
+    int l;
+    const int k = dcols.quot(idx, l), c = c0 + l;
     float2 v = make_float2(0.f, 0.f);
     if (k < a.nx && c < a.nc) {
       if (f & COL_IN_G) {
@@ -275,19 +389,20 @@ __global__ void __launch_bounds__(kGenericThreads) col_generic_kernel(ColArgs a,
           v = cmul(cmul(a.spec[(long long)k * a.ny + iy], a.rx[b * a.nx + k]), a.ry[b * a.ny + iy]);
       }
     }
-    A[l * ls + k] = v;
+    A[at<MF>(l * ls + k)] = v;
   }
   __syncthreads();
   float2* cur = A;
   float2* oth = B;
   if (f & COL_FWD) {
-    cur = lds_line_transform<MF>(cur, oth, cols, ls, a.fx, -1, tid, T);
+    cur = lds_line_transform<MF, FM>(cur, oth, cols, ls, a.fx, -1, tid, T);
     oth = (cur == A) ? B : A;
   }
   if (f & COL_MUL_P) {
     for (int idx = tid; idx < cols * a.nx; idx += T) {
-      const int k = idx / cols, l = idx - k * cols, c = c0 + l;
-      if (c < a.nc) cur[l * ls + k] = cmul(cur[l * ls + k], a.P[(long long)k * a.nc + c]);
+      int l;
+      const int k = dcols.quot(idx, l), c = c0 + l;
+      if (c < a.nc) cur[at<MF>(l * ls + k)] = cmul(cur[at<MF>(l * ls + k)], a.P[(long long)k * a.nc + c]);
     }
     __syncthreads();
   }
@@ -297,11 +412,12 @@ __global__ void __launch_bounds__(kGenericThreads) col_generic_kernel(ColArgs a,
     for (int q = 0; q < kMaxSlots; ++q) s[q] = 0.0;
     const long long pos = first_position(a.pos0, a.pos_dev) + b;
     for (int idx = tid; idx < cols * a.nx; idx += T) {
-      const int k = idx / cols, l = idx - k * cols, c = c0 + l;
+      int l;
+      const int k = dcols.quot(idx, l), c = c0 + l;
       if (c >= a.nc) continue;
       const int iy = a.keep[c];
       if (iy < 0) continue;
-      const float2 v = cur[l * ls + k];
+      const float2 v = cur[at<MF>(l * ls + k)];
       const double I = ((double)v.x * v.x + (double)v.y * v.y) * a.det.scale;
       detect_pixel(a.det, k, iy, I, pos, s);
     }
@@ -312,11 +428,12 @@ __global__ void __launch_bounds__(kGenericThreads) col_generic_kernel(ColArgs a,
                        a.det.blocks_per_wave);
     return;
   }
-  if (f & COL_INV) cur = lds_line_transform<MF>(cur, oth, cols, ls, a.fx, +1, tid, T);
+  if (f & COL_INV) cur = lds_line_transform<MF, FM>(cur, oth, cols, ls, a.fx, +1, tid, T);
   if (f & COL_OUT) {
     for (int idx = tid; idx < cols * a.nx; idx += T) {
-      const int k = idx / cols, l = idx - k * cols, c = c0 + l;
-      if (c < a.nc) a.G[b * gw + (long long)k * a.nc + c] = cur[l * ls + k];
+      int l;
+      const int k = dcols.quot(idx, l), c = c0 + l;
+      if (c < a.nc) a.G[b * gw + (long long)k * a.nc + c] = cur[at<MF>(l * ls + k)];
     }
   }
 }
@@ -336,40 +453,64 @@ __global__ void reduce_partials_kernel(DetPlan d, int pos0, const int* pos_dev, 
 // ---------------------------------------------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------------------------------------------
-// The kernel variant of a plan's line transform: 0 float32, else n2 = m / 16 of the split-BF16 stages.
+// The kernel variant of a plan's line transform: MF = 0 float32, else n2 = m / 16 of the split-BF16 stages; FM = the
+// float32 lengths with compile-time stages (384, 1024), else 0.
 int transform_kind(const LineFFT& p) {
   if (!p.use_mfma) return 0;
   if (p.m != 256 && p.m != 384 && p.m != 512 && p.m != 1024)
     fail(SK_ERR_INVALID, "split-BF16 plan of unsupported length " + std::to_string(p.m));
   return p.m / 16;
 }
+int fixed_length(const LineFFT& p) {
+  if (p.use_mfma || (p.m != 384 && p.m != 1024)) return 0;
+  // the compile-time stage lists must be the plan's: 384 = 8 8 2 3, 1024 = 8 8 8 2
+  const int want[2][4] = {{8, 8, 2, 3}, {8, 8, 8, 2}};
+  const int* w = want[p.m == 384 ? 0 : 1];
+  if (p.nst != 4 || p.radix[0] != w[0] || p.radix[1] != w[1] || p.radix[2] != w[2] || p.radix[3] != w[3])
+    fail(SK_ERR_INVALID, "plan radices differ from the compile-time stages of length " + std::to_string(p.m));
+  return p.m;
+}
+
+// The kernel of the plan's variant: Pick<MF, FM>::kernel() for the plan's MF and FM.
+template <template <int, int> class Pick>
+auto generic_variant(const LineFFT& p) {
+  switch (transform_kind(p)) {
+    case 0: {
+      const int fm = fixed_length(p);
+      return fm == 384 ? Pick<0, 384>::kernel() : fm == 1024 ? Pick<0, 1024>::kernel() : Pick<0, 0>::kernel();
+    }
+    case 16: return Pick<16, 0>::kernel();
+    case 24: return Pick<24, 0>::kernel();
+    case 32: return Pick<32, 0>::kernel();
+    default: return Pick<64, 0>::kernel();
+  }
+}
+template <int MF, int FM>
+struct RowKernel {
+  static auto kernel() { return &row_generic_kernel<MF, FM>; }
+};
+template <int MF, int FM>
+struct ColKernel {
+  static auto kernel() { return &col_generic_kernel<MF, FM>; }
+};
 
 void launch_row_generic(const RowArgs& a, int batch, hipStream_t s) {
   const int rows = row_lines(a.fy.m);
   dim3 grid((a.nx + rows - 1) / rows, batch);
-  const size_t lds = 2 * sizeof(float2) * rows * a.fy.m;
-  switch (transform_kind(a.fy)) {
-    case 0: hipLaunchKernelGGL(row_generic_kernel<0>, grid, dim3(kGenericThreads), lds, s, a, rows); break;
-    case 16: hipLaunchKernelGGL(row_generic_kernel<16>, grid, dim3(kGenericThreads), lds, s, a, rows); break;
-    case 24: hipLaunchKernelGGL(row_generic_kernel<24>, grid, dim3(kGenericThreads), lds, s, a, rows); break;
-    case 32: hipLaunchKernelGGL(row_generic_kernel<32>, grid, dim3(kGenericThreads), lds, s, a, rows); break;
-    default: hipLaunchKernelGGL(row_generic_kernel<64>, grid, dim3(kGenericThreads), lds, s, a, rows); break;
-  }
+  const size_t lds = 2 * sizeof(float2) * lds_entries(rows * a.fy.m, !a.fy.use_mfma);
+  const auto kernel = generic_variant<RowKernel>(a.fy);
+  hipLaunchKernelGGL(kernel, grid, dim3(kGenericThreads), lds, s, a, rows);
   SK_HIP(hipGetLastError());
 }
 
 void launch_col_generic(const ColArgs& a, int batch, hipStream_t s) {
   const int cols = col_generic_lines(a.fx.m);
   dim3 grid((a.nc + cols - 1) / cols, batch);
-  size_t lds = 2 * sizeof(float2) * cols * a.fx.m;
-  if (a.flags & COL_DETECT) lds = std::max(lds, sizeof(float2) * cols * a.fx.m + sizeof(double) * kGenericThreads);
-  switch (transform_kind(a.fx)) {
-    case 0: hipLaunchKernelGGL(col_generic_kernel<0>, grid, dim3(kGenericThreads), lds, s, a, cols); break;
-    case 16: hipLaunchKernelGGL(col_generic_kernel<16>, grid, dim3(kGenericThreads), lds, s, a, cols); break;
-    case 24: hipLaunchKernelGGL(col_generic_kernel<24>, grid, dim3(kGenericThreads), lds, s, a, cols); break;
-    case 32: hipLaunchKernelGGL(col_generic_kernel<32>, grid, dim3(kGenericThreads), lds, s, a, cols); break;
-    default: hipLaunchKernelGGL(col_generic_kernel<64>, grid, dim3(kGenericThreads), lds, s, a, cols); break;
-  }
+  const size_t buf = sizeof(float2) * lds_entries(cols * a.fx.m, !a.fx.use_mfma);
+  size_t lds = 2 * buf;
+  if (a.flags & COL_DETECT) lds = std::max(lds, buf + sizeof(double) * kGenericThreads);
+  const auto kernel = generic_variant<ColKernel>(a.fx);
+  hipLaunchKernelGGL(kernel, grid, dim3(kGenericThreads), lds, s, a, cols);
   SK_HIP(hipGetLastError());
 }
 
