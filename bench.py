@@ -6,11 +6,13 @@ GridScan over one 3.905 Angstrom cell, annular detector 50-200 mrad, complex64 w
 
 One step = one full 64 x 64 scan (4,096 positions x 40 slices = 163,840 position-slices) with the prepared
 potential resident in HBM and the HAADF image left in HBM. `value` = position-slices per second over the K timed
-steps at the library's positions per launch: the largest batch whose waves stay in the 256 MiB Infinity Cache,
-spread evenly over the scan (171 here). The same scan at 1024 positions per launch (`--compare-batch`) is timed
-first and reported beside it as `compare_batch`. The roofline entry times the dominant kernel with HIP events on the library's stream; the CPU baseline runs
-the float64 oracle on a bounded sample of positions on the host's cores, and the same sample gives the HIP path's
-error against float64.
+steps at the library's positions per launch and streams: each batch is split over two streams, and at 512 x 512 the
+batch is 3/4 of the largest one whose waves stay in the 256 MiB Infinity Cache, spread evenly over the scan (128
+here, 64 on each stream). The same scan at 1024 positions per launch (`--compare-batch`) is timed first and reported
+beside it as `compare_batch`. The roofline entry times every pass launch of one more scan with HIP events attached
+to the launches; that scan runs the same batches on one stream, so each launch runs alone. The CPU baseline runs the
+float64 oracle on a bounded sample of positions on the host's cores, and the same sample gives the HIP path's error
+against float64.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -61,6 +63,8 @@ def main():
     ap.add_argument("--graph", action="store_true", help="replay each batch's slice loop from a hipGraph")
     ap.add_argument("--generic", action="store_true", help="generic LDS passes instead of the fast passes")
     ap.add_argument("--col-tile", type=int, default=0, help="columns per 512-point column-pass block (8, 16)")
+    ap.add_argument("--streams", type=int, default=0,
+                    help="streams each batch is split over (1-4; 0 = the library's choice, 2 for this scan)")
     ap.add_argument("--cpu-sample", type=int, default=128, help="positions of the float64 CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(8, cores), the reference's default")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -106,7 +110,7 @@ def main():
     scan = sk.GridScan((0.0, 0.0), (sk.workloads.SRTIO3_A, sk.workloads.SRTIO3_A), w.scan_n, w.scan_n)
     det = sk.AnnularDetector(w.inner, w.outer)
     opt = sk.ExecutionOptions(batch=w.batch, use_graph=args.graph, fast_path=not args.generic,
-                              col_tile=args.col_tile)
+                              col_tile=args.col_tile, streams=args.streams)
     S = pot.slice_count
     npos = w.scan_n * w.scan_n
 
@@ -132,7 +136,7 @@ def main():
     compare = None
     if args.compare_batch > 0 and args.compare_batch != args.batch:
         o = sk.ExecutionOptions(batch=args.compare_batch, use_graph=args.graph, fast_path=not args.generic,
-                                col_tile=args.col_tile)
+                                col_tile=args.col_tile, streams=args.streams)
         el = timed(o)
         compare = {"batch": min(args.compare_batch, npos), "value": npos * S * args.steps / el,
                    "ms_per_step": el / args.steps * 1e3}
@@ -172,6 +176,8 @@ def main():
         "config": {"workload": w.name, "grid": [w.n, w.n], "slices": S, "positions": npos,
                    "batch": B, "detector": f"annular {w.inner:g}-{w.outer:g} mrad", "energy_keV": w.energy / 1e3,
                    "semiangle_mrad": w.semiangle, "fast_path": not args.generic, "graph": bool(args.graph),
+                   # the library splits this scan's batches over 2 streams unless --streams says otherwise
+                   "streams": 1 if (args.graph or args.generic) else (args.streams or 2),
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(f"{dom} pass", B),

@@ -34,7 +34,7 @@ class SkDetector(C.Structure):
 class SkOptions(C.Structure):
     _fields_ = [("batch", C.c_int), ("use_graph", C.c_int), ("fast_path", C.c_int), ("pipeline", C.c_int),
                 ("lane_waves", C.c_int), ("col_tile", C.c_int),
-                ("precision", C.c_int), ("reserved", C.c_int * 1)]
+                ("precision", C.c_int), ("streams", C.c_int)]
 
 
 class SkTiming(C.Structure):

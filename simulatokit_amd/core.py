@@ -214,6 +214,8 @@ class ExecutionOptions:
     lane_waves: int = 0      # positions per lane (0 = the library's choice)
     col_tile: int = 0        # 512-point columns: columns per column-pass block, 8 or 16 (0 = the library's choice)
     precision: str = "float32"  # the slice loop's transforms: "float32", or "split_bf16" (matrix cores)
+    streams: int = 0         # scan one potential at a time through the fast passes: streams the parts of each batch
+                             # run on (1-4), 0 the library's choice (2)
 
 
 @dataclass(frozen=True)

@@ -55,6 +55,7 @@ struct Context {
   int status = SK_ERR_UNAVAILABLE;
   std::string why;
   hipStream_t stream = nullptr;
+  std::vector<hipStream_t> side;  // further streams of split scans, created on first use (default priority)
   bool big_llc = false;   // the device has a 256 MiB last-level cache (MI300 / MI355 class: gfx94x, gfx950)
   DevBuf<double> lobato;  // 103 x 10
   // timing
@@ -502,6 +503,14 @@ Workspace& ws() {
   return *w;
 }
 
+// Streams of a split scan (sk_options.streams = 0): the parts of each batch run their slice loops on two streams, so
+// one part's row pass fills the tail of the other's column pass. Measured on the HAADF scan (4096 positions), two
+// streams against one: 256 x 256 at 683 positions 4.30 M against 4.01 M position-slices/s; 512 x 512 at 128
+// positions 963-989 k against 945-960 k at 171 positions for one stream, and 954-983 k at 171 for two. Three
+// streams gain less, four lose. At 512 x 512 the split scan takes 3/4 of the cache-resident batch.
+constexpr int kSplitStreams = 2;
+constexpr int kMaxStreams = 4;
+
 void set_col_tile(Context& c, const sk_options* opt) {
   const int t = opt ? opt->col_tile : 0;
   if (t != 0 && t != 8 && t != 16) fail(SK_ERR_INVALID, "col_tile must be 0, 8 or 16");
@@ -509,6 +518,9 @@ void set_col_tile(Context& c, const sk_options* opt) {
   const int pr = opt ? opt->precision : 0;
   if (pr != 0 && pr != 1) fail(SK_ERR_INVALID, "precision must be 0 (float32) or 1 (split-BF16)");
   c.precision = pr;
+  const int st = opt ? opt->streams : 0;
+  if (st < 0 || st > kMaxStreams) fail(SK_ERR_INVALID, "streams must be 0 (the library's choice) to " +
+                                                           std::to_string(kMaxStreams));
 }
 
 // the transform plan of length n for the slice loop under the running call's precision
@@ -527,10 +539,11 @@ constexpr int kMinCachedBatch = 64;
 // device memory the lane pipeline may take for per-lane detector accumulators when lanes share positions
 constexpr size_t kLaneCopyBytes = size_t(2) << 30;
 
-int default_batch(const sk_potential& p, int count, const sk_options* opt) {
+int default_batch(const sk_potential& p, int count, const sk_options* opt, double cache_share = 1.0) {
   int b = opt && opt->batch > 0 ? opt->batch : 0;
   if (b > 0) return std::max(1, std::min(b, count));
-  const size_t cached = ctx().big_llc ? kCachedWaveBytes / ((size_t)p.grid.nx * p.nc * sizeof(float2)) : 0;
+  const size_t cached =
+      ctx().big_llc ? (size_t)(cache_share * kCachedWaveBytes / ((double)p.grid.nx * p.nc * sizeof(float2))) : 0;
   if (cached >= (size_t)kMinCachedBatch)
     b = (int)std::min<size_t>(1024, cached);
   else
@@ -637,8 +650,9 @@ struct Timer {
 };
 
 void slice_loop(const sk_potential& p, int batch, float2* G, bool entry_full, float2* full, int first_slice_from_full,
-                const std::vector<int>& plane_out, float2* planes, const ColArgs* final_col, bool fast, Timer& tm) {
-  hipStream_t s = ctx().stream;
+                const std::vector<int>& plane_out, float2* planes, const ColArgs* final_col, bool fast, Timer& tm,
+                hipStream_t stream = nullptr) {
+  hipStream_t s = stream ? stream : ctx().stream;
   const size_t N = (size_t)p.grid.nx * p.grid.ny;
   RowArgs r = row_args(p, G);
   ColArgs c = col_args(p, G);
@@ -1731,6 +1745,20 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
         B = L * W;
       }
     }
+    // split scan: the parts of each batch run on their own streams, one potential at a time, without graph replay,
+    // through the fast passes with the probe inside the bandlimit. A timed scan (per-launch events) keeps the batch
+    // and runs it on one stream.
+    const bool graph = opt && opt->use_graph;
+    int split = 1;
+    if (!pipe && !graph && fits && fast &&
+        fast_col_supported(nx, p0.nc, COL_IN_G | COL_FWD | COL_MUL_P | COL_INV | COL_OUT)) {
+      const int want = opt ? opt->streams : 0;
+      split = want > 0 ? want : kSplitStreams;
+      if (want == 0 && split > 1 && !(opt && opt->batch > 0) && nx == 512 && ny == 512)
+        B = default_batch(p0, npos, opt, 0.75);
+      split = std::max(1, std::min(split, B));
+      if (c.timing) split = 1;
+    }
     w.G.ensure((size_t)B * nx * p0.nc);
     w.rx.ensure((size_t)B * nx);
     w.ry.ensure((size_t)B * ny);
@@ -1750,30 +1778,66 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
     SK_HIP(hipEventRecord(e0, s));
     // One batch of nb positions starting at b0 (+ *pos_dev when replayed from a graph) through potential P:
     // ramps, the first column pass (the probe, generated), every slice, the detector reduction.
-    auto enqueue_batch = [&](const sk_potential& P, int b0, const int* pos_dev, int nb, Timer& t) {
+    // `ss` and `off`: the stream and the first workspace position of a split scan's half (s and 0 otherwise).
+    auto enqueue_batch = [&](const sk_potential& P, int b0, const int* pos_dev, int nb, Timer& t,
+                             hipStream_t ss = nullptr, int off = 0) {
+      if (!ss) ss = s;
+      float2* G = w.G.p + (size_t)off * nx * p0.nc;
+      float2* rx = w.rx.p + (size_t)off * nx;
+      float2* ry = w.ry.p + (size_t)off * ny;
+      DetPlan dph = dp;
+      dph.partial = dp.partial + (size_t)off * nslots * bpw;
       std::vector<int> plane_out(P.nslices, -1);
-      launch_ramps(nx, ny, g.extent_x / nx, g.extent_y / ny, w.positions.p, b0, pos_dev, nb, w.rx.p, w.ry.p, s);
-      ColArgs init = col_args(P, fits ? w.G.p : w.full.p);
+      launch_ramps(nx, ny, g.extent_x / nx, g.extent_y / ny, w.positions.p, b0, pos_dev, nb, rx, ry, ss);
+      ColArgs init = col_args(P, fits ? G : w.full.p);
       init.spec = w.spec.p;
-      init.rx = w.rx.p;
-      init.ry = w.ry.p;
+      init.rx = rx;
+      init.ry = ry;
       init.flags = COL_GENERATE | COL_INV | COL_OUT;
       if (!fits) {
         init.nc = ny;
         init.keep = identity_map(ny);
       }
-      col_pass(init, nb, fast && fits, s);
-      ColArgs fin = col_args(P, w.G.p);
+      col_pass(init, nb, fast && fits, ss);
+      ColArgs fin = col_args(P, G);
       fin.flags = COL_IN_G | COL_FWD | COL_MUL_P | COL_DETECT;
       fin.pos0 = b0;
       fin.pos_dev = pos_dev;
-      fin.det = dp;
-      slice_loop(P, nb, w.G.p, false, w.full.p, fits ? 0 : 1, plane_out, nullptr, &fin, fast, t);
-      launch_reduce_partials(dp, nb, b0, pos_dev, w.acc.p, s);
+      fin.det = dph;
+      slice_loop(P, nb, G, false, w.full.p, fits ? 0 : 1, plane_out, nullptr, &fin, fast, t, ss);
+      launch_reduce_partials(dph, nb, b0, pos_dev, w.acc.p, ss);
     };
-    const bool graph = opt && opt->use_graph;
     if (pipe) {
       run_pipeline(pots, npots, npos, W, L, g, dp, graph, tm, timing);
+    } else if (split > 1) {
+      // k parts of each batch, each through its own slice loop on its own stream
+      const int k = split;
+      while ((int)c.side.size() < k - 1) {
+        hipStream_t t;
+        SK_HIP(hipStreamCreateWithFlags(&t, hipStreamNonBlocking));
+        c.side.push_back(t);
+      }
+      std::vector<hipStream_t> st(1, s);
+      for (int q = 0; q < k - 1; ++q) st.push_back(c.side[q]);
+      hipEvent_t fork;
+      SK_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+      SK_HIP(hipEventRecord(fork, s));
+      for (int q = 1; q < k; ++q) SK_HIP(hipStreamWaitEvent(st[q], fork, 0));
+      const int H = (B + k - 1) / k;
+      for (int pk = 0; pk < npots; ++pk)
+        for (int b0 = 0; b0 < npos; b0 += B) {
+          const int nb = std::min(B, npos - b0);
+          for (int q = 0; q < k && q * H < nb; ++q)
+            enqueue_batch(*pots[pk], b0 + q * H, nullptr, std::min(H, nb - q * H), tm, st[q], q * H);
+        }
+      std::vector<hipEvent_t> joins(k - 1);
+      for (int q = 1; q < k; ++q) {
+        SK_HIP(hipEventCreateWithFlags(&joins[q - 1], hipEventDisableTiming));
+        SK_HIP(hipEventRecord(joins[q - 1], st[q]));
+        SK_HIP(hipStreamWaitEvent(s, joins[q - 1], 0));
+      }
+      SK_HIP(hipEventDestroy(fork));
+      for (auto e : joins) SK_HIP(hipEventDestroy(e));
     } else if (!graph) {
       for (int k = 0; k < npots; ++k)
         for (int b0 = 0; b0 < npos; b0 += B) {

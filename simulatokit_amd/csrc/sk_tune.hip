@@ -93,10 +93,12 @@ extern "C" int sk_hip_autotune(const sk_potential* const* pots, int npots, const
     size_t free_b = 0, total_b = 0;
     SK_HIP(hipMemGetInfo(&free_b, &total_b));
     const double wave_bytes = 8.0 * g.nx * g.ny;
-    // the library's default (sized to stay in the last-level cache) and fixed sizes that stream from HBM
-    std::vector<int> batches{default_scan_batch(*pots[0], npos)};
+    // the library's default (batch 0: sized to stay in the last-level cache, split over two streams where the scan
+    // can) and fixed sizes that stream from HBM
+    std::vector<int> batches{0};
+    const int cached = default_scan_batch(*pots[0], npos);
     for (int b : {512, 1024, 2048, 4096})
-      if (b <= std::max(512, npos) && b != batches[0] && 3.0 * b * wave_bytes < 0.5 * (double)free_b)
+      if (b <= std::max(512, npos) && b != cached && 3.0 * b * wave_bytes < 0.5 * (double)free_b)
         batches.push_back(b);
     std::vector<int> tiles = g.nx == 512 ? std::vector<int>{8, 16} : std::vector<int>{0};
     std::vector<int> pipes = npots > 1 ? std::vector<int>{0, 1} : std::vector<int>{0};

@@ -35,12 +35,12 @@ class TuneResult:
 
 def _as_options(o: N.SkOptions) -> ExecutionOptions:
     return ExecutionOptions(batch=o.batch, use_graph=bool(o.use_graph), fast_path=bool(o.fast_path),
-                            pipeline=o.pipeline, lane_waves=o.lane_waves, col_tile=o.col_tile)
+                            pipeline=o.pipeline, lane_waves=o.lane_waves, col_tile=o.col_tile, streams=o.streams)
 
 
 def _plan_dict(o: ExecutionOptions) -> Dict:
     return {"batch": o.batch, "use_graph": o.use_graph, "fast_path": o.fast_path, "pipeline": o.pipeline,
-            "lane_waves": o.lane_waves, "col_tile": o.col_tile}
+            "lane_waves": o.lane_waves, "col_tile": o.col_tile, "streams": o.streams}
 
 
 def tune_key(device: str, grid, slice_counts: Sequence[int], detectors: Sequence[Detector]) -> str:

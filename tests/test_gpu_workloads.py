@@ -80,8 +80,9 @@ def test_full_haadf_scan_equals_the_oracle_at_sample_positions(haadf_scan):
 
 def test_default_batch_is_sized_to_the_infinity_cache(haadf_scan):
     """The library's positions per launch: the largest batch whose compact waves (nx x 352 complex64 at 512^2) take at
-    most 240 MiB, spread evenly over the scan: 4,096 positions -> 24 launches of at most 171. A fixed batch is
-    honoured as given, and the chunking changes no number the scan computes."""
+    most 240 MiB, spread evenly over the scan: 4,096 positions -> 24 launches of at most 171 on one stream. The
+    default split scan (two streams) takes 3/4 of that at 512^2: 32 launches of 128, 64 on each stream. A fixed
+    batch is honoured as given, and the chunking changes no number the scan computes."""
     w, st, pot, scan, haadf, _ = haadf_scan
     probe = sk.Probe(sk.ElectronBeam(w.energy), pot.grid, sk.Aperture(w.semiangle))
     prepared = pot.prepare(probe.beam)
@@ -89,9 +90,43 @@ def test_default_batch_is_sized_to_the_infinity_cache(haadf_scan):
     sk.set_timing(True)
     try:
         sk.STEM.scan(probe, [prepared], scan, det)
+        assert sk.last_timing()["batch"] == 128
+        one = sk.STEM.scan(probe, [prepared], scan, det, sk.ExecutionOptions(streams=1))[0]
         assert sk.last_timing()["batch"] == 171
         fixed = sk.STEM.scan(probe, [prepared], scan, det, sk.ExecutionOptions(batch=1024))[0]
         assert sk.last_timing()["batch"] == 1024
     finally:
         sk.set_timing(False)
     assert rel_max(fixed.values, haadf.values) <= 1e-12
+    assert rel_max(one.values, haadf.values) <= 1e-12
+
+
+@pytest.mark.parametrize("n", [256, 512])
+def test_split_scans_equal_one_stream(n):
+    """Scans whose batches are split over 2, 3 and 4 streams compute the same numbers as one stream, bit for bit
+    (every position runs the same kernels; only the launches it shares differ), for every detector kind, including
+    a batch that does not divide evenly and a last batch smaller than the streams' parts."""
+    st = srtio3(4, 4, 6)
+    pot = sk.Potential(st, n, n, slice_thickness=2.0)
+    probe = sk.Probe(sk.ElectronBeam(200e3), pot.grid, sk.Aperture(20.0))
+    positions = sk.GridScan((0, 0), (SRTIO3_A, SRTIO3_A), 7, 9).positions
+    dets = [sk.AnnularDetector(30, 90), sk.PixelatedDetector(crop=(32, 32)), sk.CenterOfMassDetector(),
+            sk.SegmentedDetector(2, 4, 10, 40)]
+    prepared = pot.prepare(probe.beam)
+    want = sk.STEM.scan(probe, [prepared], positions, dets, sk.ExecutionOptions(streams=1, batch=20))
+    for streams in (2, 3, 4):
+        got = sk.STEM.scan(probe, [prepared], positions, dets, sk.ExecutionOptions(streams=streams, batch=20))
+        for x, y in zip(want, got):
+            assert np.array_equal(np.asarray(x.values), np.asarray(y.values)), (streams, type(x))
+    dflt = sk.STEM.scan(probe, [prepared], positions, dets)
+    for x, y in zip(want, dflt):
+        assert np.array_equal(np.asarray(x.values), np.asarray(y.values))
+
+
+def test_streams_outside_one_to_four_are_refused():
+    st = srtio3(2, 2, 2)
+    pot = sk.Potential(st, 256, 256, slice_thickness=2.0)
+    probe = sk.Probe(sk.ElectronBeam(200e3), pot.grid, sk.Aperture(20.0))
+    for bad in (-1, 5):
+        with pytest.raises(sk.SimulatoError):
+            sk.STEM.scan(probe, [pot], [(1.0, 1.0)], [sk.AnnularDetector(30, 90)], sk.ExecutionOptions(streams=bad))
