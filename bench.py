@@ -216,7 +216,7 @@ def frozen_phonon_bench(args, sk, torch):
     npos = n * n
     out_pix = torch.empty((npos, 128, 128), dtype=torch.float32, device="cuda:0")
     out_com = torch.empty((npos, 2), dtype=torch.float64, device="cuda:0")
-    opt = sk.ExecutionOptions(batch=args.batch, pipeline=args.pipeline)
+    opt = sk.ExecutionOptions(batch=args.batch, pipeline=args.pipeline, streams=args.streams)
 
     def step():
         sk.STEM.scan(probe, prepared, scan, dets, opt, device_outputs=[out_pix.data_ptr(), out_com.data_ptr()])
@@ -237,6 +237,7 @@ def frozen_phonon_bench(args, sk, torch):
             "higher_is_better": True, "dtype": "c64", "data": "synthetic",
             "config": {"workload": f"frozen_phonons_16cfg_512sq_{n}x{n}", "configurations": len(prepared),
                        "slices": slices, "positions": npos, "batch": args.batch or "library", "pipeline": args.pipeline,
+                       "streams": args.streams or "library",
                        "detectors": "pixelated 128x128 float32 + centre of mass"},
             "prepare_ms": prepare_ms, "scan_pos_slices": work}
 
