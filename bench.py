@@ -177,7 +177,7 @@ def main():
                    "batch": B, "detector": f"annular {w.inner:g}-{w.outer:g} mrad", "energy_keV": w.energy / 1e3,
                    "semiangle_mrad": w.semiangle, "fast_path": not args.generic, "graph": bool(args.graph),
                    # the library splits this scan's batches over 2 streams unless --streams says otherwise
-                   "streams": 1 if (args.graph or args.generic) else (args.streams or 2),
+                   "streams": 1 if args.generic else (args.streams or 2),
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(f"{dom} pass", B),

@@ -84,9 +84,10 @@ typedef struct sk_options {
                        (bfloat16 hi/lo pairs, three products, float32 accumulation; grid lengths 256,
                        384, 512 and 1024, and the Bluestein lengths 65-512 that have a prime factor
                        above 13) */
-  int streams;      /* scan, one potential at a time, no graph replay, fast passes, probe inside the bandlimit: the
-                       parts of each batch run their slice loops on this many streams (1-4); 0 = the library's
-                       choice (2, with 3/4 of the cache-resident batch at 512 x 512); other scans use one stream */
+  int streams;      /* scan, one potential at a time, fast passes, probe inside the bandlimit: the parts of each
+                       batch run their slice loops on this many streams (1-4; with use_graph, each batch's graph
+                       forks into the parts); 0 = the library's choice (2, with 3/4 of the cache-resident batch at
+                       512 x 512); the lane pipeline and other scans use one stream */
 } sk_options;
 
 /* --- availability (MetalEngine.require / Multislice.metalIsAvailable, MetalEngine.swift:63-77,

@@ -1745,12 +1745,12 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
         B = L * W;
       }
     }
-    // split scan: the parts of each batch run on their own streams, one potential at a time, without graph replay,
-    // through the fast passes with the probe inside the bandlimit. A timed scan (per-launch events) keeps the batch
-    // and runs it on one stream.
+    // split scan: the parts of each batch run on their own streams, one potential at a time, through the fast
+    // passes with the probe inside the bandlimit (with graph replay, each batch's graph forks into the parts and
+    // joins them). A timed scan (per-launch events) keeps the batch and runs it on one stream.
     const bool graph = opt && opt->use_graph;
     int split = 1;
-    if (!pipe && !graph && fits && fast &&
+    if (!pipe && fits && fast &&
         fast_col_supported(nx, p0.nc, COL_IN_G | COL_FWD | COL_MUL_P | COL_INV | COL_OUT)) {
       const int want = opt ? opt->streams : 0;
       split = want > 0 ? want : kSplitStreams;
@@ -1807,11 +1807,8 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
       slice_loop(P, nb, G, false, w.full.p, fits ? 0 : 1, plane_out, nullptr, &fin, fast, t, ss);
       launch_reduce_partials(dph, nb, b0, pos_dev, w.acc.p, ss);
     };
-    if (pipe) {
-      run_pipeline(pots, npots, npos, W, L, g, dp, graph, tm, timing);
-    } else if (split > 1) {
-      // k parts of each batch, each through its own slice loop on its own stream
-      const int k = split;
+    // the streams of a split scan's parts: the scan's own, then side streams created on first use
+    auto part_streams = [&](int k) {
       while ((int)c.side.size() < k - 1) {
         hipStream_t t;
         SK_HIP(hipStreamCreateWithFlags(&t, hipStreamNonBlocking));
@@ -1819,6 +1816,14 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
       }
       std::vector<hipStream_t> st(1, s);
       for (int q = 0; q < k - 1; ++q) st.push_back(c.side[q]);
+      return st;
+    };
+    if (pipe) {
+      run_pipeline(pots, npots, npos, W, L, g, dp, graph, tm, timing);
+    } else if (split > 1 && !graph) {
+      // k parts of each batch, each through its own slice loop on its own stream
+      const int k = split;
+      const std::vector<hipStream_t> st = part_streams(k);
       hipEvent_t fork;
       SK_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
       SK_HIP(hipEventRecord(fork, s));
@@ -1857,6 +1862,20 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
       for (int i = 0; i < nbatches; ++i) host_first[i] = i * B;
       w.pos_dev.ensure(1);
       Timer off;
+      // the fork and join events of split graphs (the capture records them as dependencies between the streams)
+      struct Events {
+        std::vector<hipEvent_t> e;
+        ~Events() {
+          for (auto x : e) (void)hipEventDestroy(x);
+        }
+      } evs;
+      for (int q = 0; q < split; ++q) {
+        hipEvent_t x;
+        SK_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+        evs.e.push_back(x);
+      }
+      hipEvent_t fork_ev = evs.e[0];
+      hipEvent_t* join_ev = evs.e.data() + 1;
       try {
         for (int k = 0; k < npots; ++k) {
           hipGraphExec_t exec[2] = {nullptr, nullptr};
@@ -1867,7 +1886,21 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
               hipGraph_t gr;
               SK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
               try {
-                enqueue_batch(*pots[k], 0, w.pos_dev.p, nb, off);
+                if (split > 1) {
+                  // the graph forks into the batch's parts, one per stream, and joins them at its end
+                  const std::vector<hipStream_t> st = part_streams(split);
+                  const int H = (nb + split - 1) / split;
+                  SK_HIP(hipEventRecord(fork_ev, s));
+                  for (int q = 1; q < split; ++q) SK_HIP(hipStreamWaitEvent(st[q], fork_ev, 0));
+                  for (int q = 0; q < split && q * H < nb; ++q)
+                    enqueue_batch(*pots[k], q * H, w.pos_dev.p, std::min(H, nb - q * H), off, st[q], q * H);
+                  for (int q = 1; q < split; ++q) {
+                    SK_HIP(hipEventRecord(join_ev[q - 1], st[q]));
+                    SK_HIP(hipStreamWaitEvent(s, join_ev[q - 1], 0));
+                  }
+                } else {
+                  enqueue_batch(*pots[k], 0, w.pos_dev.p, nb, off);
+                }
               } catch (...) {
                 hipGraph_t dropped;
                 (void)hipStreamEndCapture(s, &dropped);

@@ -267,11 +267,13 @@ def test_scan_is_deterministic_and_batch_independent():
     for x, y, z in zip(a, b, c):
         assert np.array_equal(x.values, y.values)
         assert np.array_equal(x.values, z.values)
-    # the slice loop replayed from captured hipGraphs (a full-batch graph and a remainder graph) gives the same bits
-    for batch in (64, 5):
-        g = sk.STEM.scan(probe, [prepared], scan, dets, sk.ExecutionOptions(batch=batch, use_graph=True))
+    # the slice loop replayed from captured hipGraphs (a full-batch graph and a remainder graph) gives the same bits,
+    # on one stream and with each batch's graph forked into parts on 2 (the default) and 3 streams
+    for batch, streams in ((64, 0), (5, 0), (64, 1), (5, 3)):
+        g = sk.STEM.scan(probe, [prepared], scan, dets,
+                         sk.ExecutionOptions(batch=batch, use_graph=True, streams=streams))
         for x, y in zip(a, g):
-            assert np.array_equal(x.values, y.values), batch
+            assert np.array_equal(x.values, y.values), (batch, streams)
 
 
 def test_lane_pipeline_single_position_over_many_potentials():
