@@ -86,8 +86,9 @@ typedef struct sk_options {
                        above 13) */
   int streams;      /* scan, one potential at a time, fast passes, probe inside the bandlimit: the parts of each
                        batch run their slice loops on this many streams (1-4; with use_graph, each batch's graph
-                       forks into the parts); 0 = the library's choice (2, with 3/4 of the cache-resident batch at
-                       512 x 512); the lane pipeline and other scans use one stream */
+                       forks into the parts); 0 = the library's choice (2 for batches of at least 128 positions,
+                       with 3/4 of the cache-resident batch at 512 x 512, else 1); the lane pipeline and other
+                       scans use one stream */
 } sk_options;
 
 /* --- availability (MetalEngine.require / Multislice.metalIsAvailable, MetalEngine.swift:63-77,

@@ -215,7 +215,7 @@ class ExecutionOptions:
     col_tile: int = 0        # 512-point columns: columns per column-pass block, 8 or 16 (0 = the library's choice)
     precision: str = "float32"  # the slice loop's transforms: "float32", or "split_bf16" (matrix cores)
     streams: int = 0         # scan one potential at a time through the fast passes: streams the parts of each batch
-                             # run on (1-4), 0 the library's choice (2)
+                             # run on (1-4), 0 the library's choice (2 for batches of at least 128 positions)
 
 
 @dataclass(frozen=True)

@@ -507,8 +507,12 @@ Workspace& ws() {
 // one part's row pass fills the tail of the other's column pass. Measured on the HAADF scan (4096 positions), two
 // streams against one: 256 x 256 at 683 positions 4.30 M against 4.01 M position-slices/s; 512 x 512 at 128
 // positions 963-989 k against 945-960 k at 171 positions for one stream, and 954-983 k at 171 for two. Three
-// streams gain less, four lose. At 512 x 512 the split scan takes 3/4 of the cache-resident batch.
+// streams gain less, four lose. At 512 x 512 the split scan takes 3/4 of the cache-resident batch. Parts of fewer
+// than 64 positions lose instead (the launches get too short): over 16 frozen-phonon configurations at 512 x 512,
+// 24-48 positions run 2-5 % slower on two streams than on one, so the library splits batches of at least
+// kMinSplitBatch positions only.
 constexpr int kSplitStreams = 2;
+constexpr int kMinSplitBatch = 128;
 constexpr int kMaxStreams = 4;
 
 void set_col_tile(Context& c, const sk_options* opt) {
@@ -1753,9 +1757,15 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
     if (!pipe && fits && fast &&
         fast_col_supported(nx, p0.nc, COL_IN_G | COL_FWD | COL_MUL_P | COL_INV | COL_OUT)) {
       const int want = opt ? opt->streams : 0;
-      split = want > 0 ? want : kSplitStreams;
-      if (want == 0 && split > 1 && !(opt && opt->batch > 0) && nx == 512 && ny == 512)
-        B = default_batch(p0, npos, opt, 0.75);
+      if (want > 0) {
+        split = want;
+      } else {
+        const int Bs = (!(opt && opt->batch > 0) && nx == 512 && ny == 512) ? default_batch(p0, npos, opt, 0.75) : B;
+        if (Bs >= kMinSplitBatch) {
+          split = kSplitStreams;
+          B = Bs;
+        }
+      }
       split = std::max(1, std::min(split, B));
       if (c.timing) split = 1;
     }
