@@ -89,7 +89,7 @@ extern "C" int sk_hip_autotune(const sk_potential* const* pots, int npots, const
       SK_HIP(hipMalloc(&outs.ptrs[d], sizeof(double) * (size_t)npos * std::max(1, count)));
     }
     const std::vector<double> pos = spread_positions(g, npos);
-    // candidates: positions per launch x column tile width (512-point columns) x graph replay x lane pipeline
+    // candidates: positions per launch x column tile width (512-point columns) x graph replay x lane pipeline x streams
     size_t free_b = 0, total_b = 0;
     SK_HIP(hipMemGetInfo(&free_b, &total_b));
     const double wave_bytes = 8.0 * g.nx * g.ny;
@@ -102,19 +102,23 @@ extern "C" int sk_hip_autotune(const sk_potential* const* pots, int npots, const
         batches.push_back(b);
     std::vector<int> tiles = g.nx == 512 ? std::vector<int>{8, 16} : std::vector<int>{0};
     std::vector<int> pipes = npots > 1 ? std::vector<int>{0, 1} : std::vector<int>{0};
+    // the library's default batch also runs on one stream (streams = 1) beside its own choice of streams
     std::vector<sk_options> cand;
     for (int b : batches)
       for (int t : tiles)
         for (int gr : {0, 1})
-          for (int pp : pipes) {
-            sk_options o{};
-            o.batch = b;
-            o.use_graph = gr;
-            o.fast_path = 1;
-            o.pipeline = pp;
-            o.col_tile = t;
-            if ((int)cand.size() < SK_TUNE_MAX) cand.push_back(o);
-          }
+          for (int pp : pipes)
+            for (int st : {0, 1}) {
+              if (st == 1 && (b != 0 || pp != 0)) continue;
+              sk_options o{};
+              o.batch = b;
+              o.use_graph = gr;
+              o.fast_path = 1;
+              o.pipeline = pp;
+              o.col_tile = t;
+              o.streams = st;
+              if ((int)cand.size() < SK_TUNE_MAX) cand.push_back(o);
+            }
     const int reps = 2;
     int best = -1;
     for (size_t i = 0; i < cand.size(); ++i) {

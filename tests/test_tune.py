@@ -67,6 +67,8 @@ def test_autotune_picks_its_fastest_plan_and_scans_the_same_numbers(tmp_path):
     assert len(res.candidates) >= 2
     assert res.rate == max(c["rate"] for c in res.candidates)
     assert res.options.batch in (0, 1024)  # 0: the library's cache-sized batch (two launches of 512 here)
+    # the default batch is timed split over the library's streams and on one stream
+    assert {c["streams"] for c in res.candidates if c["batch"] == 0} == {0, 1}
     assert res.device.startswith("gfx950")
     again = sk.autotune(probe, [pot], dets, positions=1024, cache=cache)
     assert again.from_cache and again.options == res.options
