@@ -94,6 +94,32 @@ __device__ inline void detect_pixel(const DetPlan& d, int ix, int iy, double I, 
   }
 }
 
+// DFT of length 3: with w = e^{sign 2 pi i / 3} = -1/2 + sign i sqrt(3)/2, Y1,2 = a - (b + c)/2 +- sign i sqrt(3)/2 (b - c)
+__device__ inline void dft3(float2& a, float2& b, float2& c, int sign) {
+  const float h = 0.866025403784438647f;  // sqrt(3) / 2
+  const float2 s = cadd(b, c);
+  const float2 t = mul_si(csub(b, c), sign);
+  const float2 m = make_float2(a.x - 0.5f * s.x, a.y - 0.5f * s.y);
+  a = cadd(a, s);
+  b = make_float2(m.x + h * t.x, m.y + h * t.y);
+  c = make_float2(m.x - h * t.x, m.y - h * t.y);
+}
+
+// DFT of length 6 by the prime-factor map of 6 = 2 x 3 (Good-Thomas, no twiddles): input n = (3 n1 + 2 n2) mod 6,
+// output k = (3 k1 + 4 k2) mod 6, so that w6^{n k} = w2^{n1 k1} w3^{n2 k2}: two DFT3s over n2, then three DFT2s over n1.
+__device__ inline void dft6(float2 (&u)[6], int sign) {
+  float2 a0 = u[0], a1 = u[2], a2 = u[4];  // n1 = 0: n = 0, 2, 4
+  float2 b0 = u[3], b1 = u[5], b2 = u[1];  // n1 = 1: n = 3, 5, 1
+  dft3(a0, a1, a2, sign);
+  dft3(b0, b1, b2, sign);
+  u[0] = cadd(a0, b0);  // k2 = 0: k = 0, 3
+  u[3] = csub(a0, b0);
+  u[4] = cadd(a1, b1);  // k2 = 1: k = 4, 1
+  u[1] = csub(a1, b1);
+  u[2] = cadd(a2, b2);  // k2 = 2: k = 2, 5
+  u[5] = csub(a2, b2);
+}
+
 template <int R>
 __device__ inline void dft_pow2(float2 (&u)[R], int sign) {
   if constexpr (R == 2) {

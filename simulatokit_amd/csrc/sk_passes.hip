@@ -57,6 +57,8 @@ template <int R>
 __device__ inline void dft_small(float2 (&u)[R], int sign, const float2* roots, int m) {
   if constexpr (R == 2 || R == 4 || R == 8) {
     dft_pow2<R>(u, sign);
+  } else if constexpr (R == 6) {
+    dft6(u, sign);
   } else {
     // direct DFT of an odd prime radix with the line's roots: e^{-2 pi i j q / R} = roots[(j q mod R) m / R]
     float2 v[R];
@@ -181,9 +183,10 @@ __device__ __forceinline__ void stockham_stage_fixed(const float2* __restrict__ 
   }
 }
 
-// Compile-time stage lists of the fixed lengths: 384 = 8 8 2 3 (a grid length of the precision study) and 1024 =
-// 8 8 8 2 (Bluestein's length for grid lengths 257..512, 509 among them, and the grid length 1024). The plan's
-// radices for these lengths are the same.
+// Compile-time stage lists of the fixed lengths: 384 = 8 8 6 (a grid length of the precision study; the radix-6
+// stage is the prime-factor 2 x 3 DFT, one LDS pass where radix 2 then 3 take two) and 1024 = 8 8 8 2 (Bluestein's
+// length for grid lengths 257..512, 509 among them, and the grid length 1024). Any order of radices uses the same
+// root table of the length.
 template <int M>
 __device__ float2* lds_stockham_fixed(float2* a, float2* b, const float2* roots, int sign, int tid) {
   constexpr int LINES = kGenericLdsComplex / M;
@@ -192,11 +195,9 @@ __device__ float2* lds_stockham_fixed(float2* a, float2* b, const float2* roots,
     __syncthreads();
     stockham_stage_fixed<8, 8, M, LINES>(b, a, roots, sign, tid);
     __syncthreads();
-    stockham_stage_fixed<2, 64, M, LINES>(a, b, roots, sign, tid);
+    stockham_stage_fixed<6, 64, M, LINES>(a, b, roots, sign, tid);
     __syncthreads();
-    stockham_stage_fixed<3, 128, M, LINES>(b, a, roots, sign, tid);
-    __syncthreads();
-    return a;
+    return b;
   } else {
     static_assert(M == 1024, "fixed lengths 384 and 1024");
     stockham_stage_fixed<8, 1, M, LINES>(a, b, roots, sign, tid);
@@ -461,15 +462,7 @@ int transform_kind(const LineFFT& p) {
     fail(SK_ERR_INVALID, "split-BF16 plan of unsupported length " + std::to_string(p.m));
   return p.m / 16;
 }
-int fixed_length(const LineFFT& p) {
-  if (p.use_mfma || (p.m != 384 && p.m != 1024)) return 0;
-  // the compile-time stage lists must be the plan's: 384 = 8 8 2 3, 1024 = 8 8 8 2
-  const int want[2][4] = {{8, 8, 2, 3}, {8, 8, 8, 2}};
-  const int* w = want[p.m == 384 ? 0 : 1];
-  if (p.nst != 4 || p.radix[0] != w[0] || p.radix[1] != w[1] || p.radix[2] != w[2] || p.radix[3] != w[3])
-    fail(SK_ERR_INVALID, "plan radices differ from the compile-time stages of length " + std::to_string(p.m));
-  return p.m;
-}
+int fixed_length(const LineFFT& p) { return (!p.use_mfma && (p.m == 384 || p.m == 1024)) ? p.m : 0; }
 
 // The kernel of the plan's variant: Pick<MF, FM>::kernel() for the plan's MF and FM.
 template <template <int, int> class Pick>
