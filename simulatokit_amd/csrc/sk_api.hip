@@ -1773,7 +1773,8 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
     w.rx.ensure((size_t)B * nx);
     w.ry.ensure((size_t)B * ny);
     if (!fits) w.full.ensure((size_t)B * N);
-    int bpw = (p0.nc + col_generic_lines(line_plan(nx, s).dev.m) - 1) / col_generic_lines(line_plan(nx, s).dev.m);
+    const int gcols = col_generic_lines(slice_plan(nx, s));
+    int bpw = (p0.nc + gcols - 1) / gcols;
     if (fast && fast_col_supported(nx, p0.nc, COL_IN_G | COL_FWD | COL_MUL_P | COL_DETECT))
       bpw = fast_col_blocks_per_wave(nx, p0.nc, pipe);
     dp.blocks_per_wave = bpw;

@@ -251,7 +251,7 @@ __device__ __forceinline__ long long first_position(int pos0, const int* pos_dev
 void launch_row_generic(const RowArgs& a, int batch, hipStream_t s);
 void launch_col_generic(const ColArgs& a, int batch, hipStream_t s);
 void launch_reduce_partials(const DetPlan& d, int batch, int pos0, const int* pos_dev, double* acc, hipStream_t s);
-int col_generic_lines(int m);  // columns per block of the generic column pass
+int col_generic_lines(const LineFFT& p);  // columns per block of the generic column pass of plan p
 
 // fast passes (sk_fast.hip): return false when the length has no fast kernel
 bool fast_row_supported(int nx, int ny, int flags);

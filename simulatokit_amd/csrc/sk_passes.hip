@@ -187,9 +187,8 @@ __device__ __forceinline__ void stockham_stage_fixed(const float2* __restrict__ 
 // stage is the prime-factor 2 x 3 DFT, one LDS pass where radix 2 then 3 take two) and 1024 = 8 8 8 2 (Bluestein's
 // length for grid lengths 257..512, 509 among them, and the grid length 1024). Any order of radices uses the same
 // root table of the length.
-template <int M>
+template <int M, int LINES>
 __device__ float2* lds_stockham_fixed(float2* a, float2* b, const float2* roots, int sign, int tid) {
-  constexpr int LINES = kGenericLdsComplex / M;
   if constexpr (M == 384) {
     stockham_stage_fixed<8, 1, M, LINES>(a, b, roots, sign, tid);
     __syncthreads();
@@ -212,13 +211,13 @@ __device__ float2* lds_stockham_fixed(float2* a, float2* b, const float2* roots,
   }
 }
 
-// Unscaled transform of length p.m of every line: MF = 0, the float32 Stockham stages (FM > 0: of the length FM
-// fixed at compile time, lines = kGenericLdsComplex / FM); MF = n2 > 0, the split-BF16 matrix-core stages of length
-// p.m = 16 n2 (the plan carries their tables). Returns the buffer holding the result.
-template <int MF, int FM>
+// Unscaled transform of length p.m of every line: MF = 0, the float32 Stockham stages (FM > 0: of the length FM and
+// the FL lines fixed at compile time); MF = n2 > 0, the split-BF16 matrix-core stages of length p.m = 16 n2 (the
+// plan carries their tables). Returns the buffer holding the result.
+template <int MF, int FM, int FL>
 __device__ float2* lds_length_m(float2* a, float2* b, int lines, int ls, const LineFFT& p, int sign, int tid, int T) {
   if constexpr (MF == 0 && FM > 0) {
-    return lds_stockham_fixed<FM>(a, b, p.roots, sign, tid);
+    return lds_stockham_fixed<FM, FL>(a, b, p.roots, sign, tid);
   } else if constexpr (MF == 0) {
     return lds_stockham(a, b, lines, ls, p, sign, tid, T);
   } else {
@@ -230,10 +229,10 @@ __device__ float2* lds_length_m(float2* a, float2* b, int lines, int ls, const L
 // Unscaled transform of logical length p.n of every line (entries [0, n) of each line of stride ls).
 // Bluestein (1970): X_k = c_k sum_j (x_j c_j) conj(c_{k-j}), c_k = e^{sign pi i k^2 / n}, the convolution done with
 // two power-of-two transforms of length m.
-template <int MF, int FM>
+template <int MF, int FM, int FL>
 __device__ float2* lds_line_transform(float2* a, float2* b, int lines, int ls, const LineFFT& p, int sign, int tid,
                                       int T) {
-  if (!p.blue) return lds_length_m<MF, FM>(a, b, lines, ls, p, sign, tid, T);
+  if (!p.blue) return lds_length_m<MF, FM, FL>(a, b, lines, ls, p, sign, tid, T);
   const int n = p.n, m = p.m;
   const Div dm(m), dn(n);
   for (int idx = tid; idx < lines * m; idx += T) {
@@ -248,7 +247,7 @@ __device__ float2* lds_line_transform(float2* a, float2* b, int lines, int ls, c
     a[at<MF>(l * ls + k)] = v;
   }
   __syncthreads();
-  float2* r = lds_length_m<MF, FM>(a, b, lines, ls, p, -1, tid, T);
+  float2* r = lds_length_m<MF, FM, FL>(a, b, lines, ls, p, -1, tid, T);
   for (int idx = tid; idx < lines * m; idx += T) {
     int k;
     const int l = dm.quot(idx, k);
@@ -258,7 +257,7 @@ __device__ float2* lds_line_transform(float2* a, float2* b, int lines, int ls, c
   }
   __syncthreads();
   float2* o = (r == a) ? b : a;
-  r = lds_length_m<MF, FM>(r, o, lines, ls, p, +1, tid, T);
+  r = lds_length_m<MF, FM, FL>(r, o, lines, ls, p, +1, tid, T);
   for (int idx = tid; idx < lines * n; idx += T) {
     int k;
     const int l = dn.quot(idx, k);
@@ -276,7 +275,14 @@ __device__ float2* lds_line_transform(float2* a, float2* b, int lines, int ls, c
 // the float32 kernels keep their registers.
 // ---------------------------------------------------------------------------------------------------------------
 inline int row_lines(int m) { return m >= kGenericLdsComplex ? 1 : kGenericLdsComplex / m; }
-int col_generic_lines(int m) { return m >= kGenericLdsComplex ? 1 : kGenericLdsComplex / m; }
+// Columns per block of the generic column pass. The float32 384-point kernel takes 8 (64-byte row segments of the
+// compact wave, 52 KiB of LDS) where the 2048-point rule gives 5 (40-byte segments).
+constexpr int kCols384 = 8;
+constexpr int fixed_col_lines(int fm) { return fm == 384 ? kCols384 : fm > 0 ? kGenericLdsComplex / fm : 0; }
+int col_generic_lines(const LineFFT& p) {
+  if (!p.use_mfma && p.m == 384) return kCols384;
+  return p.m >= kGenericLdsComplex ? 1 : kGenericLdsComplex / p.m;
+}
 
 template <int MF, int FM>
 __global__ void __launch_bounds__(kGenericThreads) row_generic_kernel(RowArgs a, int rows) {
@@ -309,7 +315,7 @@ __global__ void __launch_bounds__(kGenericThreads) row_generic_kernel(RowArgs a,
   float2* cur = A;
   float2* oth = B;
   if (f & ROW_INV) {
-    cur = lds_line_transform<MF, FM>(cur, oth, rows, ls, a.fy, +1, tid, T);
+    cur = lds_line_transform<MF, FM, (FM > 0 ? kGenericLdsComplex / FM : 0)>(cur, oth, rows, ls, a.fy, +1, tid, T);
     oth = (cur == A) ? B : A;
   }
   if (f & (ROW_WRITE_PLANE | ROW_MUL_T)) {
@@ -323,7 +329,7 @@ __global__ void __launch_bounds__(kGenericThreads) row_generic_kernel(RowArgs a,
     }
     __syncthreads();
   }
-  if (f & ROW_FWD) cur = lds_line_transform<MF, FM>(cur, oth, rows, ls, a.fy, -1, tid, T);
+  if (f & ROW_FWD) cur = lds_line_transform<MF, FM, (FM > 0 ? kGenericLdsComplex / FM : 0)>(cur, oth, rows, ls, a.fy, -1, tid, T);
   if (f & ROW_OUT_COMPACT) {
     for (int idx = tid; idx < rows * a.nc; idx += T) {
       int c;
@@ -396,7 +402,7 @@ __global__ void __launch_bounds__(kGenericThreads) col_generic_kernel(ColArgs a,
   float2* cur = A;
   float2* oth = B;
   if (f & COL_FWD) {
-    cur = lds_line_transform<MF, FM>(cur, oth, cols, ls, a.fx, -1, tid, T);
+    cur = lds_line_transform<MF, FM, fixed_col_lines(FM)>(cur, oth, cols, ls, a.fx, -1, tid, T);
     oth = (cur == A) ? B : A;
   }
   if (f & COL_MUL_P) {
@@ -429,7 +435,7 @@ __global__ void __launch_bounds__(kGenericThreads) col_generic_kernel(ColArgs a,
                        a.det.blocks_per_wave);
     return;
   }
-  if (f & COL_INV) cur = lds_line_transform<MF, FM>(cur, oth, cols, ls, a.fx, +1, tid, T);
+  if (f & COL_INV) cur = lds_line_transform<MF, FM, fixed_col_lines(FM)>(cur, oth, cols, ls, a.fx, +1, tid, T);
   if (f & COL_OUT) {
     for (int idx = tid; idx < cols * a.nx; idx += T) {
       int l;
@@ -497,7 +503,7 @@ void launch_row_generic(const RowArgs& a, int batch, hipStream_t s) {
 }
 
 void launch_col_generic(const ColArgs& a, int batch, hipStream_t s) {
-  const int cols = col_generic_lines(a.fx.m);
+  const int cols = col_generic_lines(a.fx);
   dim3 grid((a.nc + cols - 1) / cols, batch);
   const size_t buf = sizeof(float2) * lds_entries(cols * a.fx.m, !a.fx.use_mfma);
   size_t lds = 2 * buf;
