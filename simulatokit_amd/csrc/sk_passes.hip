@@ -155,9 +155,15 @@ __device__ float2* lds_stockham(float2* a, float2* b, int lines, int ls, const L
 
 // The same pass with the radix, sub-length, length and line count fixed at compile time (line stride M), so every
 // index folds to shifts and constant multiplies.
-template <int R, int L, int M, int LINES>
+// Pointwise work fused into a transform's first reads or last writes: hook(e, v) for element e of the line.
+struct NoHook {
+  __device__ float2 operator()(int, float2 v) const { return v; }
+};
+
+template <int R, int L, int M, int LINES, class Pre = NoHook, class Post = NoHook>
 __device__ __forceinline__ void stockham_stage_fixed(const float2* __restrict__ src, float2* __restrict__ dst,
-                                                     const float2* __restrict__ roots, int sign, int tid) {
+                                                     const float2* __restrict__ roots, int sign, int tid,
+                                                     Pre pre = {}, Post post = {}) {
   constexpr int stride = M / R, step = M / (L * R), total = LINES * stride;
   for (int w = tid; w < total; w += kGenericThreads) {
     const int line = w / stride, i = w - line * stride;
@@ -166,7 +172,7 @@ __device__ __forceinline__ void stockham_stage_fixed(const float2* __restrict__ 
     float2 u[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-      float2 v = src[pad16(lb + i + j * stride)];
+      float2 v = pre(i + j * stride, src[pad16(lb + i + j * stride)]);
       if constexpr (L > 1) {
         if (j > 0) {
           float2 tw = roots[j * k * step];  // roots[0] = 1 where k = 0
@@ -177,35 +183,49 @@ __device__ __forceinline__ void stockham_stage_fixed(const float2* __restrict__ 
       u[j] = v;
     }
     dft_small<R>(u, sign, roots, M);
-    const int base = lb + (i - k) * R + k;
+    const int e0 = (i - k) * R + k;
 #pragma unroll
-    for (int j = 0; j < R; ++j) dst[pad16(base + j * L)] = u[j];
+    for (int j = 0; j < R; ++j) dst[pad16(lb + e0 + j * L)] = post(e0 + j * L, u[j]);
   }
 }
 
 // Compile-time stage lists of the fixed lengths: 384 = 8 8 6 (a grid length of the precision study; the radix-6
-// stage is the prime-factor 2 x 3 DFT, one LDS pass where radix 2 then 3 take two) and 1024 = 8 8 8 2 (Bluestein's
-// length for grid lengths 257..512, 509 among them, and the grid length 1024). Any order of radices uses the same
-// root table of the length.
-template <int M, int LINES>
-__device__ float2* lds_stockham_fixed(float2* a, float2* b, const float2* roots, int sign, int tid) {
-  if constexpr (M == 384) {
-    stockham_stage_fixed<8, 1, M, LINES>(a, b, roots, sign, tid);
+// stage is the prime-factor 2 x 3 DFT, one LDS pass where radix 2 then 3 take two); the Bluestein lengths of the
+// grid lengths 65..1024, 256 = 8 8 4, 512 = 8 8 8, 1024 = 8 8 8 2 (509 among them) and 2048 = 8 8 8 4, which are
+// also the grid lengths 256, 512 and 1024 where the generic passes run them. Any order of radices uses the same root
+// table of the length.
+// `pre` applies to the first stage's reads, `post` to the last stage's writes.
+template <int M, int LINES, class Pre = NoHook, class Post = NoHook>
+__device__ float2* lds_stockham_fixed(float2* a, float2* b, const float2* roots, int sign, int tid, Pre pre = {},
+                                      Post post = {}) {
+  if constexpr (M == 384 || M == 256 || M == 512) {
+    constexpr int R3 = M / 64;  // 6, 4 or 8
+    stockham_stage_fixed<8, 1, M, LINES>(a, b, roots, sign, tid, pre);
     __syncthreads();
     stockham_stage_fixed<8, 8, M, LINES>(b, a, roots, sign, tid);
     __syncthreads();
-    stockham_stage_fixed<6, 64, M, LINES>(a, b, roots, sign, tid);
+    stockham_stage_fixed<R3, 64, M, LINES>(a, b, roots, sign, tid, NoHook{}, post);
     __syncthreads();
     return b;
-  } else {
-    static_assert(M == 1024, "fixed lengths 384 and 1024");
-    stockham_stage_fixed<8, 1, M, LINES>(a, b, roots, sign, tid);
+  } else if constexpr (M == 2048) {
+    stockham_stage_fixed<8, 1, M, LINES>(a, b, roots, sign, tid, pre);
     __syncthreads();
     stockham_stage_fixed<8, 8, M, LINES>(b, a, roots, sign, tid);
     __syncthreads();
     stockham_stage_fixed<8, 64, M, LINES>(a, b, roots, sign, tid);
     __syncthreads();
-    stockham_stage_fixed<2, 512, M, LINES>(b, a, roots, sign, tid);
+    stockham_stage_fixed<4, 512, M, LINES>(b, a, roots, sign, tid, NoHook{}, post);
+    __syncthreads();
+    return a;
+  } else {
+    static_assert(M == 1024, "fixed lengths 256, 384, 512, 1024 and 2048");
+    stockham_stage_fixed<8, 1, M, LINES>(a, b, roots, sign, tid, pre);
+    __syncthreads();
+    stockham_stage_fixed<8, 8, M, LINES>(b, a, roots, sign, tid);
+    __syncthreads();
+    stockham_stage_fixed<8, 64, M, LINES>(a, b, roots, sign, tid);
+    __syncthreads();
+    stockham_stage_fixed<2, 512, M, LINES>(b, a, roots, sign, tid, NoHook{}, post);
     __syncthreads();
     return a;
   }
@@ -233,6 +253,33 @@ template <int MF, int FM, int FL>
 __device__ float2* lds_line_transform(float2* a, float2* b, int lines, int ls, const LineFFT& p, int sign, int tid,
                                       int T) {
   if (!p.blue) return lds_length_m<MF, FM, FL>(a, b, lines, ls, p, sign, tid, T);
+  if constexpr (MF == 0 && FM > 0) {
+    // fixed-length Bluestein: the chirp product fused into the forward transform's first reads (zero past n), the
+    // kernel product into its last writes, the closing chirp product into the inverse transform's last writes
+    const int n = p.n;
+    const float2* chirp = p.chirp;
+    const float2* khat = p.khat;
+    auto first = [=](int e, float2 v) {
+      if (e >= n) return make_float2(0.f, 0.f);
+      float2 c = chirp[e];
+      if (sign > 0) c.y = -c.y;
+      return cmul(v, c);
+    };
+    auto kernel = [=](int e, float2 v) {
+      float2 kh = khat[e];
+      if (sign > 0) kh.y = -kh.y;
+      return cmul(v, kh);
+    };
+    auto last = [=](int e, float2 v) {
+      if (e >= n) return v;
+      float2 c = chirp[e];
+      if (sign > 0) c.y = -c.y;
+      return cmul(v, c);
+    };
+    float2* r = lds_stockham_fixed<FM, FL>(a, b, p.roots, -1, tid, first, kernel);
+    float2* o = (r == a) ? b : a;
+    return lds_stockham_fixed<FM, FL>(r, o, p.roots, +1, tid, NoHook{}, last);
+  }
   const int n = p.n, m = p.m;
   const Div dm(m), dn(n);
   for (int idx = tid; idx < lines * m; idx += T) {
@@ -461,23 +508,34 @@ __global__ void reduce_partials_kernel(DetPlan d, int pos0, const int* pos_dev, 
 // Launchers
 // ---------------------------------------------------------------------------------------------------------------
 // The kernel variant of a plan's line transform: MF = 0 float32, else n2 = m / 16 of the split-BF16 stages; FM = the
-// float32 lengths with compile-time stages (384, 1024), else 0.
+// float32 lengths with compile-time stages (256, 384, 512, 1024, 2048), else 0.
 int transform_kind(const LineFFT& p) {
   if (!p.use_mfma) return 0;
   if (p.m != 256 && p.m != 384 && p.m != 512 && p.m != 1024)
     fail(SK_ERR_INVALID, "split-BF16 plan of unsupported length " + std::to_string(p.m));
   return p.m / 16;
 }
-int fixed_length(const LineFFT& p) { return (!p.use_mfma && (p.m == 384 || p.m == 1024)) ? p.m : 0; }
+int fixed_length(const LineFFT& p) {
+  if (p.use_mfma) return 0;
+  switch (p.m) {
+    case 256: case 384: case 512: case 1024: case 2048: return p.m;
+    default: return 0;
+  }
+}
 
 // The kernel of the plan's variant: Pick<MF, FM>::kernel() for the plan's MF and FM.
 template <template <int, int> class Pick>
 auto generic_variant(const LineFFT& p) {
   switch (transform_kind(p)) {
-    case 0: {
-      const int fm = fixed_length(p);
-      return fm == 384 ? Pick<0, 384>::kernel() : fm == 1024 ? Pick<0, 1024>::kernel() : Pick<0, 0>::kernel();
-    }
+    case 0:
+      switch (fixed_length(p)) {
+        case 256: return Pick<0, 256>::kernel();
+        case 384: return Pick<0, 384>::kernel();
+        case 512: return Pick<0, 512>::kernel();
+        case 1024: return Pick<0, 1024>::kernel();
+        case 2048: return Pick<0, 2048>::kernel();
+        default: return Pick<0, 0>::kernel();
+      }
     case 16: return Pick<16, 0>::kernel();
     case 24: return Pick<24, 0>::kernel();
     case 32: return Pick<32, 0>::kernel();

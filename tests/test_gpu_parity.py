@@ -117,10 +117,12 @@ def test_odd_grid_and_tilt_equal_abtem(run):
     assert rel_max(w.field, S4.array(run)) <= ACC.wave
 
 
-def test_bluestein_grid_equals_the_oracle():
-    # 67 x 53: both lengths prime, so both axes take Bluestein's algorithm on the device
+@pytest.mark.parametrize("nx,ny", [(67, 53), (131, 137), (1021, 1019)])
+def test_bluestein_grid_equals_the_oracle(nx, ny):
+    # both lengths prime, so both axes take Bluestein's algorithm on the device: convolution lengths 256 and 128
+    # (67 x 53), 512 (131 x 137) and 2048 (1021 x 1019), at about 0.14 A sampling
     rng = np.random.default_rng(5)
-    nx, ny, L = 67, 53, (9.0, 7.5)
+    L = (nx * 9.0 / 67, ny * 7.5 / 53)
     numbers = rng.choice([8, 14, 38], 30)
     xyz = rng.uniform(0, 1, (30, 3)) * np.array([L[0], L[1], 6.0])
     st = sk.AtomicStructure((L[0], L[1], 6.0), numbers, xyz)
