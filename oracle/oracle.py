@@ -1,7 +1,7 @@
 """ctypes binding of the float64 CPU oracle (oracle/simulato_oracle.cpp). TEST INFRASTRUCTURE ONLY.
 
-Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module, as the checker of the HIP
-library; the product package never imports it. Arrays are NumPy, row-major ``[ix, iy]``; complex arrays are
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module, as the float64 reference
+the HIP library is tested against; the product package never imports it. Arrays are NumPy, row-major ``[ix, iy]``; complex arrays are
 complex128. Each wrapper names the reference function it restates (paths relative to the SimulatoKit checkout).
 """
 from __future__ import annotations

@@ -1,7 +1,7 @@
 // simulato_oracle.cpp — TEST INFRASTRUCTURE ONLY.
 //
-// A float64 CPU restatement of SimulatoKit's multislice STEM path (the reference's `.cpu` backend), used as the
-// checker of the HIP library: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it. The
+// A float64 CPU restatement of SimulatoKit's multislice STEM path (the reference's `.cpu` backend), against which the
+// tests hold the HIP library: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it. The
 // product library never links, loads or calls it.
 //
 // Every function cites the reference file:line it follows (paths relative to the SimulatoKit checkout). The
