@@ -323,11 +323,16 @@ __device__ float2* lds_line_transform(float2* a, float2* b, int lines, int ls, c
 // ---------------------------------------------------------------------------------------------------------------
 inline int row_lines(int m) { return m >= kGenericLdsComplex ? 1 : kGenericLdsComplex / m; }
 // Columns per block of the generic column pass. The float32 384-point kernel takes 8 (64-byte row segments of the
-// compact wave, 52 KiB of LDS) where the 2048-point rule gives 5 (40-byte segments).
+// compact wave, 52 KiB of LDS) where the 2048-point rule gives 5 (40-byte segments), the 1024-point one 3 (52 KiB)
+// where the rule gives 2.
 constexpr int kCols384 = 8;
-constexpr int fixed_col_lines(int fm) { return fm == 384 ? kCols384 : fm > 0 ? kGenericLdsComplex / fm : 0; }
+constexpr int kCols1024 = 3;
+constexpr int fixed_col_lines(int fm) {
+  return fm == 384 ? kCols384 : fm == 1024 ? kCols1024 : fm > 0 ? kGenericLdsComplex / fm : 0;
+}
 int col_generic_lines(const LineFFT& p) {
   if (!p.use_mfma && p.m == 384) return kCols384;
+  if (!p.use_mfma && p.m == 1024) return kCols1024;
   return p.m >= kGenericLdsComplex ? 1 : kGenericLdsComplex / p.m;
 }
 
