@@ -120,14 +120,52 @@ __device__ inline void dft6(float2 (&u)[6], int sign) {
   u[5] = csub(a2, b2);
 }
 
+// DFT of length 16 as 4 x 4 (Cooley-Tukey): n = 4 n1 + n2, k = k1 + 4 k2; DFT4 over n1 for each n2, the twiddle
+// w16^{n2 k1} (w16 = e^{sign 2 pi i / 16}), then DFT4 over n2 for each k1.
+__device__ inline void dft16(float2 (&u)[16], int sign) {
+  const float c1 = 0.923879532511286756f, s1 = 0.382683432365089772f, r2 = 0.707106781186547524f;
+  float2 y[4][4];  // y[n2][k1]
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) {
+    float2 a0 = u[n2], a1 = u[4 + n2], a2 = u[8 + n2], a3 = u[12 + n2];
+    dft4(a0, a1, a2, a3, sign);
+    y[n2][0] = a0;
+    y[n2][1] = a1;
+    y[n2][2] = a2;
+    y[n2][3] = a3;
+  }
+  const float sg = sign < 0 ? -1.f : 1.f;
+  // w16^m for m = n2 k1 in {1, 2, 3, 4, 6, 9}: (cos(2 pi m / 16), sign sin(2 pi m / 16))
+  y[1][1] = cmul(y[1][1], make_float2(c1, sg * s1));
+  y[1][2] = cmul(y[1][2], make_float2(r2, sg * r2));
+  y[1][3] = cmul(y[1][3], make_float2(s1, sg * c1));
+  y[2][1] = cmul(y[2][1], make_float2(r2, sg * r2));
+  y[2][2] = mul_si(y[2][2], sign);
+  y[2][3] = cmul(y[2][3], make_float2(-r2, sg * r2));
+  y[3][1] = cmul(y[3][1], make_float2(s1, sg * c1));
+  y[3][2] = cmul(y[3][2], make_float2(-r2, sg * r2));
+  y[3][3] = cmul(y[3][3], make_float2(-c1, -sg * s1));
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    float2 b0 = y[0][k1], b1 = y[1][k1], b2 = y[2][k1], b3 = y[3][k1];
+    dft4(b0, b1, b2, b3, sign);
+    u[k1] = b0;
+    u[k1 + 4] = b1;
+    u[k1 + 8] = b2;
+    u[k1 + 12] = b3;
+  }
+}
+
 template <int R>
 __device__ inline void dft_pow2(float2 (&u)[R], int sign) {
   if constexpr (R == 2) {
     dft2(u[0], u[1]);
   } else if constexpr (R == 4) {
     dft4(u[0], u[1], u[2], u[3], sign);
+  } else if constexpr (R == 16) {
+    dft16(u, sign);
   } else {
-    static_assert(R == 8, "radix 2, 4 or 8");
+    static_assert(R == 8, "radix 2, 4, 8 or 16");
     dft8(u, sign);
   }
 }

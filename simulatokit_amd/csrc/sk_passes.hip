@@ -55,7 +55,7 @@ struct Div {
 // ---------------------------------------------------------------------------------------------------------------
 template <int R>
 __device__ inline void dft_small(float2 (&u)[R], int sign, const float2* roots, int m) {
-  if constexpr (R == 2 || R == 4 || R == 8) {
+  if constexpr (R == 2 || R == 4 || R == 8 || R == 16) {
     dft_pow2<R>(u, sign);
   } else if constexpr (R == 6) {
     dft6(u, sign);
@@ -191,9 +191,9 @@ __device__ __forceinline__ void stockham_stage_fixed(const float2* __restrict__ 
 
 // Compile-time stage lists of the fixed lengths: 384 = 8 8 6 (a grid length of the precision study; the radix-6
 // stage is the prime-factor 2 x 3 DFT, one LDS pass where radix 2 then 3 take two); the Bluestein lengths of the
-// grid lengths 65..1024, 256 = 8 8 4, 512 = 8 8 8, 1024 = 8 8 8 2 (509 among them) and 2048 = 8 8 8 4, which are
-// also the grid lengths 256, 512 and 1024 where the generic passes run them. Any order of radices uses the same root
-// table of the length.
+// grid lengths 65..1024, 256 = 8 8 4, 512 = 8 8 8, 1024 = 8 8 16 (509 among them; dft16 as 4 x 4) and
+// 2048 = 8 8 8 4, which are also the grid lengths 256, 512 and 1024 where the generic passes run them. Any order of
+// radices uses the same root table of the length.
 // `pre` applies to the first stage's reads, `post` to the last stage's writes.
 template <int M, int LINES, class Pre = NoHook, class Post = NoHook>
 __device__ float2* lds_stockham_fixed(float2* a, float2* b, const float2* roots, int sign, int tid, Pre pre = {},
@@ -223,11 +223,9 @@ __device__ float2* lds_stockham_fixed(float2* a, float2* b, const float2* roots,
     __syncthreads();
     stockham_stage_fixed<8, 8, M, LINES>(b, a, roots, sign, tid);
     __syncthreads();
-    stockham_stage_fixed<8, 64, M, LINES>(a, b, roots, sign, tid);
+    stockham_stage_fixed<16, 64, M, LINES>(a, b, roots, sign, tid, NoHook{}, post);
     __syncthreads();
-    stockham_stage_fixed<2, 512, M, LINES>(b, a, roots, sign, tid, NoHook{}, post);
-    __syncthreads();
-    return a;
+    return b;
   }
 }
 
