@@ -1887,6 +1887,7 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
       }
       hipEvent_t fork_ev = evs.e[0];
       hipEvent_t* join_ev = evs.e.data() + 1;
+      if (split > 1) (void)part_streams(split);  // create the side streams before any capture begins
       try {
         for (int k = 0; k < npots; ++k) {
           hipGraphExec_t exec[2] = {nullptr, nullptr};
