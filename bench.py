@@ -65,7 +65,8 @@ def main():
     ap.add_argument("--col-tile", type=int, default=0, help="columns per 512-point column-pass block (8, 16)")
     ap.add_argument("--streams", type=int, default=0,
                     help="streams each batch is split over (1-4; 0 = the library's choice, 2 for this scan)")
-    ap.add_argument("--cpu-sample", type=int, default=128, help="positions of the float64 CPU baseline sample")
+    ap.add_argument("--cpu-sample", type=int, default=32, help="positions of the float64 CPU baseline sample")
+    ap.add_argument("--cpu-runs", type=int, default=5, help="timed runs of the CPU baseline (after one warm-up)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(8, cores), the reference's default")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="haadf", choices=["haadf", "frozen_phonons"],
@@ -260,28 +261,69 @@ def copy_bandwidth(torch, nbytes: int = 1 << 30, reps: int = 5) -> float:
     return 2.0 * nbytes / (ms * 1e-3) / 1e9
 
 
+def host_cpu():
+    """The host's CPU model, the logical CPUs it reports, and the share this process may use (the affinity mask,
+    capped by OMP_NUM_THREADS when the environment sets it, as a shared GPU box does)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    logical = os.cpu_count() or 1
+    try:
+        share = len(os.sched_getaffinity(0))
+    except AttributeError:
+        share = logical
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        share = min(share, int(omp))
+    return model, logical, share
+
+
 def cpu_baseline(args, w, st, pot, positions, haadf, S):
-    """The float64 oracle (the reference's CPU path restated) on the first `cpu_sample` positions, with the
-    reference's threading (min(8, cores) workers, one position each); also the HIP path's error on them."""
+    """The float64 oracle (the reference's CPU path restated) on the first `cpu_sample` positions of the scan, timed
+    as the reference times its benchmarks (one warm-up, then five runs; median and min-max;
+    Tests/SimulatoKitParityTests/Detector/BenchmarkTableTests.swift:25-33) with the reference's threading
+    (min(8, cores) workers, one position each; MetalMultislice.swift:28-47) and again on every core of the host's
+    share. Only the position loop is timed (the transmission functions are prepared first), and the rate is
+    position-slices per second of the sample, which scales linearly with positions as the reference's own
+    subset benchmarks assume (BenchmarkTests.swift:46-47,68). The same sample gives the HIP path's error."""
     import numpy as np
 
     from oracle import oracle as O
 
-    cores = os.cpu_count() or 1
-    threads = args.cpu_threads or min(8, cores)
+    model, logical, share = host_cpu()
     k = min(args.cpu_sample, len(positions))
     V, _ = O.potential(w.n, w.n, st.numbers, st.positions, st.box, pot.slice_thicknesses)
-    ref = O.stem_scan([V], [pot.slice_thicknesses], (st.box[0], st.box[1]), w.energy, w.semiangle, positions[:k],
-                      [O.det_annular(w.inner, w.outer)], threads=threads)[0]
-    secs = O.last_scan_seconds()
+
+    def runs(threads):
+        secs, ref = [], None
+        for r in range(1 + args.cpu_runs):
+            ref = O.stem_scan([V], [pot.slice_thicknesses], (st.box[0], st.box[1]), w.energy, w.semiangle,
+                              positions[:k], [O.det_annular(w.inner, w.outer)], threads=threads)[0]
+            if r > 0:  # the first run warms up
+                secs.append(O.last_scan_seconds())
+        rates = sorted(k * S / t for t in secs)
+        return {"threads": threads, "median": float(np.median(rates)), "min": rates[0], "max": rates[-1],
+                "runs": len(rates), "seconds_per_run": float(np.median(secs))}, ref
+
+    ref_threads = args.cpu_threads or min(8, share)
+    base, ref = runs(ref_threads)
+    every = runs(share) if share != ref_threads else base
     got = haadf[:k]
     rel_l2 = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
     rel_max = float(np.abs(got - ref).max() / np.abs(ref).max())
-    base = {"value": k * S / secs, "unit": "probe-positions*slices/s", "cores": threads, "kind": "port",
-            "sample": f"{k} of {len(positions)} positions x {S} slices of the same scan (float64 oracle, "
-                      f"transmission functions prepared before the timed loop), {secs:.1f} s on {threads} threads "
-                      f"of {cores} host cores"}
-    return base, {"haadf_rel_l2": rel_l2, "haadf_rel_max": rel_max, "positions": k}
+    out = {"value": base["median"], "unit": "probe-positions*slices/s", "cores": ref_threads, "kind": "port",
+           "sample": f"{k} of {len(positions)} positions x {S} slices of the same scan (float64 oracle, transmission "
+                     f"functions prepared before the timed loop); one warm-up then {args.cpu_runs} runs, median, on "
+                     f"{ref_threads} threads (the reference's min(8, cores)); {base['seconds_per_run']:.1f} s per run",
+           "min": base["min"], "max": base["max"], "runs": base["runs"],
+           "all_cores": every, "cpu_model": model, "host_logical_cpus": logical, "host_cpu_share": share}
+    return out, {"haadf_rel_l2": rel_l2, "haadf_rel_max": rel_max, "positions": k}
 
 
 if __name__ == "__main__":

@@ -95,6 +95,9 @@ typedef struct sk_options {
  *     MetalMultislice.swift:13-21) ------------------------------------------------------------------------- */
 int sk_hip_available(void);
 const char* sk_hip_last_error(void);
+/* Build-only (no counterpart in the reference): the SHA-256 of the library's sources (simulatokit_amd/csrc and
+ * include) that this binary was compiled from, so that a loader can refuse a binary built from other sources. */
+const char* sk_hip_build_id(void);
 
 /* --- potentials --------------------------------------------------------------------------------------------- */
 
@@ -192,6 +195,21 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
  * simulation grid. out_re / out_im: npos x rows x cols, position order kept. */
 int sk_hip_scan_waves(const sk_potential* p, const sk_probe* probe, int npos, const double* positions, int rows,
                       int cols, double* out_re, double* out_im, const sk_options* opt);
+
+/* A caller's detector (any conformer of the open `Detector` protocol, Detectors.swift:84-88) receives the diffraction
+ * intensities |FFT2(psi_exit)|^2 of `count` consecutive positions starting at `first`: count x nx x ny float64 values,
+ * each position's in FFT order [ix * ny + iy] (DiffractionIntensity(grid:beam:values:), Detectors.swift:18-36). The
+ * array is valid during the call only. Return 0 to go on; any other value stops the scan, which then returns
+ * SK_ERR_INVALID. The callback runs on the calling thread and must not call into this library. */
+typedef int (*sk_deliver_fn)(void* user, int first, int count, const double* intensities);
+
+/* The diffraction intensity of every position's probe through one potential, delivered batch by batch in position
+ * order (Multislice.metalScanIntensities -> MetalMultislice.scanIntensities, MetalEngine.swift:583-676). The STEM
+ * driver calls it once per potential and reduces the intensities with the caller's detectors on the host, summed in
+ * potential order (STEM.metalScan / accumulate / average, STEM.swift:120-150); the built-in detector kinds are reduced
+ * on the device by sk_hip_scan instead. The next batch computes while `deliver` reads the current one. */
+int sk_hip_scan_intensities(const sk_potential* p, const sk_probe* probe, int npos, const double* positions,
+                            sk_deliver_fn deliver, void* user, const sk_options* opt);
 
 /* The values each detector returns per position (Detector.shape(for:beam:), Detectors.swift:84-88): writes the
  * shape (up to 2 ints) and returns the count, or a negative status for a refused detector. */

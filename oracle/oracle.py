@@ -102,7 +102,7 @@ def antialias(nx, ny, ex, ey):  # Antialias.swift:18-39
 
 
 # ---- slicing, potential, transmission, propagator ---------------------------------------------------------------
-def slice_atoms(xyz, box, thicknesses):  # Slicing.swift:238-271
+def slice_atoms(xyz, box, thicknesses):  # Slicing.swift:74-104
     xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3).copy()
     box = np.ascontiguousarray(box, dtype=np.float64)
     th = np.ascontiguousarray(thicknesses, dtype=np.float64)

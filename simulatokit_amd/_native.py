@@ -59,6 +59,8 @@ class SkTuneReport(C.Structure):
 
 _lib = None
 _dp = C.POINTER(C.c_double)
+# sk_deliver_fn: (user, first position, count, count x nx x ny float64 intensities) -> 0 to go on
+DELIVER_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double))
 _ip = C.POINTER(C.c_int)
 _vp = C.c_void_p
 
@@ -77,6 +79,13 @@ def lib():
         except ImportError:
             pass
         L = C.CDLL(LIB_PATH)
+        # the binary must have been built from the sources beside it: a stale build fails loudly here
+        from .build import source_hash
+        L.sk_hip_build_id.restype = C.c_char_p
+        built, want = L.sk_hip_build_id().decode(), source_hash()
+        if built != want:
+            raise RuntimeError(f"{LIB_PATH} was built from other sources (build id {built[:16]}, sources "
+                               f"{want[:16]}); rebuild it with `python -m simulatokit_amd.build`")
         L.sk_hip_available.restype = C.c_int
         L.sk_hip_last_error.restype = C.c_char_p
         L.sk_hip_potential_from_values.argtypes = [C.POINTER(SkGrid), C.c_int, _dp, _dp, C.c_double, _dp,
@@ -95,6 +104,8 @@ def lib():
         L.sk_hip_probes.argtypes = [C.POINTER(SkGrid), C.POINTER(SkProbe), C.c_int, _dp, _dp, _dp]
         L.sk_hip_scan.argtypes = [C.POINTER(_vp), C.c_int, C.POINTER(SkProbe), C.c_int, _dp, C.POINTER(SkDetector),
                                   C.c_int, C.POINTER(_vp), C.c_int, C.POINTER(SkOptions)]
+        L.sk_hip_scan_intensities.argtypes = [_vp, C.POINTER(SkProbe), C.c_int, _dp, DELIVER_FN, _vp,
+                                              C.POINTER(SkOptions)]
         L.sk_hip_scan_waves.argtypes = [_vp, C.POINTER(SkProbe), C.c_int, _dp, C.c_int, C.c_int, _dp, _dp,
                                         C.POINTER(SkOptions)]
         L.sk_hip_ctf_kernel.argtypes = [C.POINTER(SkGrid), C.c_double, C.POINTER(SkCtf), _dp, _dp]
@@ -123,11 +134,11 @@ def iptr(a: np.ndarray):
     return a.ctypes.data_as(_ip)
 
 
-EXPORTED = ("sk_hip_available", "sk_hip_last_error", "sk_hip_potential_from_values", "sk_hip_potential_from_atoms",
+EXPORTED = ("sk_hip_available", "sk_hip_last_error", "sk_hip_build_id", "sk_hip_potential_from_values", "sk_hip_potential_from_atoms",
             "sk_hip_potential_release", "sk_hip_potential_slice_count", "sk_hip_potential_values",
             "sk_hip_potential_transmissions", "sk_hip_run", "sk_hip_run_batch", "sk_hip_probes", "sk_hip_scan",
             "sk_hip_detector_shape", "sk_hip_last_timing", "sk_hip_set_timing", "sk_hip_potential_grid",
-            "sk_hip_autotune", "sk_hip_device_name", "sk_hip_potential_from_atoms_finite", "sk_hip_scan_waves",
+            "sk_hip_autotune", "sk_hip_device_name", "sk_hip_potential_from_atoms_finite", "sk_hip_scan_waves", "sk_hip_scan_intensities",
             "sk_hip_ctf_kernel", "sk_hip_apply_ctf",
             # include/simulato_surrogate.h
             "sk_conv5_forward", "sk_conv5_backward_data", "sk_conv5_backward_weights", "sk_relu_backward",

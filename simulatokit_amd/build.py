@@ -6,6 +6,7 @@ Each source compiles to its own object in parallel, then one link.
 """
 from __future__ import annotations
 
+import hashlib
 import os
 import subprocess
 import sys
@@ -15,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "_lib")
 LIB_PATH = os.path.join(LIB_DIR, "libsimulato_hip.so")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 SOURCES = ["sk_passes.hip", "sk_fast.hip", "sk_prepare.hip", "sk_api.hip", "sk_tune.hip", "sk_conv.hip"]
 ARCH = os.environ.get("SK_OFFLOAD_ARCH", "gfx950")
 
@@ -22,6 +24,22 @@ ARCH = os.environ.get("SK_OFFLOAD_ARCH", "gfx950")
 # them from plain float code, and some of their forms have given wrong results without any error.
 HIPCC_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
+
+
+def source_hash() -> str:
+    """SHA-256 over the library's sources: every file of csrc/ and include/, by name and content, in name order.
+    The build embeds it (sk_hip_build_id) and the loader compares it with the sources beside the binary."""
+    h = hashlib.sha256()
+    for d, tag in ((CSRC, "csrc"), (INCLUDE, "include")):
+        for name in sorted(os.listdir(d)):
+            path = os.path.join(d, name)
+            if not os.path.isfile(path) or not name.endswith((".hip", ".h", ".inc")):
+                continue
+            with open(path, "rb") as fh:
+                data = fh.read()
+            h.update(f"{tag}/{name}:{len(data)}:".encode())
+            h.update(data)
+    return h.hexdigest()
 
 
 def _run(cmd, verbose):
@@ -37,7 +55,8 @@ def build(verbose: bool = False, out: str = LIB_PATH, extra_flags=()) -> str:
     obj_dir = os.path.join(os.path.dirname(out), "obj")
     os.makedirs(obj_dir, exist_ok=True)
     objs = [os.path.join(obj_dir, os.path.splitext(s)[0] + ".o") for s in SOURCES]
-    cmds = [["hipcc", *HIPCC_FLAGS, *extra_flags, "-c", os.path.join(CSRC, s), "-o", o]
+    build_id = f'-DSK_BUILD_ID="{source_hash()}"'
+    cmds = [["hipcc", *HIPCC_FLAGS, build_id, *extra_flags, "-c", os.path.join(CSRC, s), "-o", o]
             for s, o in zip(SOURCES, objs)]
     with ThreadPoolExecutor(max_workers=len(cmds)) as pool:
         list(pool.map(lambda c: _run(c, verbose), cmds))

@@ -540,6 +540,8 @@ LineFFT slice_plan(int n, hipStream_t s) {
 // kMinCachedBatch positions (enough to fill the GPU); otherwise a 4 GiB budget of at most 1024 positions. It then
 // spreads `count` evenly over the batches, so no small remainder batch runs at a fraction of the GPU.
 constexpr int kMinCachedBatch = 64;
+// float64 intensities one batch of sk_hip_scan_intensities may hold (device, and each of two pinned host buffers)
+constexpr size_t kIntensityBytes = size_t(256) << 20;
 // device memory the lane pipeline may take for per-lane detector accumulators when lanes share positions
 constexpr size_t kLaneCopyBytes = size_t(2) << 30;
 
@@ -925,6 +927,12 @@ int sk_hip_available(void) {
 }
 
 const char* sk_hip_last_error(void) { return last_error().c_str(); }
+
+#ifndef SK_BUILD_ID
+#define SK_BUILD_ID "unversioned"
+#endif
+// the hash of the sources this library was compiled from, set by the build script
+const char* sk_hip_build_id(void) { return SK_BUILD_ID; }
 
 void sk_hip_set_timing(int enabled) {
   std::lock_guard<std::recursive_mutex> g(lock());
@@ -1543,6 +1551,100 @@ int sk_hip_scan_waves(const sk_potential* p, const sk_probe* probe, int npos, co
       SK_HIP(hipMemcpyAsync(out_im + (size_t)b0 * M, w.tmp_im.p, nb * M * sizeof(double), hipMemcpyDeviceToHost, s));
       SK_HIP(hipStreamSynchronize(s));
     }
+  });
+}
+
+int sk_hip_scan_intensities(const sk_potential* p, const sk_probe* probe, int npos, const double* positions,
+                            sk_deliver_fn deliver, void* user, const sk_options* opt) {
+  return guarded([&] {
+    Context& c = require();
+    hipStream_t s = c.stream;
+    set_col_tile(c, opt);
+    if (!p) fail(SK_ERR_INVALID, "invalid scan: no potential");
+    check_probe(probe);
+    if (npos < 1 || !positions) fail(SK_ERR_INVALID, "invalid scan: no positions");
+    if (!deliver) fail(SK_ERR_INVALID, "no delivery callback");
+    if (p->energy != probe->energy) fail(SK_ERR_INVALID, "the probe's energy differs from the potential's");
+    if (p->tilt[0] != probe->tilt[0] || p->tilt[1] != probe->tilt[1])
+      fail(SK_ERR_INVALID, "invalid tilt: the probe's tilt differs from the potential's");
+    for (int q = 0; q < npos; ++q)
+      if (!(std::isfinite(positions[2 * q]) && std::isfinite(positions[2 * q + 1])))
+        fail(SK_ERR_INVALID, "invalid position");
+    const sk_grid g = p->grid;
+    const int nx = g.nx, ny = g.ny;
+    const size_t N = (size_t)nx * ny;
+    const bool fast = (!opt || opt->fast_path != 0) && c.precision == 0;
+    // positions per launch: the scan's batch, within kIntensityBytes of float64 intensities per batch (the device
+    // buffer and each of the two pinned host buffers hold one batch)
+    int B = default_batch(*p, npos, opt);
+    B = (int)std::max<size_t>(1, std::min<size_t>((size_t)B, kIntensityBytes / (N * sizeof(double))));
+    Workspace& w = ws();
+    w.G.ensure((size_t)B * nx * p->nc);
+    w.rx.ensure((size_t)B * nx);
+    w.ry.ensure((size_t)B * ny);
+    w.spec.ensure(N);
+    const bool fits = build_spectrum(g, *probe, 1.0 / double(N), &p->keepinv, w.spec.p);
+    if (!fits) w.full.ensure((size_t)B * N);
+    w.positions.upload(positions, 2 * (size_t)npos, s);
+    DevBuf<double> inten((size_t)B * N);
+    // two pinned host buffers: the caller's detectors read one batch while the device computes the next
+    struct Pinned {
+      double* h[2] = {nullptr, nullptr};
+      ~Pinned() {
+        for (auto x : h)
+          if (x) (void)hipHostFree(x);
+      }
+    } pin;
+    for (auto& x : pin.h) SK_HIP(hipHostMalloc(&x, sizeof(double) * (size_t)B * N, hipHostMallocDefault));
+    hipEvent_t done[2];
+    SK_HIP(hipEventCreateWithFlags(&done[0], hipEventDisableTiming));
+    SK_HIP(hipEventCreateWithFlags(&done[1], hipEventDisableTiming));
+    struct Events {
+      hipEvent_t* e;
+      ~Events() {
+        (void)hipEventDestroy(e[0]);
+        (void)hipEventDestroy(e[1]);
+      }
+    } evs{done};
+    std::vector<int> plane_out(p->nslices, -1);
+    Timer tm;
+    int pend_first = -1, pend_count = 0, pend_buf = 0;
+    auto hand_over = [&]() {
+      if (pend_first < 0) return;
+      SK_HIP(hipEventSynchronize(done[pend_buf]));
+      if (deliver(user, pend_first, pend_count, pin.h[pend_buf]) != 0)
+        fail(SK_ERR_INVALID, "the delivery callback stopped the scan");
+      pend_first = -1;
+    };
+    int buf = 0;
+    for (int b0 = 0; b0 < npos; b0 += B) {
+      const int nb = std::min(B, npos - b0);
+      launch_ramps(nx, ny, g.extent_x / nx, g.extent_y / ny, w.positions.p, b0, nullptr, nb, w.rx.p, w.ry.p, s);
+      ColArgs init = col_args(*p, fits ? w.G.p : w.full.p);
+      init.spec = w.spec.p;
+      init.rx = w.rx.p;
+      init.ry = w.ry.p;
+      init.flags = COL_GENERATE | COL_INV | COL_OUT;
+      if (!fits) {
+        init.nc = ny;
+        init.keep = identity_map(ny);
+      }
+      col_pass(init, nb, fast && fits, s);
+      // the last slice's column pass stops after the propagator: G then holds FFT2(psi_exit) / N
+      ColArgs fin = col_args(*p, w.G.p);
+      fin.flags = COL_IN_G | COL_FWD | COL_MUL_P | COL_OUT;
+      slice_loop(*p, nb, w.G.p, false, w.full.p, fits ? 0 : 1, plane_out, nullptr, &fin, fast, tm);
+      launch_compact_intensity(w.G.p, nx, ny, p->nc, p->keepinv_d.p, nb, double(N) * double(N), inten.p, s);
+      SK_HIP(hipMemcpyAsync(pin.h[buf], inten.p, sizeof(double) * (size_t)nb * N, hipMemcpyDeviceToHost, s));
+      SK_HIP(hipEventRecord(done[buf], s));
+      hand_over();  // the previous batch, while this one runs
+      pend_first = b0;
+      pend_count = nb;
+      pend_buf = buf;
+      buf ^= 1;
+    }
+    hand_over();
+    SK_HIP(hipStreamSynchronize(s));
   });
 }
 

@@ -283,6 +283,9 @@ int default_scan_batch(const sk_potential& p, int count);
 void launch_phase_to_complex(const double* V, double sigma, float2* out, size_t n, hipStream_t s);
 void launch_real_to_complex(const double* re, const double* im, float2* out, size_t n, hipStream_t s);
 void launch_complex_to_split(const float2* in, double* re, double* im, size_t n, double scale, hipStream_t s);
+// float64 |G|^2 * scale of compact spectra on the full grid, zero at the pruned columns
+void launch_compact_intensity(const float2* G, int nx, int ny, int nc, const int* keepinv, size_t batch, double scale,
+                              double* out, hipStream_t s);
 // the CTF's parameters on the device (ContrastTransferFunction.swift:36-71)
 struct CtfParams {
   double ab[25];

@@ -55,6 +55,32 @@ void launch_complex_to_split(const float2* in, double* re, double* im, size_t n,
   SK_HIP(hipGetLastError());
 }
 
+// |FFT2(psi_exit)|^2 of `batch` waves held as compact spectra G [batch][nx][nc] (the final column pass's output,
+// spectrum / N): every grid point in FFT order [ix * ny + iy], float64, zero at the columns the bandlimit prunes
+// (the propagator is zero there), times `scale` (N^2) (DiffractionIntensity, Detectors.swift:28-36).
+__global__ void compact_intensity_kernel(const float2* G, int nx, int ny, int nc, const int* keepinv, size_t total,
+                                         double scale, double* out) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const size_t N = (size_t)nx * ny;
+  const size_t b = i / N, r = i - b * N;
+  const int ix = int(r / ny), iy = int(r - (size_t)ix * ny);
+  const int c = keepinv[iy];
+  double v = 0.0;
+  if (c >= 0) {
+    const float2 z = G[(b * nx + ix) * nc + c];
+    v = ((double)z.x * z.x + (double)z.y * z.y) * scale;
+  }
+  out[i] = v;
+}
+void launch_compact_intensity(const float2* G, int nx, int ny, int nc, const int* keepinv, size_t batch, double scale,
+                              double* out, hipStream_t s) {
+  const size_t total = batch * (size_t)nx * ny;
+  hipLaunchKernelGGL(compact_intensity_kernel, grid_for(total), dim3(kThreads), 0, s, G, nx, ny, nc, keepinv, total,
+                     scale, out);
+  SK_HIP(hipGetLastError());
+}
+
 // Antialias.aperture (Antialias.swift:18-39), float64
 __global__ void antialias_kernel(int nx, int ny, double ex, double ey, double* out) {
   size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;

@@ -13,7 +13,7 @@ from typing import List, Optional, Sequence, Tuple, Union
 import numpy as np
 
 from . import _native as N
-from .core import ElectronBeam, ExecutionOptions, Grid, Normalization, SimulatoError, Waves
+from .core import ElectronBeam, ExecutionOptions, Grid, Normalization, SimulatoError, Waves, fftfreq
 from .multislice import AnyPotential, PreparedPotential, _options, _prepared, _raise, _sk_grid
 from .probe import GridScan, LineScan, Probe
 
@@ -91,7 +91,68 @@ class SegmentedDetector:
         return d
 
 
+BUILTIN_DETECTORS = (AnnularDetector, PixelatedDetector, CenterOfMassDetector, SegmentedDetector)
 Detector = Union[AnnularDetector, PixelatedDetector, CenterOfMassDetector, SegmentedDetector]
+
+
+@dataclass(frozen=True)
+class DiffractionIntensity:
+    """The full, unshifted diffraction intensity |FFT2(psi)|^2 of one exit wave, in FFT order [ix * ny + iy]: what
+    every detector reads (DiffractionIntensity, Detectors.swift:11-81). A caller's own detector receives these."""
+
+    grid: Grid
+    beam: ElectronBeam
+    values: np.ndarray  # nx * ny float64
+
+    def __post_init__(self):
+        v = np.asarray(self.values, dtype=np.float64).reshape(-1)
+        if v.size != self.grid.count:
+            raise ValueError(f"DiffractionIntensity: {self.grid.nx} x {self.grid.ny} values required")
+        object.__setattr__(self, "values", v)
+
+    @property
+    def pattern(self) -> np.ndarray:
+        """The values as an [nx, ny] array (FFT order, not shifted)."""
+        return self.values.reshape(self.grid.nx, self.grid.ny)
+
+    @property
+    def angular_sampling(self) -> Tuple[float, float]:
+        """mrad per pixel per axis: (1 / extent) * lambda * 1e3 (Detectors.swift:39-41)."""
+        lam = self.beam.wavelength
+        return (self.grid.reciprocal_sampling_x * lam * 1e3, self.grid.reciprocal_sampling_y * lam * 1e3)
+
+    @staticmethod
+    def angles(m: int, d: float) -> np.ndarray:
+        """The angles [mrad] of the pixels of an m-point axis at angular sampling d: fftfreq(m, 1 / d / m)
+        (Detectors.swift:43-44)."""
+        return fftfreq(m, 1 / d / float(m))
+
+    @property
+    def cutoff_angles(self) -> Tuple[float, float]:
+        """floor(n'/2) * d per axis, n' the antialias cutoff points with the grid's parity (Detectors.swift:46-67)."""
+        kcut = 2.0 / 3.0 / max(self.grid.sampling_x, self.grid.sampling_y)
+
+        def size(n, extent):
+            f = 10.0 ** 7
+            m = int(math.floor(round(kcut * extent * f) / f))
+            return m + 1 if m % 2 != n % 2 else m
+
+        mx = size(self.grid.nx, self.grid.nx * self.grid.sampling_x)
+        my = size(self.grid.ny, self.grid.ny * self.grid.sampling_y)
+        ax, ay = self.angular_sampling
+        return ((mx // 2) * ax, (my // 2) * ay)
+
+
+def is_builtin(d) -> bool:
+    return isinstance(d, BUILTIN_DETECTORS)
+
+
+def _check_custom(d):
+    """A caller's detector follows the reference's open Detector protocol (Detectors.swift:84-88): shape(grid, beam)
+    gives the shape of one position's values, detect(intensity) the values."""
+    if not (callable(getattr(d, "shape", None)) and callable(getattr(d, "detect", None))):
+        raise SimulatoError("invalidDetector", f"invalid detector: {type(d).__name__} has no shape(grid, beam) and "
+                            "detect(intensity)")
 
 
 @dataclass
@@ -168,12 +229,15 @@ class ScanWaves:
 class STEM:
     @staticmethod
     def scan(probe: Probe, potentials: Sequence[AnyPotential], scan: Union[GridScan, np.ndarray],
-             detectors: Sequence[Detector], options: Optional[ExecutionOptions] = None,
+             detectors: Sequence, options: Optional[ExecutionOptions] = None,
              device_outputs: Optional[Sequence[int]] = None) -> List[ScanMeasurement]:
         """Every position's probe through every potential, each detector averaged over the potentials
         (STEM.scan, STEM.swift:49-81, 120-150). `scan` is a GridScan or an [npos, 2] array of positions (a line).
-        `device_outputs`: optional device pointers (one per detector) that receive the values instead of host
-        arrays; the returned measurements then hold None."""
+        The built-in detectors are reduced on the device (sk_hip_scan); any other object with shape(grid, beam) and
+        detect(intensity) methods (the reference's open Detector protocol) reads each position's
+        DiffractionIntensity on the host, delivered by sk_hip_scan_intensities, as the reference's `.metal` scan does
+        for every detector. `device_outputs`: optional device pointers (one per detector, built-in detectors only)
+        that receive the values instead of host arrays; the returned measurements then hold None."""
         if not potentials:
             raise SimulatoError("invalidScan", "invalid scan: no potential")
         if isinstance(scan, (GridScan, LineScan)):
@@ -184,6 +248,23 @@ class STEM:
         for p in potentials:
             if not p.grid.matches(probe.grid):
                 raise SimulatoError("gridMismatch", "the waves and the potential are on different grids")
+        custom = [k for k, d in enumerate(detectors) if not is_builtin(d)]
+        for k in custom:
+            _check_custom(detectors[k])
+        if custom:
+            if device_outputs is not None:
+                raise SimulatoError("invalidDetector", "device outputs take the built-in detectors only")
+            builtin = [k for k in range(len(detectors)) if k not in set(custom)]
+            potentials = [_prepared(p, probe.beam, probe.tilt) for p in potentials]  # built once for both paths
+            out: List[Optional[ScanMeasurement]] = [None] * len(detectors)
+            if builtin:
+                for k, m in zip(builtin, STEM.scan(probe, potentials, scan, [detectors[k] for k in builtin],
+                                                   options)):
+                    out[k] = m
+            for k, m in zip(custom, STEM._scan_intensities(probe, potentials, positions, scan_shape,
+                                                           [detectors[k] for k in custom], options)):
+                out[k] = m
+            return out
         prepared = [_prepared(p, probe.beam, probe.tilt) for p in potentials]
         handles = (C.c_void_p * len(prepared))(*[p.handle.value for p in prepared])
         pos = np.ascontiguousarray(positions, dtype=np.float64)
@@ -204,6 +285,50 @@ class STEM:
                                    len(detectors), ptr_arr, 1 if on_device else 0, C.byref(opt)), "STEM.scan")
         return [ScanMeasurement(tuple(scan_shape), s, None if on_device else a.reshape(-1))
                 for s, a in zip(shapes, outs)]
+
+    @staticmethod
+    def _scan_intensities(probe: Probe, potentials: Sequence[AnyPotential], positions: np.ndarray,
+                          scan_shape: Tuple[int, ...], detectors: Sequence, options: Optional[ExecutionOptions]
+                          ) -> List[ScanMeasurement]:
+        """The caller's detectors over every position and potential, from the intensities the device delivers,
+        summed in potential order and divided by the potential count (STEM.metalScan / accumulate / average,
+        STEM.swift:120-150)."""
+        grid, beam = probe.grid, probe.beam
+        shapes = [tuple(int(v) for v in d.shape(grid, beam)) for d in detectors]
+        counts = [int(np.prod(s)) if s else 1 for s in shapes]
+        pos = np.ascontiguousarray(positions, dtype=np.float64)
+        npos = len(pos)
+        sums = [np.zeros(npos * c, dtype=np.float64) for c in counts]
+        prepared = [_prepared(p, beam, probe.tilt) for p in potentials]
+        n = grid.count
+        failure: List[BaseException] = []
+
+        def deliver(_user, first, count, ptr):
+            try:
+                block = np.ctypeslib.as_array(ptr, shape=(count * n,)).reshape(count, n)
+                for j in range(count):
+                    inten = DiffractionIntensity(grid, beam, block[j].copy())
+                    q = first + j
+                    for d, det in enumerate(detectors):
+                        v = np.asarray(det.detect(inten), dtype=np.float64).reshape(-1)
+                        if v.size != counts[d]:
+                            raise SimulatoError("invalidDetector", "a detector returned the wrong number of values")
+                        sums[d][q * counts[d]:(q + 1) * counts[d]] += v
+                return 0
+            except BaseException as e:  # handed back after the library has returned
+                failure.append(e)
+                return 1
+
+        cb = N.DELIVER_FN(deliver)
+        opt = _options(options)
+        for prep in prepared:
+            rc = N.lib().sk_hip_scan_intensities(prep.handle, C.byref(probe.descriptor()), npos, N.dptr(pos), cb,
+                                                 None, C.byref(opt))
+            if failure:
+                raise failure[0]
+            _raise(rc, "STEM.scan")
+        inv = 1.0 / len(prepared)
+        return [ScanMeasurement(tuple(scan_shape), s, v * inv) for s, v in zip(shapes, sums)]
 
     @staticmethod
     def scan_waves(probe: Probe, potential: AnyPotential, scan: Union[GridScan, np.ndarray],
@@ -258,4 +383,5 @@ def set_timing(enabled: bool):
 
 
 __all__ = ["AnnularDetector", "PixelatedDetector", "CenterOfMassDetector", "SegmentedDetector", "ScanMeasurement",
+           "DiffractionIntensity", "BUILTIN_DETECTORS", "is_builtin",
            "STEM", "WavesDetector", "ScanWaves", "last_timing", "set_timing", "detector_shape"]

@@ -68,7 +68,7 @@ def _options(opt: Optional[ExecutionOptions]) -> N.SkOptions:
 class Slicing:
     @staticmethod
     def thicknesses(depth: float, requested: float, maximum_count: int = MAXIMUM_SLICE_COUNT) -> List[float]:
-        """n = ceil(Lz / s) equal slices of Lz / n (Slicing.swift:184-192)."""
+        """n = ceil(Lz / s) equal slices of Lz / n (Slicing.swift:17-25)."""
         if not (math.isfinite(requested) and requested > 0):
             raise SimulatoError("invalidSliceThickness", f"slice thickness must be finite and positive, got {requested} A")
         n = math.ceil(depth / requested)
@@ -78,7 +78,7 @@ class Slicing:
 
     @staticmethod
     def validated(thicknesses: Sequence[float], depth: float, maximum_count: int = MAXIMUM_SLICE_COUNT) -> List[float]:
-        """Slicing.swift:196-205"""
+        """A list of thicknesses summing to the depth within NumPy's isclose (Slicing.swift:29-38)."""
         t = [float(x) for x in thicknesses]
         if not t or len(t) > maximum_count:
             raise SimulatoError("invalidSliceThickness", "slice thickness must be finite and positive, got 0 A")
@@ -93,8 +93,34 @@ class Slicing:
         return t
 
     @staticmethod
+    def crystal_slice_thicknesses(structure: AtomicStructure, tolerance: float = 0.2) -> List[float]:
+        """Thicknesses that follow the structure's atomic planes (abTEM's crystal_slice_thicknesses; Slicing.swift:
+        51-70): every atom's z as given (not wrapped), with 0 and the box depth, falls in bin floor(z / tolerance);
+        the thicknesses are the gaps between the centres (b + 1/2) * tolerance of the occupied bins, bottom up. They
+        must sum to the depth within NumPy's isclose, as abTEM asserts."""
+        if not (math.isfinite(tolerance) and tolerance > 0):
+            raise SimulatoError("invalidSliceThickness", f"slice thickness must be finite and positive, got {tolerance} A")
+        depth = float(structure.box[2])
+        bins = set()
+        for z in list(np.asarray(structure.positions, dtype=np.float64)[:, 2]) + [0.0, depth]:
+            q = float(z) / tolerance
+            if not (math.isfinite(q) and abs(math.floor(q)) <= float(1 << 53)):
+                raise SimulatoError("invalidSliceThickness",
+                                    f"slice thickness must be finite and positive, got {tolerance} A")
+            bins.add(int(math.floor(q)))
+        centres = [(b + 0.5) * tolerance for b in sorted(bins)]
+        out = [hi - lo for lo, hi in zip(centres[:-1], centres[1:])]
+        total = 0.0
+        for s in out:
+            total += s
+        if not out or not abs(total - depth) <= 1e-8 + 1e-5 * abs(depth):
+            raise SimulatoError("sliceThicknessesDoNotSumToBox",
+                                f"slice thicknesses sum to {total} A, the box is {depth} A deep")
+        return out
+
+    @staticmethod
     def wrapped(structure: AtomicStructure) -> np.ndarray:
-        """Fractional coordinates mod 1 (ASE wrap), z within 1e-10 of the top face set to 0 (Slicing.swift:238-251)."""
+        """Fractional coordinates mod 1 (ASE wrap), z within 1e-10 of the top face set to 0 (Slicing.swift:74-88)."""
         box = np.array(structure.box)
         f = np.fmod(structure.positions / box, 1.0)
         f = np.where(f < 0, f + 1.0, f)
@@ -105,7 +131,7 @@ class Slicing:
 
     @staticmethod
     def slice_indices(z: np.ndarray, thicknesses: Sequence[float]) -> np.ndarray:
-        """e_{j-1} <= z < e_j with e_j = sum_{i<=j} s_i - 1e-12; past the last edge: the last slice (Slicing.swift:255-271)."""
+        """e_{j-1} <= z < e_j with e_j = sum_{i<=j} s_i - 1e-12; past the last edge: the last slice (Slicing.swift:92-104)."""
         edges = []
         acc = 0.0
         for s in thicknesses:

@@ -71,13 +71,14 @@ class SurrogateWorkload:
 
     def haadf(self, st: AtomicStructure, positions: Optional[np.ndarray] = None,
               options: Optional[ExecutionOptions] = None) -> np.ndarray:
-        """Plain multislice HAADF at `positions` (default: the whole scan, as a scan_n x scan_n image)."""
+        """Plain multislice HAADF at `positions` (default: the whole scan, as a scan_n x scan_n image), with the
+        library's default execution plan unless `options` names another."""
         pot = self.potential(st)
         probe = self.probe(pot)
         prepared = pot.prepare(probe.beam)
         scan = self.scan() if positions is None else positions
         out = STEM.scan(probe, [prepared], scan, [AnnularDetector(self.inner, self.outer)],
-                        options or ExecutionOptions(batch=1024))[0].values
+                        options or ExecutionOptions())[0].values
         prepared.release()
         return out.reshape(self.scan_n, self.scan_n) if positions is None else out
 

@@ -3,8 +3,8 @@
 `autotune` times the scan of the caller's own workload under each candidate plan on this device
 (`sk_hip_autotune`: positions per launch, the 512-point column tile width, hipGraph replay, the lane pipeline when
 there are several potentials) and returns the fastest as ExecutionOptions. A plan changes rounding only, never which
-numbers are computed. Results persist in a JSON cache keyed by device, grid lengths, slice counts and
-detector kinds, so a second process with the same workload reuses the measurement.
+numbers are computed. Results persist in a JSON cache keyed by device, grid lengths, slice counts, the power-of-two
+bucket of the position count and detector kinds, so a second process with the same workload reuses the measurement.
 """
 from __future__ import annotations
 
@@ -43,11 +43,20 @@ def _plan_dict(o: ExecutionOptions) -> Dict:
             "lane_waves": o.lane_waves, "col_tile": o.col_tile, "streams": o.streams}
 
 
-def tune_key(device: str, grid, slice_counts: Sequence[int], detectors: Sequence[Detector]) -> str:
-    """The cache key: the device, the grid lengths, the slice counts and the detector kinds (the plan's cost does
-    not depend on the atoms or the positions)."""
+def positions_bucket(positions: int) -> int:
+    """The power of two at or above the position count. The best plan depends strongly on how many positions a
+    scan has (the lane pipeline wins by 3.4x for one position over 16 configurations and loses by 8 % for 64), so
+    plans are cached per bucket."""
+    n = max(1, int(positions))
+    return 1 << (n - 1).bit_length()
+
+
+def tune_key(device: str, grid, slice_counts: Sequence[int], detectors: Sequence[Detector], positions: int) -> str:
+    """The cache key: the device, the grid lengths, the slice counts, the bucket of the position count and the
+    detector kinds (the plan's cost does not depend on the atoms or where the positions lie)."""
     kinds = ",".join(type(d).__name__ for d in detectors)
-    return f"{device}|{grid.nx}x{grid.ny}|S={'+'.join(str(s) for s in slice_counts)}|{kinds}"
+    return (f"{device}|{grid.nx}x{grid.ny}|S={'+'.join(str(s) for s in slice_counts)}|"
+            f"P<={positions_bucket(positions)}|{kinds}")
 
 
 def load_cache(path: str) -> Dict:
@@ -88,7 +97,7 @@ def autotune(probe: Probe, potentials: Sequence[AnyPotential], detectors: Sequen
     descs = (N.SkDetector * max(1, len(detectors)))(*[d.descriptor() for d in detectors])
     slice_counts = [p.slice_count for p in prepared]
     if cache and not retune:
-        key = tune_key(device_name(), probe.grid, slice_counts, detectors)
+        key = tune_key(device_name(), probe.grid, slice_counts, detectors, positions)
         entry = load_cache(cache).get(key)
         if entry is not None:
             return TuneResult(ExecutionOptions(**entry["options"]), float(entry["rate"]), entry["device"],
@@ -99,10 +108,11 @@ def autotune(probe: Probe, potentials: Sequence[AnyPotential], detectors: Sequen
     cands = [dict(_plan_dict(_as_options(rep.candidates[k])), rate=rep.rates[k]) for k in range(rep.ncandidates)]
     device = rep.device.decode()
     result = TuneResult(_as_options(rep.best), rep.best_rate, device, cands,
-                        tune_key(device, probe.grid, slice_counts, detectors))
+                        tune_key(device, probe.grid, slice_counts, detectors, positions))
     if cache:
         store_cache(cache, result.key, result)
     return result
 
 
-__all__ = ["TuneResult", "autotune", "device_name", "tune_key", "load_cache", "store_cache", "DEFAULT_CACHE"]
+__all__ = ["TuneResult", "autotune", "device_name", "tune_key", "positions_bucket", "load_cache", "store_cache",
+           "DEFAULT_CACHE"]

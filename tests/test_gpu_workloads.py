@@ -123,6 +123,25 @@ def test_split_scans_equal_one_stream(n):
         assert np.array_equal(np.asarray(x.values), np.asarray(y.values))
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_split_scans_over_several_potentials_equal_one_stream(graph):
+    """Several potentials of different slice counts scanned one at a time (pipeline=0), each batch split over 2 and 3
+    streams, with and without graph replay (which captures each potential's batches anew): the same numbers as one
+    stream, bit for bit. Each part accumulates its positions' sums on its own stream, in potential order."""
+    pots = [sk.Potential(srtio3(4, 4, tz), 256, 256, slice_thickness=2.0) for tz in (2, 3, 5)]
+    assert [p.slice_count for p in pots] == [4, 6, 10]
+    probe = sk.Probe(sk.ElectronBeam(200e3), pots[0].grid, sk.Aperture(20.0))
+    positions = sk.GridScan((0, 0), (SRTIO3_A, SRTIO3_A), 7, 9).positions
+    dets = [sk.AnnularDetector(30, 90), sk.PixelatedDetector(crop=(32, 32)), sk.CenterOfMassDetector()]
+    prepared = [p.prepare(probe.beam) for p in pots]
+    want = sk.STEM.scan(probe, prepared, positions, dets, sk.ExecutionOptions(streams=1, batch=20, pipeline=0))
+    for streams in (2, 3):
+        got = sk.STEM.scan(probe, prepared, positions, dets,
+                           sk.ExecutionOptions(streams=streams, batch=20, pipeline=0, use_graph=graph))
+        for x, y in zip(want, got):
+            assert np.array_equal(np.asarray(x.values), np.asarray(y.values)), (streams, graph, type(x))
+
+
 def test_streams_outside_one_to_four_are_refused():
     st = srtio3(2, 2, 2)
     pot = sk.Potential(st, 256, 256, slice_thickness=2.0)

@@ -17,7 +17,7 @@ PI = {"abtem": 1, "exact": 0}
 
 
 def n_slices(depth, requested):
-    """Slicing.thicknesses (Slicing.swift:184-192)."""
+    """Slicing.thicknesses (Slicing.swift:17-25)."""
     n = math.ceil(depth / requested)
     return [depth / n] * n
 

@@ -17,8 +17,13 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def test_cache_round_trip_and_key(tmp_path):
     g = sk.Grid(512, 512, 31.24, 31.24)
-    key = T.tune_key("gfx950 256 CU", g, [40], [sk.AnnularDetector(50, 200)])
-    assert key == "gfx950 256 CU|512x512|S=40|AnnularDetector"
+    key = T.tune_key("gfx950 256 CU", g, [40], [sk.AnnularDetector(50, 200)], 4096)
+    assert key == "gfx950 256 CU|512x512|S=40|P<=4096|AnnularDetector"
+    # a plan tuned for a large scan is not reused for a one-position scan, nor for one of 4097 positions
+    assert T.tune_key("gfx950 256 CU", g, [40], [sk.AnnularDetector(50, 200)], 1) != key
+    assert T.tune_key("gfx950 256 CU", g, [40], [sk.AnnularDetector(50, 200)], 4097) != key
+    assert T.tune_key("gfx950 256 CU", g, [40], [sk.AnnularDetector(50, 200)], 3000) == key
+    assert [T.positions_bucket(n) for n in (0, 1, 2, 3, 64, 65, 4096)] == [1, 1, 2, 4, 64, 128, 4096]
     path = str(tmp_path / "sub" / "tune.json")
     res = T.TuneResult(sk.ExecutionOptions(batch=1024, col_tile=8), 7.5e5, "gfx950 256 CU",
                        [{"batch": 1024, "rate": 7.5e5}], key)
@@ -73,6 +78,10 @@ def test_autotune_picks_its_fastest_plan_and_scans_the_same_numbers(tmp_path):
     again = sk.autotune(probe, [pot], dets, positions=1024, cache=cache)
     assert again.from_cache and again.options == res.options
     assert json.load(open(cache))[res.key]["rate"] == res.rate
+    # another position count is another entry, tuned afresh
+    small = sk.autotune(probe, [pot], dets, positions=8, cache=cache)
+    assert not small.from_cache and small.key != res.key
+    assert set(json.load(open(cache))) == {res.key, small.key}
     scan = sk.GridScan((0, 0), (a, a), 8, 8)
     want = sk.STEM.scan(probe, [pot], scan, dets)
     got = sk.STEM.scan(probe, [pot], scan, dets, res.options)
