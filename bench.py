@@ -2,23 +2,30 @@
 
 Workload (the headline HAADF configuration): SrTiO3 tiled 8 x 8 x 20 (31.24 x 31.24 x 78.1 Angstrom, 6,400 atoms),
 512 x 512 grid, 40 slices of 1.9525 Angstrom, 200 keV probe with a 20 mrad soft aperture (C10 = C30 = 0), a 64 x 64
-GridScan over one 3.905 Angstrom cell, annular detector 50-200 mrad, complex64 waves.
+GridScan over one 3.905 Angstrom cell, annular detector 50-200 mrad, complex64 waves, 1024 probe positions per launch
+with the slice loop captured in a hipGraph and replayed.
 
 One step = one full 64 x 64 scan (4,096 positions x 40 slices = 163,840 position-slices) with the prepared
 potential resident in HBM and the HAADF image left in HBM. `value` = position-slices per second over the K timed
-steps at the library's positions per launch and streams: each batch is split over two streams, and at 512 x 512 the
-batch is 3/4 of the largest one whose waves stay in the 256 MiB Infinity Cache, spread evenly over the scan (128
-here, 64 on each stream). The same scan at 1024 positions per launch (`--compare-batch`) is timed first and reported
-beside it as `compare_batch`. The roofline entry times every pass launch of one more scan with HIP events attached
-to the launches; that scan runs the same batches on one stream, so each launch runs alone. The CPU baseline runs the
-float64 oracle on a bounded sample of positions on the host's cores, and the same sample gives the HIP path's error
-against float64.
+steps of the headline plan: 1024 positions per graph replay, run by the library as tiles of 128 positions (64 on each
+of two streams) whose waves stay in the 256 MiB Infinity Cache between passes.
+
+`roofline` is the HBM roofline of the same scan with the waves streamed through HBM: 1024 positions per kernel launch
+(`--streamed-tile`), graph-replayed. Its dominant pass is read three ways: algorithmic bytes (16 N per position and
+pass, the full complex64 wave read and written), the compact bytes it stores (the columns inside the bandlimit), and
+the L2-to-fabric bytes of the committed rocprofv3 counter summary; `step` checks the scan's algorithmic bytes per
+second against the peak. `llc_resident` gives the headline plan's kernels, whose algorithmic rate is an effective
+bandwidth above what HBM could deliver, not an HBM fraction. Kernel times come from HIP events attached to the
+launches of one more scan of the same tiles run on one stream. The CPU baseline runs the float64 oracle on a bounded
+sample of positions on the host's cores (one warm-up, five runs, median and min-max), and the same sample gives the
+HIP path's error against float64.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -56,11 +63,18 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=0,
-                    help="positions per launch; 0: the library's choice (sized to stay in the Infinity Cache)")
-    ap.add_argument("--compare-batch", type=int, default=1024,
-                    help="also time the scan at this many positions per launch (0: skip)")
-    ap.add_argument("--graph", action="store_true", help="replay each batch's slice loop from a hipGraph")
+    ap.add_argument("--batch", type=int, default=None,
+                    help="positions per launch (one graph replay); 0: the library's choice (default: 1024 for the "
+                         "HAADF scan, the library's choice for frozen phonons)")
+    ap.add_argument("--tile", type=int, default=0,
+                    help="positions per kernel launch within a batch; 0: the library's choice (tiles that stay in "
+                         "the Infinity Cache)")
+    ap.add_argument("--streamed-tile", type=int, default=1024,
+                    help="also time the plan whose kernel launches carry this many positions, streaming the waves "
+                         "through HBM: the roofline entry (0: skip)")
+    ap.add_argument("--only", default="all", choices=["all", "headline", "streamed"],
+                    help="run only one of the two plans (for a profiler)")
+    ap.add_argument("--no-graph", action="store_true", help="launch the slice loop's kernels one by one")
     ap.add_argument("--generic", action="store_true", help="generic LDS passes instead of the fast passes")
     ap.add_argument("--col-tile", type=int, default=0, help="columns per 512-point column-pass block (8, 16)")
     ap.add_argument("--streams", type=int, default=0,
@@ -99,6 +113,8 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    if args.batch is None:
+        args.batch = 1024 if args.workload == "haadf" else 0
     if args.workload == "frozen_phonons":
         print(json.dumps(frozen_phonon_bench(args, sk, torch)), flush=True)
         return
@@ -110,10 +126,14 @@ def main():
     probe = sk.Probe(beam, pot.grid, sk.Aperture(w.semiangle))
     scan = sk.GridScan((0.0, 0.0), (sk.workloads.SRTIO3_A, sk.workloads.SRTIO3_A), w.scan_n, w.scan_n)
     det = sk.AnnularDetector(w.inner, w.outer)
-    opt = sk.ExecutionOptions(batch=w.batch, use_graph=args.graph, fast_path=not args.generic,
-                              col_tile=args.col_tile, streams=args.streams)
     S = pot.slice_count
     npos = w.scan_n * w.scan_n
+    N = w.n * w.n
+    graph = not args.no_graph
+
+    def plan(tile, use_graph=graph):
+        return sk.ExecutionOptions(batch=args.batch, tile=tile, use_graph=use_graph, fast_path=not args.generic,
+                                   col_tile=args.col_tile, streams=args.streams)
 
     t0 = time.perf_counter()
     prepared = pot.prepare(beam)
@@ -121,7 +141,7 @@ def main():
     out = torch.zeros(npos, dtype=torch.float64, device="cuda:0")
     positions = scan.positions
 
-    def step(o=opt):
+    def step(o):
         sk.STEM.scan(probe, [prepared], positions, [det], o, device_outputs=[out.data_ptr()])
 
     def timed(o):
@@ -134,32 +154,56 @@ def main():
         torch.cuda.synchronize()
         return time.perf_counter() - t0
 
-    compare = None
-    if args.compare_batch > 0 and args.compare_batch != args.batch:
-        o = sk.ExecutionOptions(batch=args.compare_batch, use_graph=args.graph, fast_path=not args.generic,
-                                col_tile=args.col_tile, streams=args.streams)
-        el = timed(o)
-        compare = {"batch": min(args.compare_batch, npos), "value": npos * S * args.steps / el,
-                   "ms_per_step": el / args.steps * 1e3}
-    elapsed = timed(opt)
-    ms_per_step = elapsed / args.steps * 1e3
-    value = npos * S * args.steps / elapsed
+    def kernel_times(o):
+        """Mean device time per launch of each pass, from HIP events attached to the launches of one more scan of
+        the same tiles run on one stream (so each launch runs alone; graph replay does not change the kernels)."""
+        sk.set_timing(True)
+        step(dataclasses.replace(o, use_graph=False))
+        sk.set_timing(False)
+        return sk.last_timing()
 
-    # per-kernel device time of one more scan, with HIP events around every pass launch
-    sk.set_timing(True)
-    step()
-    sk.set_timing(False)
-    tm = sk.last_timing()
-    N = w.n * w.n
-    B = int(tm["batch"])  # positions per launch of the timed scans
-    alg_bytes = 16.0 * N * B  # per pass launch: the complex64 wave of every position read and written once
-    col_ms, row_ms = tm["col_ms"], tm["row_ms"]  # the average launch of each pass
-    dom = "column" if col_ms >= row_ms else "row"
-    dom_ms = max(col_ms, row_ms)
-    achieved = alg_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
-    stored_bytes = tm["col_bytes"]  # per launch: the compact columns of every wave read and written once
+    def roofline_of(tm, rate):
+        """Three readings of one plan's dominant pass: algorithmic bytes (16 N per position read + written, the
+        full complex64 wave), the compact bytes the pass stores (the kept columns only), and the measured L2-to-
+        fabric bytes of the committed counter profile at this tile; plus the step-level check."""
+        B = int(tm["batch"])
+        col_ms, row_ms = tm["col_ms"], tm["row_ms"]
+        dom = "column pass" if col_ms >= row_ms else "row pass"
+        dom_ms = max(col_ms, row_ms)
+        alg = 16.0 * N * B
+        stored = tm["col_bytes"]
+        measured = measured_traffic(dom, B)
+        gbs = lambda b: b / (dom_ms * 1e-3) / 1e9
+        step_gbs = 32.0 * N * rate / 1e9  # algorithmic bytes per position-slice times the scan's rate
+        r = {"bound": "hbm", "achieved": gbs(alg), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": gbs(alg) / HBM_PEAK_GBS, "traffic": measured, "kernel": dom, "kernel_ms": dom_ms,
+             "row_ms": row_ms, "col_ms": col_ms, "positions_per_kernel_launch": B,
+             "algorithmic_bytes_per_launch": alg,
+             "stored": {"bytes_per_launch": stored, "GBs": gbs(stored), "frac": gbs(stored) / HBM_PEAK_GBS},
+             "measured": None if measured is None else {"bytes_per_launch": measured, "GBs": gbs(measured),
+                                                        "frac": gbs(measured) / HBM_PEAK_GBS},
+             "step": {"GBs": step_gbs, "frac": step_gbs / HBM_PEAK_GBS,
+                      "ok": bool(step_gbs <= HBM_PEAK_GBS and gbs(alg) <= HBM_PEAK_GBS)}}
+        return r
+
+    result_plans = {}
+    # the plan whose waves stream through HBM: each kernel launch carries the whole batch
+    if args.streamed_tile > 0 and args.only in ("all", "streamed"):
+        o = plan(args.streamed_tile)
+        el = timed(o)
+        rate = npos * S * args.steps / el
+        result_plans["streamed"] = (o, rate, el / args.steps * 1e3, kernel_times(o))
+    if args.only in ("all", "headline"):
+        o = plan(args.tile)
+        el = timed(o)
+        rate = npos * S * args.steps / el
+        result_plans["headline"] = (o, rate, el / args.steps * 1e3, kernel_times(o))
     haadf = out.cpu().numpy()
     copy_gbs = copy_bandwidth(torch)
+    if "headline" in result_plans:
+        o, value, ms_per_step, tm_head = result_plans["headline"]
+    else:
+        o, value, ms_per_step, tm_head = result_plans["streamed"]
 
     result = {
         "metric": "probe-positions*slices/sec (HAADF scan) + fraction of HBM roofline at <=1e-4 rel err",
@@ -175,21 +219,35 @@ def main():
         "dtype": "c64",
         "data": "synthetic",
         "config": {"workload": w.name, "grid": [w.n, w.n], "slices": S, "positions": npos,
-                   "batch": B, "detector": f"annular {w.inner:g}-{w.outer:g} mrad", "energy_keV": w.energy / 1e3,
-                   "semiangle_mrad": w.semiangle, "fast_path": not args.generic, "graph": bool(args.graph),
-                   # the library splits this scan's batches over 2 streams unless --streams says otherwise
+                   "batch": int(tm_head["group"]), "graph": bool(o.use_graph),
+                   "positions_per_kernel_launch": int(tm_head["batch"]),
+                   "detector": f"annular {w.inner:g}-{w.outer:g} mrad", "energy_keV": w.energy / 1e3,
+                   "semiangle_mrad": w.semiangle, "fast_path": not args.generic,
+                   # the library splits this scan's tiles over 2 streams unless --streams says otherwise
                    "streams": 1 if args.generic else (args.streams or 2),
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(f"{dom} pass", B),
-                     "kernel": f"{dom} pass", "kernel_ms": dom_ms, "row_ms": row_ms, "col_ms": col_ms,
-                     "algorithmic_bytes_per_launch": alg_bytes, "stored_bytes_per_launch": stored_bytes,
-                     "copy_GBs": copy_gbs},
         "prepare_ms": prepare_ms,
         "scan_pos_slices": npos * S,
     }
-    if compare is not None:
-        result["compare_batch"] = compare
+    if "streamed" in result_plans:
+        so, srate, sms, stm = result_plans["streamed"]
+        roof = roofline_of(stm, srate)
+        roof.update({"copy_GBs": copy_gbs, "plan": {
+            "what": f"{int(stm['group'])} positions per launch and per kernel launch, graph-replayed slice loop: "
+                    "the waves stream through HBM between passes",
+            "value": srate, "ms_per_step": sms, "graph": bool(so.use_graph)}})
+        result["roofline"] = roof
+    if "headline" in result_plans:
+        llc = roofline_of(tm_head, value)
+        result["llc_resident"] = {
+            "what": f"the headline plan's kernels: {int(tm_head['batch'])} positions per kernel launch, whose waves "
+                    "stay in the 256 MiB Infinity Cache between passes; algorithmic bytes over their time are an "
+                    "effective bandwidth, not an HBM fraction",
+            "kernel": llc["kernel"], "kernel_ms": llc["kernel_ms"], "row_ms": llc["row_ms"], "col_ms": llc["col_ms"],
+            "effective_GBs": llc["achieved"], "effective_x_hbm_peak": llc["frac"],
+            "step_effective_GBs": llc["step"]["GBs"], "stored": llc["stored"], "measured": llc["measured"]}
+        if "roofline" not in result:
+            result["roofline"] = llc
 
     if not args.no_cpu_baseline:
         result["cpu_baseline"], result["rel_err_vs_fp64"] = cpu_baseline(args, w, st, pot, positions, haadf, S)

@@ -71,7 +71,8 @@ typedef struct sk_detector {
 
 /* Execution knobs (ExecutionOptions.swift:21-49). None changes which numbers are computed, only their rounding. */
 typedef struct sk_options {
-  int batch;        /* waves resident per launch; 0 = the library's choice for the grid */
+  int batch;        /* positions per launch (sk_hip_scan: per host chunk, or per graph replay with use_graph; the
+                       other entry points: waves resident per kernel launch); 0 = the library's choice for the grid */
   int use_graph;    /* capture the slice loop in hipGraphs and replay them */
   int fast_path;    /* 1 (default): register-resident FFT passes where the length has one; 0: generic LDS passes */
   int pipeline;     /* scan: 1 = lanes of positions advance one slice per launch, each through its own potential
@@ -89,6 +90,11 @@ typedef struct sk_options {
                        forks into the parts); 0 = the library's choice (2 for batches of at least 128 positions,
                        with 3/4 of the cache-resident batch at 512 x 512, else 1); the lane pipeline and other
                        scans use one stream */
+  int tile;         /* sk_hip_scan, one potential at a time: positions each kernel launch carries, a part of the
+                       batch; 0 = the library's choice: the whole batch when its waves fit the device's last-level
+                       cache, else parts that do (on an MI355X, 128-171 positions at 512 x 512), so a batch larger
+                       than the cache still runs from the cache between passes; tile = batch streams the batch's
+                       waves through HBM */
 } sk_options;
 
 /* --- availability (MetalEngine.require / Multislice.metalIsAvailable, MetalEngine.swift:63-77,
@@ -225,7 +231,8 @@ typedef struct sk_timing {
   long long row_launches, col_launches;
   double row_bytes, col_bytes;
   double total_ms;
-  long long batch;
+  long long batch;   /* positions per kernel launch (the tile) */
+  long long group;   /* positions per launch (the batch: a host chunk, or one graph replay) */
 } sk_timing;
 int sk_hip_last_timing(sk_timing* t);
 void sk_hip_set_timing(int enabled);

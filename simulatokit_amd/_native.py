@@ -34,13 +34,13 @@ class SkDetector(C.Structure):
 class SkOptions(C.Structure):
     _fields_ = [("batch", C.c_int), ("use_graph", C.c_int), ("fast_path", C.c_int), ("pipeline", C.c_int),
                 ("lane_waves", C.c_int), ("col_tile", C.c_int),
-                ("precision", C.c_int), ("streams", C.c_int)]
+                ("precision", C.c_int), ("streams", C.c_int), ("tile", C.c_int)]
 
 
 class SkTiming(C.Structure):
     _fields_ = [("row_ms", C.c_double), ("col_ms", C.c_double), ("row_launches", C.c_longlong),
                 ("col_launches", C.c_longlong), ("row_bytes", C.c_double), ("col_bytes", C.c_double),
-                ("total_ms", C.c_double), ("batch", C.c_longlong)]
+                ("total_ms", C.c_double), ("batch", C.c_longlong), ("group", C.c_longlong)]
 
 
 class SkCtf(C.Structure):

@@ -205,7 +205,8 @@ class ExecutionOptions:
     """How the `.hip` path executes (ExecutionOptions.swift:21-49). None of these changes which numbers are
     computed, only their rounding."""
 
-    batch: int = 0          # waves per launch; 0 = the library's choice
+    batch: int = 0          # positions per launch (a scan's host chunk, or one graph replay with use_graph); 0 =
+                            # the library's choice
     use_graph: bool = False  # replay the slice loop from captured hipGraphs
     fast_path: bool = True   # register-resident FFT passes where the grid length has one
     pipeline: int = -1       # scan: 1 lanes of positions each on its own potential and slice per launch, 0 one
@@ -216,6 +217,9 @@ class ExecutionOptions:
     precision: str = "float32"  # the slice loop's transforms: "float32", or "split_bf16" (matrix cores)
     streams: int = 0         # scan one potential at a time through the fast passes: streams the parts of each batch
                              # run on (1-4), 0 the library's choice (2 for batches of at least 128 positions)
+    tile: int = 0            # scan one potential at a time: positions per kernel launch within a batch; 0 = the
+                             # library's choice (parts that stay in the last-level cache); tile = batch streams the
+                             # batch through HBM
 
 
 @dataclass(frozen=True)

@@ -1851,9 +1851,20 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
         B = L * W;
       }
     }
-    // split scan: the parts of each batch run on their own streams, one potential at a time, through the fast
-    // passes with the probe inside the bandlimit (with graph replay, each batch's graph forks into the parts and
-    // joins them). A timed scan (per-launch events) keeps the batch and runs it on one stream.
+    // Tiles: a batch (positions per launch: a host chunk, or one graph replay) runs as parts of T positions, each
+    // part through the whole slice loop. The caller's tile, or, when the caller names a batch larger than the
+    // last-level cache holds, parts sized to the cache, so the waves stay in the cache between passes whatever the
+    // batch (the default batch is itself cache-sized: one part).
+    const bool user_batch = opt && opt->batch > 0, user_tile = opt && opt->tile > 0;
+    if (opt && opt->tile < 0) fail(SK_ERR_INVALID, "tile must be 0 (the library's choice) or positive");
+    int T = B;
+    if (!pipe) {
+      if (user_tile) T = std::min(opt->tile, B);
+      else if (user_batch) T = std::min(B, default_batch(p0, B, nullptr));
+    }
+    // split scan: the parts of each tile run on their own streams, one potential at a time, through the fast
+    // passes with the probe inside the bandlimit (with graph replay, each batch's graph forks into the streams and
+    // joins them at its end). A timed scan (per-launch events) keeps the tile and runs it on one stream.
     const bool graph = opt && opt->use_graph;
     int split = 1;
     if (!pipe && fits && fast &&
@@ -1862,25 +1873,29 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
       if (want > 0) {
         split = want;
       } else {
-        const int Bs = (!(opt && opt->batch > 0) && nx == 512 && ny == 512) ? default_batch(p0, npos, opt, 0.75) : B;
-        if (Bs >= kMinSplitBatch) {
+        // at 512 x 512 the split scan takes 3/4 of the cache-resident batch
+        const int Ts = (!user_tile && nx == 512 && ny == 512)
+                           ? std::min(T, default_batch(p0, user_batch ? B : npos, nullptr, 0.75))
+                           : T;
+        if (Ts >= kMinSplitBatch) {
           split = kSplitStreams;
-          B = Bs;
+          T = Ts;
+          if (!user_batch) B = T;
         }
       }
-      split = std::max(1, std::min(split, B));
+      split = std::max(1, std::min(split, T));
       if (c.timing) split = 1;
     }
-    w.G.ensure((size_t)B * nx * p0.nc);
-    w.rx.ensure((size_t)B * nx);
-    w.ry.ensure((size_t)B * ny);
-    if (!fits) w.full.ensure((size_t)B * N);
+    w.G.ensure((size_t)T * nx * p0.nc);
+    w.rx.ensure((size_t)T * nx);
+    w.ry.ensure((size_t)T * ny);
+    if (!fits) w.full.ensure((size_t)T * N);
     const int gcols = col_generic_lines(slice_plan(nx, s));
     int bpw = (p0.nc + gcols - 1) / gcols;
     if (fast && fast_col_supported(nx, p0.nc, COL_IN_G | COL_FWD | COL_MUL_P | COL_DETECT))
       bpw = fast_col_blocks_per_wave(nx, p0.nc, pipe);
     dp.blocks_per_wave = bpw;
-    w.partial.ensure((size_t)B * nslots * bpw);
+    w.partial.ensure((size_t)T * nslots * bpw);
     dp.partial = w.partial.p;
     Timer tm;
     tm.on = c.timing;
@@ -1931,23 +1946,37 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
       for (int q = 0; q < k - 1; ++q) st.push_back(c.side[q]);
       return st;
     };
+    // One batch of nb positions starting at b0 through P: tiles of at most T positions, spread evenly, in order;
+    // each tile's positions split over the streams `st` in parts of H = ceil(T / k), stream q always on the
+    // workspace positions [q H, q H + H): a stream runs its parts of consecutive tiles one after another, and no two
+    // streams ever share workspace, whatever the tile's size.
+    auto enqueue_tiles = [&](const sk_potential& P, int b0, const int* pos_dev, int nb, Timer& t,
+                             const std::vector<hipStream_t>& st) {
+      const int tiles = (nb + T - 1) / T;
+      const int Tp = (nb + tiles - 1) / tiles;
+      const int k = (int)st.size();
+      const int H = (T + k - 1) / k;
+      for (int i = 0; i < tiles; ++i) {
+        const int first = i * Tp, count = std::min(Tp, nb - first);
+        if (count <= 0) break;
+        for (int q = 0; q < k && q * H < count; ++q)
+          enqueue_batch(P, b0 + first + q * H, pos_dev, std::min(H, count - q * H), t, k > 1 ? st[q] : nullptr,
+                        q * H);
+        if (k == 1) t.harvest(timing);
+      }
+    };
     if (pipe) {
       run_pipeline(pots, npots, npos, W, L, g, dp, graph, tm, timing);
     } else if (split > 1 && !graph) {
-      // k parts of each batch, each through its own slice loop on its own stream
+      // the parts of each tile, each through its own slice loop on its own stream
       const int k = split;
       const std::vector<hipStream_t> st = part_streams(k);
       hipEvent_t fork;
       SK_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
       SK_HIP(hipEventRecord(fork, s));
       for (int q = 1; q < k; ++q) SK_HIP(hipStreamWaitEvent(st[q], fork, 0));
-      const int H = (B + k - 1) / k;
       for (int pk = 0; pk < npots; ++pk)
-        for (int b0 = 0; b0 < npos; b0 += B) {
-          const int nb = std::min(B, npos - b0);
-          for (int q = 0; q < k && q * H < nb; ++q)
-            enqueue_batch(*pots[pk], b0 + q * H, nullptr, std::min(H, nb - q * H), tm, st[q], q * H);
-        }
+        for (int b0 = 0; b0 < npos; b0 += B) enqueue_tiles(*pots[pk], b0, nullptr, std::min(B, npos - b0), tm, st);
       std::vector<hipEvent_t> joins(k - 1);
       for (int q = 1; q < k; ++q) {
         SK_HIP(hipEventCreateWithFlags(&joins[q - 1], hipEventDisableTiming));
@@ -1957,11 +1986,9 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
       SK_HIP(hipEventDestroy(fork));
       for (auto e : joins) SK_HIP(hipEventDestroy(e));
     } else if (!graph) {
+      const std::vector<hipStream_t> one(1, s);
       for (int k = 0; k < npots; ++k)
-        for (int b0 = 0; b0 < npos; b0 += B) {
-          enqueue_batch(*pots[k], b0, nullptr, std::min(B, npos - b0), tm);
-          tm.harvest(timing);
-        }
+        for (int b0 = 0; b0 < npos; b0 += B) enqueue_tiles(*pots[k], b0, nullptr, std::min(B, npos - b0), tm, one);
     } else {
       // hipGraph replay: per potential, one graph per batch shape (full, remainder), captured once and replayed for
       // every batch; the first position of a replay is read from device memory, copied in from pinned host memory
@@ -2001,19 +2028,17 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
               SK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
               try {
                 if (split > 1) {
-                  // the graph forks into the batch's parts, one per stream, and joins them at its end
+                  // the graph forks into the streams, runs the batch's tiles on them and joins them at its end
                   const std::vector<hipStream_t> st = part_streams(split);
-                  const int H = (nb + split - 1) / split;
                   SK_HIP(hipEventRecord(fork_ev, s));
                   for (int q = 1; q < split; ++q) SK_HIP(hipStreamWaitEvent(st[q], fork_ev, 0));
-                  for (int q = 0; q < split && q * H < nb; ++q)
-                    enqueue_batch(*pots[k], q * H, w.pos_dev.p, std::min(H, nb - q * H), off, st[q], q * H);
+                  enqueue_tiles(*pots[k], 0, w.pos_dev.p, nb, off, st);
                   for (int q = 1; q < split; ++q) {
                     SK_HIP(hipEventRecord(join_ev[q - 1], st[q]));
                     SK_HIP(hipStreamWaitEvent(s, join_ev[q - 1], 0));
                   }
                 } else {
-                  enqueue_batch(*pots[k], 0, w.pos_dev.p, nb, off);
+                  enqueue_tiles(*pots[k], 0, w.pos_dev.p, nb, off, std::vector<hipStream_t>(1, s));
                 }
               } catch (...) {
                 hipGraph_t dropped;
@@ -2082,9 +2107,10 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
     timing.total_ms = total;
     if (timing.row_launches) timing.row_ms /= double(timing.row_launches);
     if (timing.col_launches) timing.col_ms /= double(timing.col_launches);
-    timing.row_bytes = 2.0 * sizeof(float2) * double(B) * nx * p0.nc;
+    timing.row_bytes = 2.0 * sizeof(float2) * double(T) * nx * p0.nc;
     timing.col_bytes = timing.row_bytes;
-    timing.batch = B;
+    timing.batch = T;
+    timing.group = B;
     c.last = timing;
   });
 }

@@ -112,6 +112,7 @@ extern "C" int sk_hip_autotune(const sk_potential* const* pots, int npots, const
               if (st == 1 && (b != 0 || pp != 0)) continue;
               sk_options o{};
               o.batch = b;
+              o.tile = b;  // a fixed batch runs whole per kernel launch, streaming from HBM
               o.use_graph = gr;
               o.fast_path = 1;
               o.pipeline = pp;

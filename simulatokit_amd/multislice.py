@@ -61,6 +61,7 @@ def _options(opt: Optional[ExecutionOptions]) -> N.SkOptions:
         raise ValueError(f"precision must be 'float32' or 'split_bf16', got {o.precision!r}")
     s.precision = 1 if o.precision == "split_bf16" else 0
     s.streams = int(o.streams)
+    s.tile = int(o.tile)
     return s
 
 

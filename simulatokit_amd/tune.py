@@ -35,12 +35,13 @@ class TuneResult:
 
 def _as_options(o: N.SkOptions) -> ExecutionOptions:
     return ExecutionOptions(batch=o.batch, use_graph=bool(o.use_graph), fast_path=bool(o.fast_path),
-                            pipeline=o.pipeline, lane_waves=o.lane_waves, col_tile=o.col_tile, streams=o.streams)
+                            pipeline=o.pipeline, lane_waves=o.lane_waves, col_tile=o.col_tile, streams=o.streams,
+                            tile=o.tile)
 
 
 def _plan_dict(o: ExecutionOptions) -> Dict:
     return {"batch": o.batch, "use_graph": o.use_graph, "fast_path": o.fast_path, "pipeline": o.pipeline,
-            "lane_waves": o.lane_waves, "col_tile": o.col_tile, "streams": o.streams}
+            "lane_waves": o.lane_waves, "col_tile": o.col_tile, "streams": o.streams, "tile": o.tile}
 
 
 def positions_bucket(positions: int) -> int:

@@ -277,6 +277,13 @@ def test_scan_is_deterministic_and_batch_independent():
                          sk.ExecutionOptions(batch=batch, use_graph=True, streams=streams))
         for x, y in zip(a, g):
             assert np.array_equal(x.values, y.values), (batch, streams)
+    # a batch run as tiles (uneven: 64 positions in tiles of at most 20, spread as 4 x 16), with and without graph
+    # replay, on one and two streams
+    for graph, streams in ((False, 1), (False, 2), (True, 1), (True, 2)):
+        t = sk.STEM.scan(probe, [prepared], scan, dets,
+                         sk.ExecutionOptions(batch=64, tile=20, use_graph=graph, streams=streams))
+        for x, y in zip(a, t):
+            assert np.array_equal(x.values, y.values), ("tiles", graph, streams)
 
 
 def test_lane_pipeline_single_position_over_many_potentials():

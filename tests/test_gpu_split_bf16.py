@@ -66,7 +66,7 @@ def test_precision_study_at_4096_positions(n, tz, outer):
                              [O.det_annular(50, outer), O.det_pixelated("cutoff")])
     rows = []
     for name, prec in (("float32", "float32"), ("split-BF16", "split_bf16")):
-        opt = sk.ExecutionOptions(batch=4096, precision=prec)
+        opt = sk.ExecutionOptions(batch=4096, tile=4096, precision=prec)
         t0 = time.perf_counter()
         full = sk.STEM.scan(probe, [prepared], positions, [sk.AnnularDetector(50, outer)], opt)[0].values
         wall = time.perf_counter() - t0

@@ -81,8 +81,9 @@ def test_full_haadf_scan_equals_the_oracle_at_sample_positions(haadf_scan):
 def test_default_batch_is_sized_to_the_infinity_cache(haadf_scan):
     """The library's positions per launch: the largest batch whose compact waves (nx x 352 complex64 at 512^2) take at
     most 240 MiB, spread evenly over the scan: 4,096 positions -> 24 launches of at most 171 on one stream. The
-    default split scan (two streams) takes 3/4 of that at 512^2: 32 launches of 128, 64 on each stream. A fixed
-    batch is honoured as given, and the chunking changes no number the scan computes."""
+    default split scan (two streams) takes 3/4 of that at 512^2: 32 launches of 128, 64 on each stream. A caller's
+    batch of 1024 runs as tiles of that size (1024 positions per launch, 128 per kernel launch), and with tile = 1024
+    as whole 1024-position kernel launches streaming from HBM; the chunking changes no number the scan computes."""
     w, st, pot, scan, haadf, _ = haadf_scan
     probe = sk.Probe(sk.ElectronBeam(w.energy), pot.grid, sk.Aperture(w.semiangle))
     prepared = pot.prepare(probe.beam)
@@ -93,12 +94,20 @@ def test_default_batch_is_sized_to_the_infinity_cache(haadf_scan):
         assert sk.last_timing()["batch"] == 128
         one = sk.STEM.scan(probe, [prepared], scan, det, sk.ExecutionOptions(streams=1))[0]
         assert sk.last_timing()["batch"] == 171
-        fixed = sk.STEM.scan(probe, [prepared], scan, det, sk.ExecutionOptions(batch=1024))[0]
-        assert sk.last_timing()["batch"] == 1024
+        tiled = sk.STEM.scan(probe, [prepared], scan, det, sk.ExecutionOptions(batch=1024))[0]
+        assert (sk.last_timing()["batch"], sk.last_timing()["group"]) == (128, 1024)
+        streamed = sk.STEM.scan(probe, [prepared], scan, det, sk.ExecutionOptions(batch=1024, tile=1024))[0]
+        assert (sk.last_timing()["batch"], sk.last_timing()["group"]) == (1024, 1024)
     finally:
         sk.set_timing(False)
-    assert rel_max(fixed.values, haadf.values) <= 1e-12
+    assert rel_max(tiled.values, haadf.values) <= 1e-12
+    assert rel_max(streamed.values, haadf.values) <= 1e-12
     assert rel_max(one.values, haadf.values) <= 1e-12
+    # 1024 positions per graph replay, in cache-sized tiles on two streams, and whole from HBM
+    for o in (sk.ExecutionOptions(batch=1024, use_graph=True), sk.ExecutionOptions(batch=1024, tile=1024,
+                                                                                   use_graph=True)):
+        g = sk.STEM.scan(probe, [prepared], scan, det, o)[0]
+        assert rel_max(g.values, haadf.values) <= 1e-12
 
 
 @pytest.mark.parametrize("n", [256, 512])
