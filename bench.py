@@ -371,7 +371,7 @@ def cpu_baseline(args, w, st, pot, positions, haadf, S):
 
     ref_threads = args.cpu_threads or min(8, share)
     base, ref = runs(ref_threads)
-    every = runs(share) if share != ref_threads else base
+    every = runs(share)[0] if share != ref_threads else base
     got = haadf[:k]
     rel_l2 = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
     rel_max = float(np.abs(got - ref).max() / np.abs(ref).max())

@@ -37,6 +37,13 @@ int sk_separable_map(const float* in, int batch, int n, const float* A, int m, f
 /* grad = 2 (pred - label) / n; partials[(n + 255) / 256] receive the squared error's per-block sums (float64) */
 int sk_mse_grad(const float* pred, const float* label, long long n, float* grad, double* partials, void* stream);
 
+/* as sk_mse_grad with a weight per image: element i belongs to image i / per (n a multiple of per), grad =
+ * 2 weights[i / per] (pred - label) / n and partials the per-block sums of weights[i / per] (pred - label)^2 (the
+ * training loss that weights each image by the inverse of its squared norm, so every image counts by its relative
+ * error) */
+int sk_weighted_mse_grad(const float* pred, const float* label, const float* weights, long long per, long long n,
+                         float* grad, double* partials, void* stream);
+
 /* one Adam step (bias-corrected, step >= 1) of the float32 masters p with gradient g and moments m, v; p_bf16 (may be
  * NULL) receives the bfloat16 rounding (to nearest, ties to even) of the updated p */
 int sk_adam_step(float* p, const float* g, float* m, float* v, uint16_t* p_bf16, long long n, float lr, float beta1,

@@ -1,6 +1,7 @@
 """Float64 restatement of the HAADF surrogate's layers (test infrastructure only; the reference has no learned part,
 so these are pinned by the mathematics in tests/test_oracle_surrogate.py: direct loops against circular convolution
-by FFT, backward passes against finite differences, the resampling matrix against interpolation of linear functions).
+by FFT, backward passes against finite differences, the linear resampling matrix against interpolation of linear
+functions, the band-limited one against exact reproduction of trigonometric polynomials).
 
 Shapes: x [batch, cin, h, w], weights [cout, cin, 5, 5], circular padding of 2 on both axes.
 """
@@ -61,6 +62,27 @@ def resample_matrix(m, n):
         f = t - i
         A[u, i % n] += 1 - f
         A[u, (i + 1) % n] += f
+    return A
+
+
+def fourier_resample_matrix(m, n):
+    """The m x n matrix of band-limited periodic interpolation from n samples to m <= n samples over one period, built
+    column by column through the DFT: the n-point spectrum of each unit input, cropped to the frequencies |k| < m / 2
+    with the m-point Nyquist bin (even m) taking the mean of the +m/2 and -m/2 bins, back through the m-point inverse
+    DFT and scaled by m / n."""
+    A = np.zeros((m, n))
+    for i in range(n):
+        e = np.zeros(n)
+        e[i] = 1.0
+        X = np.fft.fft(e)
+        Y = np.zeros(m, dtype=complex)
+        half = (m + 1) // 2
+        Y[:half] = X[:half]
+        if m > 1:
+            Y[m - (m - 1) // 2:] = X[n - (m - 1) // 2:]
+        if m % 2 == 0:
+            Y[m // 2] = 0.5 * (X[m // 2] + X[n - m // 2])
+        A[:, i] = np.real(np.fft.ifft(Y)) * m / n
     return A
 
 

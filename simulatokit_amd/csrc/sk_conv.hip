@@ -166,14 +166,17 @@ __global__ void sep_cols_kernel(const float* tmp, const float* A, int m, int n, 
 }
 
 // g = 2 (pred - label) / count; loss partials per block (fixed-order sums)
-__global__ void mse_grad_kernel(const float* pred, const float* label, long long n, float* g, double* partial) {
+// g = 2 w (pred - label) / n and the block's sum of w (pred - label)^2, w = weights[i / per] (1 when weights is null)
+__global__ void mse_grad_kernel(const float* pred, const float* label, long long n, const float* weights,
+                                long long per, float* g, double* partial) {
   __shared__ double red[256];
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   double e = 0;
   if (i < n) {
     const float d = pred[i] - label[i];
-    g[i] = 2.f * d / float(n);
-    e = (double)d * d;
+    const float wt = weights ? weights[i / per] : 1.f;
+    g[i] = 2.f * wt * d / float(n);
+    e = (double)wt * d * d;
   }
   red[threadIdx.x] = e;
   __syncthreads();
@@ -287,8 +290,21 @@ int sk_mse_grad(const float* pred, const float* label, long long n, float* grad,
   return guarded_call([&] {
     if (!pred || !label || !grad || !partials || n < 1) fail(SK_ERR_INVALID, "mse_grad: invalid");
     if (sk_hip_available() != 1) fail(SK_ERR_UNAVAILABLE, sk_hip_last_error());
-    hipLaunchKernelGGL(mse_grad_kernel, dim3(blocks_for(n)), dim3(256), 0, pick(stream), pred, label, n, grad,
-                       partials);
+    hipLaunchKernelGGL(mse_grad_kernel, dim3(blocks_for(n)), dim3(256), 0, pick(stream), pred, label, n,
+                       (const float*)nullptr, 1LL, grad, partials);
+    SK_HIP(hipGetLastError());
+  });
+}
+
+int sk_weighted_mse_grad(const float* pred, const float* label, const float* weights, long long per, long long n,
+                         float* grad, double* partials, void* stream) {
+  using namespace sk;
+  return guarded_call([&] {
+    if (!pred || !label || !weights || !grad || !partials || n < 1 || per < 1 || n % per)
+      fail(SK_ERR_INVALID, "weighted_mse_grad: invalid");
+    if (sk_hip_available() != 1) fail(SK_ERR_UNAVAILABLE, sk_hip_last_error());
+    hipLaunchKernelGGL(mse_grad_kernel, dim3(blocks_for(n)), dim3(256), 0, pick(stream), pred, label, n, weights,
+                       per, grad, partials);
     SK_HIP(hipGetLastError());
   });
 }

@@ -5,8 +5,16 @@ as its error judge and standing in for the whole image when that error exceeds t
 
 Network: 5 x 5 convolutions with circular padding (the cells are periodic), 2 -> 32 -> 32 -> 32 -> 32 -> 32 -> 1
 channels, ReLU between layers, bfloat16 weights, float32 activations and accumulation, all in the library's own HIP
-kernels (sk_conv5_forward / _backward_data / _backward_weights, sk_adam_step); the 128 x 128 output is resampled to
-the 96 x 96 scan grid by a fixed periodic linear interpolation (sk_separable_map). Torch only holds device memory.
+kernels (sk_conv5_forward / _backward_data / _backward_weights, sk_weighted_mse_grad, sk_adam_step); the 128 x 128
+output is resampled to the 96 x 96 scan grid by a fixed separable map (sk_separable_map): band-limited (Fourier)
+interpolation by default, which is exact for images without spatial frequencies at or above the 96-point grid's
+Nyquist limit (a HAADF image is band-limited by the probe at 2 alpha / lambda, 47.8 cycles per 30 Angstrom at
+20 mrad and 200 keV), or periodic linear interpolation. Torch only holds device memory.
+
+Input: the cell's summed projected potential on a 128 x 128 grid and a thickness channel (the specified input), or,
+as a labelled variant (`depth_slabs` = K > 0), the projected potential summed over each of K equal depth ranges of
+[0, t_max) and the thickness channel: the probe spreads with depth, so the same projected atom scatters differently
+at the entrance and the exit of the cell, which the summed projection cannot tell apart.
 """
 from __future__ import annotations
 
@@ -82,13 +90,25 @@ class SurrogateWorkload:
         prepared.release()
         return out.reshape(self.scan_n, self.scan_n) if positions is None else out
 
-    def features(self, st: AtomicStructure) -> np.ndarray:
-        """The network's input: the summed projected potential on an input_n^2 grid and a thickness channel."""
+    def features(self, st: AtomicStructure, depth_slabs: int = 0) -> np.ndarray:
+        """The network's input: the summed projected potential on an input_n^2 grid and a thickness channel; with
+        depth_slabs = K > 0 the projected potential summed over each of K equal depth ranges of [0, t_max) (a slice
+        counts in the range that holds its centre) instead of the one sum."""
         pot = Potential(st, self.input_n, self.input_n, slice_thickness=self.slice_thickness)
         prepared = pot.prepare(ElectronBeam(self.energy))
-        vsum = prepared.values().sum(axis=0)
+        V = prepared.values()
         prepared.release()
-        return np.stack([vsum, np.full_like(vsum, st.box[2])]).astype(np.float32)
+        thick = np.full(V.shape[1:], st.box[2])
+        if depth_slabs <= 0:
+            return np.stack([V.sum(axis=0), thick]).astype(np.float32)
+        edges = np.cumsum(pot.slice_thicknesses)
+        centres = edges - 0.5 * np.asarray(pot.slice_thicknesses)
+        slab = np.minimum((centres / (self.t_max / depth_slabs)).astype(int), depth_slabs - 1)
+        out = np.zeros((depth_slabs + 1,) + V.shape[1:])
+        for j, k in enumerate(slab):
+            out[k] += V[j]
+        out[depth_slabs] = thick
+        return out.astype(np.float32)
 
 
 @dataclass
@@ -98,13 +118,16 @@ class Dataset:
     y: np.ndarray   # [n, scan_n, scan_n] float32 HAADF (fraction of the incident intensity)
 
 
-def make_dataset(work: SurrogateWorkload, seeds: Sequence[int], log=None) -> Dataset:
+def make_dataset(work: SurrogateWorkload, seeds: Sequence[int], log=None, depth_slabs: int = 0,
+                 labels: Optional[np.ndarray] = None) -> Dataset:
+    """The cells of `seeds`: their features and their multislice HAADF images (or the given `labels`, images made
+    earlier from the same seeds)."""
     xs, ys = [], []
     t0 = time.perf_counter()
     for k, s in enumerate(seeds):
         st = work.cell(int(s))
-        xs.append(work.features(st))
-        ys.append(work.haadf(st).astype(np.float32))
+        xs.append(work.features(st, depth_slabs))
+        ys.append(work.haadf(st).astype(np.float32) if labels is None else np.asarray(labels[k], np.float32))
         if log and (k + 1) % max(1, len(seeds) // 20) == 0:
             log(f"cells {k + 1}/{len(seeds)} in {time.perf_counter() - t0:.1f} s")
     return Dataset(list(seeds), np.stack(xs), np.stack(ys))
@@ -126,10 +149,27 @@ def _bind():
     L.sk_relu_backward.argtypes = [vp, vp, ll, vp]
     L.sk_separable_map.argtypes = [vp, C.c_int, C.c_int, vp, C.c_int, vp, vp, vp]
     L.sk_mse_grad.argtypes = [vp, vp, ll, vp, vp, vp]
+    L.sk_weighted_mse_grad.argtypes = [vp, vp, vp, ll, ll, vp, vp, vp]
     L.sk_adam_step.argtypes = [vp, vp, vp, vp, vp, ll, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, vp]
     L.sk_to_bf16.argtypes = [vp, vp, ll, vp]
     L._surrogate_bound = True
     return L
+
+
+def fourier_resample_matrix(m: int, n: int) -> np.ndarray:
+    """Band-limited periodic interpolation from n samples to m <= n samples over one period (output u at input
+    coordinate u n / m): the n-point DFT cropped to the m-point grid's frequencies |k| < m / 2, with the m-point
+    Nyquist frequency (even m) kept as a cosine, then the inverse m-point DFT. Real; exact for periodic signals with
+    no frequency at or above m / 2 cycles per period."""
+    if m > n:
+        raise ValueError("fourier_resample_matrix: m must not exceed n")
+    d = np.arange(m)[:, None] / m - np.arange(n)[None, :] / n  # output position minus input position, in periods
+    A = np.ones((m, n))
+    for k in range(1, (m + 1) // 2):
+        A += 2.0 * np.cos(2.0 * np.pi * k * d)
+    if m % 2 == 0:
+        A += np.cos(np.pi * m * d)
+    return A / n
 
 
 def resample_matrix(m: int, n: int) -> np.ndarray:
@@ -148,13 +188,19 @@ class HaadfSurrogate:
     """The network on the device. Weights: float32 masters (trained) and their bfloat16 rounding (used)."""
 
     def __init__(self, seed: int = 0, channels: Sequence[int] = CHANNELS, input_n: int = INPUT_N,
-                 output_n: int = SCAN_N):
+                 output_n: int = SCAN_N, resample: str = "fourier", depth_slabs: int = 0):
         import torch
 
         self.torch = torch
         self.dev = torch.device("cuda:0")
         self.channels = list(channels)
         self.input_n, self.output_n = input_n, output_n
+        if resample not in ("fourier", "linear"):
+            raise ValueError("resample must be 'fourier' or 'linear'")
+        self.resample, self.depth_slabs = resample, int(depth_slabs)
+        want = self.depth_slabs + 1 if self.depth_slabs > 0 else 2
+        if self.channels[0] != want:
+            raise ValueError(f"the first layer takes {want} input channels with depth_slabs={self.depth_slabs}")
         rng = np.random.default_rng(seed)
         self.w32, self.wbf, self.b = [], [], []
         for k, (ci, co) in enumerate(zip(self.channels[:-1], self.channels[1:])):
@@ -163,7 +209,8 @@ class HaadfSurrogate:
             self.w32.append(torch.tensor(w, device=self.dev))
             self.wbf.append(torch.empty(w.size, dtype=torch.int16, device=self.dev))
             self.b.append(torch.zeros(co, dtype=torch.float32, device=self.dev))
-        self.A = torch.tensor(resample_matrix(output_n, input_n), dtype=torch.float32, device=self.dev)
+        A = fourier_resample_matrix(output_n, input_n) if resample == "fourier" else resample_matrix(output_n, input_n)
+        self.A = torch.tensor(A, dtype=torch.float32, device=self.dev)
         self.At = self.A.t().contiguous()
         self.norm = {"v": 1.0, "t": 100.0, "y": 1.0}
         self._publish()
@@ -183,12 +230,12 @@ class HaadfSurrogate:
 
     # -- normalisation
     def fit_normalisation(self, data: Dataset):
-        self.norm = {"v": float(np.std(data.x[:, 0])) or 1.0, "t": 100.0, "y": float(np.mean(data.y)) or 1.0}
+        self.norm = {"v": float(np.std(data.x[:, :-1])) or 1.0, "t": 100.0, "y": float(np.mean(data.y)) or 1.0}
 
     def _inputs(self, x: np.ndarray):
         x = np.array(x, dtype=np.float32, copy=True)
-        x[:, 0] /= self.norm["v"]
-        x[:, 1] /= self.norm["t"]
+        x[:, :-1] /= self.norm["v"]  # the potential channels
+        x[:, -1] /= self.norm["t"]   # the thickness channel
         return self.torch.tensor(x, device=self.dev)
 
     # -- forward / backward
@@ -262,11 +309,19 @@ class HaadfSurrogate:
 
     # -- training
     def train(self, data: Dataset, epochs: int = 60, batch: int = 16, lr: float = 2e-3, seed: int = 0,
-              log=None, augment: bool = True) -> List[float]:
+              log=None, augment: bool = True, loss: str = "relative") -> List[float]:
+        """Adam on the squared error of the normalised images; loss="relative" weights each image by the inverse of
+        its squared norm (scaled to a mean weight of one), so that thin cells, whose HAADF images are faint, count as
+        much as thick ones by their relative error; loss="mse" weights every pixel alike."""
         torch, L = self.torch, _bind()
+        if loss not in ("relative", "mse"):
+            raise ValueError("loss must be 'relative' or 'mse'")
         self.fit_normalisation(data)
         X = self._inputs(data.x)
         Y = torch.tensor(data.y / self.norm["y"], dtype=torch.float32, device=self.dev)
+        norms2 = np.sum(np.asarray(data.y, np.float64).reshape(len(data.y), -1) ** 2, axis=1)
+        weights = np.mean(norms2) / np.maximum(norms2, 1e-300) if loss == "relative" else np.ones(len(norms2))
+        Wimg = torch.tensor(weights, dtype=torch.float32, device=self.dev)
         m = [torch.zeros_like(p) for p in self.parameters()]
         v = [torch.zeros_like(p) for p in self.parameters()]
         rng = np.random.default_rng(seed)
@@ -279,14 +334,15 @@ class HaadfSurrogate:
             for s0 in range(0, n - batch + 1, batch):
                 idx = torch.tensor(order[s0:s0 + batch], device=self.dev)
                 xb, yb = X.index_select(0, idx).contiguous(), Y.index_select(0, idx).contiguous()
+                wb = Wimg.index_select(0, idx).contiguous()
                 if augment:
                     xb, yb = self._augment(xb, yb, rng)
                 out, acts = self.forward(xb, keep=True)
                 g = torch.empty_like(out)
                 nb = out.numel()
                 partial = torch.empty((nb + 255) // 256, dtype=torch.float64, device=self.dev)
-                _check(L.sk_mse_grad(out.data_ptr(), yb.data_ptr(), nb, g.data_ptr(), partial.data_ptr(),
-                                     self._stream), "mse_grad")
+                _check(L.sk_weighted_mse_grad(out.data_ptr(), yb.data_ptr(), wb.data_ptr(), nb // len(idx), nb,
+                                              g.data_ptr(), partial.data_ptr(), self._stream), "mse_grad")
                 grads = self.backward(acts, g)
                 step += 1
                 for k, (dw, db) in enumerate(grads):
@@ -311,21 +367,45 @@ class HaadfSurrogate:
         return out.cpu().numpy() * self.norm["y"]
 
     # -- persistence
-    def state(self) -> Dict[str, np.ndarray]:
-        d = {f"w{k}": w.cpu().numpy() for k, w in enumerate(self.w32)}
+    def state(self, compact: bool = False) -> Dict[str, np.ndarray]:
+        """The weights (float32 masters, or with compact=True only the bfloat16 bits the network uses), biases,
+        normalisation, channel list, resampling and depth slabs."""
+        if compact:
+            d = {f"wbf{k}": w.cpu().numpy().view(np.uint16) for k, w in enumerate(self.wbf)}
+        else:
+            d = {f"w{k}": w.cpu().numpy() for k, w in enumerate(self.w32)}
         d.update({f"b{k}": b.cpu().numpy() for k, b in enumerate(self.b)})
         d["norm"] = np.array([self.norm["v"], self.norm["t"], self.norm["y"]])
         d["channels"] = np.array(self.channels)
+        d["resample"] = np.array(self.resample)
+        d["depth_slabs"] = np.array(self.depth_slabs)
+        d["io"] = np.array([self.input_n, self.output_n])
         return d
 
-    def save(self, path: str):
-        np.savez(path, **self.state())
+    def save(self, path: str, compact: bool = False):
+        np.savez(path, **self.state(compact))
+
+    @classmethod
+    def from_file(cls, path: str) -> "HaadfSurrogate":
+        """A network with the shape, resampling and weights saved in `path`."""
+        d = np.load(path, allow_pickle=False)
+        io = d["io"] if "io" in d.files else np.array([INPUT_N, SCAN_N])
+        model = cls(channels=[int(c) for c in d["channels"]], input_n=int(io[0]), output_n=int(io[1]),
+                    resample=str(d["resample"]) if "resample" in d.files else "linear",
+                    depth_slabs=int(d["depth_slabs"]) if "depth_slabs" in d.files else 0)
+        model.load(path)
+        return model
 
     def load(self, path: str):
-        d = np.load(path)
+        d = np.load(path, allow_pickle=False)
+        torch = self.torch
         for k in range(len(self.w32)):
-            self.w32[k].copy_(self.torch.tensor(d[f"w{k}"], device=self.dev))
-            self.b[k].copy_(self.torch.tensor(d[f"b{k}"], device=self.dev))
+            if f"w{k}" in d.files:
+                self.w32[k].copy_(torch.tensor(d[f"w{k}"], device=self.dev))
+            else:  # compact file: the masters become the bfloat16 values themselves
+                bits = d[f"wbf{k}"].astype(np.uint32) << 16
+                self.w32[k].copy_(torch.tensor(bits.view(np.float32).reshape(self.w32[k].shape), device=self.dev))
+            self.b[k].copy_(torch.tensor(d[f"b{k}"], device=self.dev))
         nv, nt, ny = d["norm"]
         self.norm = {"v": float(nv), "t": float(nt), "y": float(ny)}
         self._publish()
@@ -347,7 +427,7 @@ def judged_scan(model: HaadfSurrogate, work: SurrogateWorkload, st: AtomicStruct
     """The surrogate's HAADF image of cell `st`, judged by plain multislice on a random `fraction` of the positions;
     when the network's relative L2 error there exceeds `tolerance`, the whole image is re-simulated."""
     t0 = time.perf_counter()
-    img = model.predict(work.features(st)[None])[0]
+    img = model.predict(work.features(st, model.depth_slabs)[None])[0]
     t1 = time.perf_counter()
     npos = work.scan_n * work.scan_n
     k = max(1, int(round(fraction * npos)))
@@ -367,78 +447,178 @@ def judged_scan(model: HaadfSurrogate, work: SurrogateWorkload, st: AtomicStruct
 
 
 __all__ = ["SurrogateWorkload", "Dataset", "make_dataset", "HaadfSurrogate", "JudgedScan", "judged_scan",
-           "resample_matrix", "CHANNELS"]
+           "resample_matrix", "fourier_resample_matrix", "generate_labels", "evaluate", "run_study", "CHANNELS"]
 
 
 # ---- the study: train on generated cells, judge on held-out cells -----------------------------------------------------
+def generate_labels(work: SurrogateWorkload, seeds: Sequence[int], log=None) -> Tuple[np.ndarray, np.ndarray]:
+    """The multislice HAADF images [len(seeds), scan_n, scan_n] of the seeded cells (the training labels) and the
+    cells' thicknesses."""
+    out = np.empty((len(seeds), work.scan_n, work.scan_n), dtype=np.float32)
+    thick = np.empty(len(seeds))
+    t0 = time.perf_counter()
+    for k, s in enumerate(seeds):
+        st = work.cell(int(s))
+        thick[k] = st.box[2]
+        out[k] = work.haadf(st)
+        if log and (k + 1) % max(1, len(seeds) // 40) == 0:
+            log(f"labels {k + 1}/{len(seeds)} in {time.perf_counter() - t0:.1f} s")
+    return out, thick
+
+
+def evaluate(model: HaadfSurrogate, work: SurrogateWorkload, seeds: Sequence[int], tolerance: float = 0.02,
+             fraction: float = 0.05, log=print) -> Dict:
+    """Scans each held-out cell both ways, plain multislice with the library's default plan (the truth and the
+    baseline time) and the judged surrogate, and returns the errors, fallbacks and throughputs."""
+    import torch
+
+    rows = []
+    for s in seeds:
+        st = work.cell(int(s))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        truth = work.haadf(st)
+        plain_s = time.perf_counter() - t0
+        js = judged_scan(model, work, st, fraction=fraction, tolerance=tolerance, seed=int(s))
+        err_net = float(np.linalg.norm(js.surrogate - truth) / np.linalg.norm(truth))
+        err_out = float(np.linalg.norm(js.image - truth) / np.linalg.norm(truth))
+        slices = math.ceil(st.box[2] / work.slice_thickness)
+        rows.append({"seed": int(s), "thickness": float(st.box[2]), "atoms": len(st.numbers), "slices": slices,
+                     "plain_s": plain_s, "judged_s": js.seconds["total"],
+                     "surrogate_and_judge_s": js.seconds["surrogate"] + js.seconds["judge"],
+                     "surrogate_err": err_net, "delivered_err": err_out, "spot_err": js.spot_error,
+                     "fallback": bool(js.fallback)})
+        if log:
+            log(f"held-out {s} ({st.box[2]:.0f} A): network {err_net:.4f}, spot {js.spot_error:.4f}, fallback "
+                f"{js.fallback}, plain {plain_s * 1e3:.0f} ms, judged {js.seconds['total'] * 1e3:.0f} ms")
+    npos = work.scan_n ** 2
+    work_ps = sum(npos * r["slices"] for r in rows)
+    plain_rate = work_ps / sum(r["plain_s"] for r in rows)
+    judged_rate = work_ps / sum(r["judged_s"] for r in rows)
+    accepted_rate = work_ps / sum(r["surrogate_and_judge_s"] for r in rows)
+    errs = np.array([r["surrogate_err"] for r in rows])
+    return {"held_out_cells": len(rows), "tolerance": tolerance, "fraction": fraction,
+            "network_err_mean": float(errs.mean()), "network_err_median": float(np.median(errs)),
+            "network_err_min": float(errs.min()), "network_err_max": float(errs.max()),
+            "accepted": int(sum(not r["fallback"] for r in rows)),
+            "delivered_err_max": float(np.max([r["delivered_err"] for r in rows])),
+            "delivered_err_mean": float(np.mean([r["delivered_err"] for r in rows])),
+            "fallbacks": int(sum(r["fallback"] for r in rows)),
+            "plain_pos_slices_per_s": plain_rate, "judged_pos_slices_per_s": judged_rate,
+            "speedup": judged_rate / plain_rate,
+            "accepted_pos_slices_per_s": accepted_rate, "speedup_when_accepted": accepted_rate / plain_rate,
+            "rows": rows}
+
+
 def run_study(train: int = 256, held_out: int = 32, epochs: int = 60, batch: int = 16, seed: int = 20261202,
               work: SurrogateWorkload = SurrogateWorkload(), tolerance: float = 0.02, fraction: float = 0.05,
-              weights: Optional[str] = None, log=print, lr: float = 2e-3) -> Dict:
+              weights: Optional[str] = None, log=print, lr: float = 2e-3, depth_slabs: int = 0) -> Dict:
     """Generates `train` + `held_out` seeded cells with the multislice kernels, trains the network on the first, and
     scans the others both ways: plain multislice (the truth and the baseline time) and the judged surrogate. Returns
     the errors, fallback count and the speed-up."""
-    import torch
-
     seeds = [seed + i for i in range(train + held_out)]
     t0 = time.perf_counter()
-    data = make_dataset(work, seeds[:train], log=log)
+    data = make_dataset(work, seeds[:train], log=log, depth_slabs=depth_slabs)
     gen_s = time.perf_counter() - t0
-    model = HaadfSurrogate(seed=seed)
+    channels = [depth_slabs + 1 if depth_slabs > 0 else 2] + CHANNELS[1:]
+    model = HaadfSurrogate(seed=seed, channels=channels, depth_slabs=depth_slabs)
     t0 = time.perf_counter()
     history = model.train(data, epochs=epochs, batch=batch, lr=lr, seed=seed, log=log)
     train_s = time.perf_counter() - t0
     if weights:
         model.save(weights)
-    rows = []
-    for s in seeds[train:]:
-        st = work.cell(s)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        truth = work.haadf(st)
-        plain_s = time.perf_counter() - t0
-        js = judged_scan(model, work, st, fraction=fraction, tolerance=tolerance, seed=s)
-        err_net = float(np.linalg.norm(js.surrogate - truth) / np.linalg.norm(truth))
-        err_out = float(np.linalg.norm(js.image - truth) / np.linalg.norm(truth))
-        slices = math.ceil(st.box[2] / work.slice_thickness)
-        rows.append({"seed": s, "slices": slices, "plain_s": plain_s, "judged_s": js.seconds["total"],
-                     "surrogate_and_judge_s": js.seconds["surrogate"] + js.seconds["judge"],
-                     "surrogate_err": err_net, "delivered_err": err_out, "spot_err": js.spot_error,
-                     "fallback": js.fallback})
-        log(f"held-out {s}: network {err_net:.4f}, spot {js.spot_error:.4f}, fallback {js.fallback}, "
-            f"plain {plain_s * 1e3:.0f} ms, judged {js.seconds['total'] * 1e3:.0f} ms")
-    npos = work.scan_n ** 2
-    plain_rate = sum(npos * r["slices"] for r in rows) / sum(r["plain_s"] for r in rows)
-    judged_rate = sum(npos * r["slices"] for r in rows) / sum(r["judged_s"] for r in rows)
-    accepted_rate = sum(npos * r["slices"] for r in rows) / sum(r["surrogate_and_judge_s"] for r in rows)
-    return {"train_cells": train, "held_out_cells": held_out, "epochs": epochs, "generate_s": gen_s,
-            "train_s": train_s, "final_mse": history[-1], "first_mse": history[0],
-            "network_err_mean": float(np.mean([r["surrogate_err"] for r in rows])),
-            "delivered_err_max": float(np.max([r["delivered_err"] for r in rows])),
-            "fallbacks": int(sum(r["fallback"] for r in rows)),
-            "plain_pos_slices_per_s": plain_rate, "judged_pos_slices_per_s": judged_rate,
-            "speedup": judged_rate / plain_rate,
-            "accepted_pos_slices_per_s": accepted_rate, "speedup_when_accepted": accepted_rate / plain_rate,
-            "network_err_median": float(np.median([r["surrogate_err"] for r in rows])), "rows": rows}
+    res = evaluate(model, work, seeds[train:], tolerance=tolerance, fraction=fraction, log=log)
+    res.update({"train_cells": train, "epochs": epochs, "generate_s": gen_s, "train_s": train_s,
+                "final_loss": history[-1], "first_loss": history[0], "depth_slabs": depth_slabs})
+    return res
 
 
-if __name__ == "__main__":
+def _main(argv=None):
     import argparse
     import json
 
-    ap = argparse.ArgumentParser(description="train the HAADF surrogate on generated cells and judge it")
-    ap.add_argument("--train", type=int, default=256)
-    ap.add_argument("--held-out", type=int, default=32)
-    ap.add_argument("--epochs", type=int, default=60)
-    ap.add_argument("--batch", type=int, default=16)
-    ap.add_argument("--seed", type=int, default=20261202)
-    ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--weights", default=None)
-    ap.add_argument("--out", default=None, help="write the summary JSON here")
-    a = ap.parse_args()
-    res = run_study(a.train, a.held_out, a.epochs, a.batch, a.seed, weights=a.weights,
-                    log=lambda m: print(m, flush=True), lr=a.lr)
-    summary = {k: v for k, v in res.items() if k != "rows"}
-    print(json.dumps(summary), flush=True)
-    if a.out:
-        with open(a.out, "w") as f:
-            json.dump(res, f, indent=1)
+    ap = argparse.ArgumentParser(description="the HAADF surrogate: generate labels, train, evaluate (or all three)")
+    sub = ap.add_subparsers(dest="cmd")
+    g = sub.add_parser("generate", help="multislice HAADF labels of seeded cells")
+    g.add_argument("--first", type=int, default=20261202, help="seed of the first cell")
+    g.add_argument("--count", type=int, required=True)
+    g.add_argument("--out", required=True, help=".npz: seeds, thickness, labels")
+    g.add_argument("--dtype", default="float32", choices=["float32", "float16"])
+    t = sub.add_parser("train", help="train on label files written by `generate`")
+    t.add_argument("--labels", nargs="+", required=True)
+    t.add_argument("--count", type=int, default=0, help="use the first COUNT cells (0: all)")
+    t.add_argument("--epochs", type=int, default=150)
+    t.add_argument("--batch", type=int, default=16)
+    t.add_argument("--lr", type=float, default=2e-3)
+    t.add_argument("--loss", default="relative", choices=["relative", "mse"])
+    t.add_argument("--resample", default="fourier", choices=["fourier", "linear"])
+    t.add_argument("--depth-slabs", type=int, default=0)
+    t.add_argument("--seed", type=int, default=20261202)
+    t.add_argument("--weights", required=True, help="output .npz")
+    t.add_argument("--compact", action="store_true", help="save only the bfloat16 weights the network uses")
+    e = sub.add_parser("evaluate", help="judge held-out cells with saved weights")
+    e.add_argument("--weights", required=True)
+    e.add_argument("--first", type=int, default=20261202 + 2000)
+    e.add_argument("--count", type=int, default=256)
+    e.add_argument("--tolerance", type=float, default=0.02)
+    e.add_argument("--fraction", type=float, default=0.05)
+    e.add_argument("--out", default=None, help="write the summary JSON here")
+    st = sub.add_parser("study", help="generate, train and evaluate in one run")
+    st.add_argument("--train", type=int, default=256)
+    st.add_argument("--held-out", type=int, default=32)
+    st.add_argument("--epochs", type=int, default=60)
+    st.add_argument("--batch", type=int, default=16)
+    st.add_argument("--seed", type=int, default=20261202)
+    st.add_argument("--lr", type=float, default=2e-3)
+    st.add_argument("--depth-slabs", type=int, default=0)
+    st.add_argument("--weights", default=None)
+    st.add_argument("--out", default=None, help="write the summary JSON here")
+    a = ap.parse_args(argv)
+    log = lambda m: print(m, flush=True)  # noqa: E731
+    work = SurrogateWorkload()
+    if a.cmd == "generate":
+        seeds = list(range(a.first, a.first + a.count))
+        t0 = time.perf_counter()
+        y, thick = generate_labels(work, seeds, log=log)
+        np.savez(a.out, seeds=np.array(seeds), thickness=thick, labels=y.astype(a.dtype))
+        log(json.dumps({"cells": len(seeds), "seconds": time.perf_counter() - t0, "out": a.out}))
+    elif a.cmd == "train":
+        seeds, labels = [], []
+        for f in a.labels:
+            d = np.load(f, allow_pickle=False)
+            seeds += [int(v) for v in d["seeds"]]
+            labels.append(d["labels"].astype(np.float32))
+        labels = np.concatenate(labels)
+        if a.count:
+            seeds, labels = seeds[:a.count], labels[:a.count]
+        t0 = time.perf_counter()
+        data = make_dataset(work, seeds, log=log, depth_slabs=a.depth_slabs, labels=labels)
+        log(f"features of {len(seeds)} cells in {time.perf_counter() - t0:.1f} s")
+        channels = [a.depth_slabs + 1 if a.depth_slabs > 0 else 2] + CHANNELS[1:]
+        model = HaadfSurrogate(seed=a.seed, channels=channels, resample=a.resample, depth_slabs=a.depth_slabs)
+        t0 = time.perf_counter()
+        hist = model.train(data, epochs=a.epochs, batch=a.batch, lr=a.lr, seed=a.seed, log=log, loss=a.loss)
+        model.save(a.weights, compact=a.compact)
+        log(json.dumps({"cells": len(seeds), "epochs": a.epochs, "train_s": time.perf_counter() - t0,
+                        "first_loss": hist[0], "final_loss": hist[-1], "weights": a.weights}))
+    elif a.cmd == "evaluate":
+        model = HaadfSurrogate.from_file(a.weights)
+        res = evaluate(model, work, list(range(a.first, a.first + a.count)), tolerance=a.tolerance,
+                       fraction=a.fraction, log=log)
+        log(json.dumps({k: v for k, v in res.items() if k != "rows"}))
+        if a.out:
+            with open(a.out, "w") as f:
+                json.dump(res, f, indent=1)
+    else:
+        if a.cmd is None:
+            a = ap.parse_args(["study"] + list(argv or []))
+        res = run_study(a.train, a.held_out, a.epochs, a.batch, a.seed, weights=a.weights, log=log, lr=a.lr,
+                        depth_slabs=a.depth_slabs)
+        log(json.dumps({k: v for k, v in res.items() if k != "rows"}))
+        if a.out:
+            with open(a.out, "w") as f:
+                json.dump(res, f, indent=1)
+
+
+if __name__ == "__main__":
+    _main()

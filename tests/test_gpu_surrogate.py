@@ -86,7 +86,8 @@ def test_network_gradients_equal_the_oracle():
     grads = model.backward(acts, torch.tensor(dout, device=model.dev))
     layers = [(S.bf16_round(w.cpu().numpy()), b.cpu().numpy()) for w, b in zip(model.w32, model.b)]
     oacts = S.network_forward(x, layers)
-    A = S.resample_matrix(24, 32)
+    assert model.resample == "fourier"
+    A = S.fourier_resample_matrix(24, 32)
     assert rel(out.cpu().numpy(), S.separable_map(oacts[-1][:, 0], A)) <= 1e-5
     g_last = S.separable_map(dout, A.T)[:, None]
     ograds = S.network_backward(oacts, layers, g_last)

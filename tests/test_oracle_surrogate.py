@@ -81,6 +81,31 @@ def test_resampling_matrix_interpolates_linear_functions_and_keeps_constants():
     assert np.allclose(S.separable_map(x, A)[0], A @ x[0] @ A.T)
 
 
+def test_fourier_resampling_reproduces_band_limited_signals():
+    m, n = 96, 128
+    A = S.fourier_resample_matrix(m, n)
+    t, u = np.arange(n) / n, np.arange(m) / m
+    rng = np.random.default_rng(7)
+    for k in (0, 1, 5, 31, 47):  # every frequency below the 96-point Nyquist limit comes through exactly
+        ph = rng.uniform(0, 2 * np.pi)
+        assert np.abs(A @ np.cos(2 * np.pi * k * t + ph) - np.cos(2 * np.pi * k * u + ph)).max() <= 1e-12
+    # the 96-point Nyquist cosine keeps half its amplitude (mean of the +48 and -48 bins); frequencies above vanish
+    assert np.abs(A @ np.cos(2 * np.pi * 48 * t) - 0.5 * np.cos(2 * np.pi * 48 * u)).max() <= 1e-12
+    for k in (49, 63, 64):
+        assert np.abs(A @ np.cos(2 * np.pi * k * t) - 0.0).max() <= 1e-12 or k % m == 0
+    # odd m: no Nyquist bin
+    B = S.fourier_resample_matrix(45, 60)
+    assert np.abs(B @ np.sin(2 * np.pi * 22 * np.arange(60) / 60) - np.sin(2 * np.pi * 22 * np.arange(45) / 45)).max() \
+        <= 1e-12
+
+
+def test_product_fourier_resampling_equals_the_oracle():
+    from simulatokit_amd.surrogate import fourier_resample_matrix
+
+    for m, n in ((96, 128), (48, 64), (45, 60), (24, 32)):
+        assert np.abs(fourier_resample_matrix(m, n) - S.fourier_resample_matrix(m, n)).max() <= 1e-13
+
+
 def test_bf16_rounding_is_nearest_even():
     v = np.array([1.0, 1.00390625, 1.005859375, 1.0078125, -2.0 ** -133, 3.140625], dtype=np.float32)
     r = S.bf16_round(v)
