@@ -52,10 +52,10 @@ def measured_traffic(kernel: str, batch: int):
         return None
     end = text.find("```", start + 7)
     data = json.loads(text[start + 7:end])
-    entry = data.get(kernel)
-    if entry is None or int(entry.get("batch", -1)) != batch:
-        return None
-    return float(entry["hbm_bytes_per_launch"])
+    for entry in data.get("entries", []):
+        if entry["kernel"] == kernel and int(entry["positions_per_launch"]) == batch:
+            return float(entry["hbm_bytes_per_launch"])
+    return None
 
 
 def main():
