@@ -195,6 +195,22 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
                 const sk_detector* dets, int ndets, void* const* outputs, int outputs_on_device,
                 const sk_options* opt);
 
+/* Build-only, with no counterpart in the reference: sk_hip_scan for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). The same scan as sk_hip_scan
+ * with annular detectors only, run on one stream without graph replay, lanes or tiles; it calls no hipMalloc,
+ * hipFree, hipHostMalloc or allocating library routine. Its device memory is one static 64 MiB array per device,
+ * held until the process ends: the prepared potentials, the positions and the outputs stay the caller's, and the
+ * array holds the probe spectrum, the positions, the detector sums (npos x ndets float64, plus npos x ndets more for
+ * host outputs) and the waves of as many positions per launch as the rest holds (at 512 x 512 about 40). A scan whose
+ * fixed part leaves no room for one position, or a grid length above 4096, is refused with SK_ERR_MEMORY and a
+ * message naming the limit. Every call on a device shares that device's array, which keeps the last call's contents
+ * (the call clears what must start at zero): calls that overlap on one device race. The potentials must have been
+ * prepared by this library (which builds the transform tables this entry point reads). The sums, per position, equal
+ * sk_hip_scan's with the same positions per launch. */
+int sk_hip_scan_noalloc(const sk_potential* const* pots, int npots, const sk_probe* probe, int npos,
+                        const double* positions, const sk_detector* dets, int ndets, void* const* outputs,
+                        int outputs_on_device, const sk_options* opt);
+
 /* The complex exit wave of every position's probe through one potential (WavesDetector, STEM.scan(probe:potentials:
  * scan:detector:), WavesDetector.swift:107-215; abTEM's WavesDetector(gpts)): rows x cols > 0 resamples each wave in
  * Fourier space to that many points over the same extent (Waves.resampled, WavesResample.swift:16-49); 0 keeps the

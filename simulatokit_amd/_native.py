@@ -104,6 +104,7 @@ def lib():
         L.sk_hip_probes.argtypes = [C.POINTER(SkGrid), C.POINTER(SkProbe), C.c_int, _dp, _dp, _dp]
         L.sk_hip_scan.argtypes = [C.POINTER(_vp), C.c_int, C.POINTER(SkProbe), C.c_int, _dp, C.POINTER(SkDetector),
                                   C.c_int, C.POINTER(_vp), C.c_int, C.POINTER(SkOptions)]
+        L.sk_hip_scan_noalloc.argtypes = L.sk_hip_scan.argtypes
         L.sk_hip_scan_intensities.argtypes = [_vp, C.POINTER(SkProbe), C.c_int, _dp, DELIVER_FN, _vp,
                                               C.POINTER(SkOptions)]
         L.sk_hip_scan_waves.argtypes = [_vp, C.POINTER(SkProbe), C.c_int, _dp, C.c_int, C.c_int, _dp, _dp,
@@ -139,7 +140,7 @@ EXPORTED = ("sk_hip_available", "sk_hip_last_error", "sk_hip_build_id", "sk_hip_
             "sk_hip_potential_transmissions", "sk_hip_run", "sk_hip_run_batch", "sk_hip_probes", "sk_hip_scan",
             "sk_hip_detector_shape", "sk_hip_last_timing", "sk_hip_set_timing", "sk_hip_potential_grid",
             "sk_hip_autotune", "sk_hip_device_name", "sk_hip_potential_from_atoms_finite", "sk_hip_scan_waves", "sk_hip_scan_intensities",
-            "sk_hip_ctf_kernel", "sk_hip_apply_ctf",
+            "sk_hip_ctf_kernel", "sk_hip_apply_ctf", "sk_hip_scan_noalloc",
             # include/simulato_surrogate.h
             "sk_conv5_forward", "sk_conv5_backward_data", "sk_conv5_backward_weights", "sk_relu_backward",
             "sk_separable_map", "sk_mse_grad", "sk_weighted_mse_grad", "sk_adam_step", "sk_to_bf16")

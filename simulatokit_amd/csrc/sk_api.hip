@@ -17,6 +17,12 @@
 
 #include "sk_internal.h"
 
+// The device memory of sk_hip_scan_noalloc: one array per device (each device that loads this library holds its own
+// copy until the process ends). Host code never uses this name as an address: the library's set-up (require()) asks
+// the runtime for the array's device address and every kernel argument, copy and clear uses that address.
+constexpr size_t kNoallocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
 struct sk_potential {
   sk_grid grid{};
   int nslices = 0;
@@ -65,6 +71,10 @@ struct Context {
   int precision = 0;  // the running call's sk_options.precision
   // identity keep maps per length (full-width transforms)
   std::map<int, std::unique_ptr<DevBuf<int>>> identity;
+  // device address of the scratch array on the library's device (null when the runtime gave none; scratch_why says
+  // why), taken once at set-up with that device current
+  unsigned char* scratch_dev = nullptr;
+  std::string scratch_why;
 };
 
 std::recursive_mutex& lock() { return api_lock(); }
@@ -119,6 +129,11 @@ Context& require() {
           }
         c.lobato.upload(tab, c.stream);
         SK_HIP(hipStreamSynchronize(c.stream));
+        // the library runs on device 0, current here: its copy of the scratch array
+        void* sp = nullptr;
+        const hipError_t se = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+        if (se == hipSuccess && sp) c.scratch_dev = static_cast<unsigned char*>(sp);
+        else c.scratch_why = se == hipSuccess ? "null address" : hipGetErrorString(se);
       }
     }
   }
@@ -132,11 +147,16 @@ int guarded(F&& f) {
 }
 
 // ---- host geometry (Grid.swift, Antialias.swift, Detectors.swift) -----------------------------------------------
-std::vector<double> fftfreq(int n, double d) {
+// entry i of NumPy's fftfreq(n, d) (Grid.swift:82-86)
+inline double fftfreq_at(int i, int n, double d) {
   const double val = 1.0 / (double(n) * d);
   const int positive = (n - 1) / 2 + 1;
+  return double(i < positive ? i : i - n) * val;
+}
+
+std::vector<double> fftfreq(int n, double d) {
   std::vector<double> o(n);
-  for (int i = 0; i < n; ++i) o[i] = double(i < positive ? i : i - n) * val;
+  for (int i = 0; i < n; ++i) o[i] = fftfreq_at(i, n, d);
   return o;
 }
 
@@ -656,7 +676,7 @@ struct Timer {
 };
 
 void slice_loop(const sk_potential& p, int batch, float2* G, bool entry_full, float2* full, int first_slice_from_full,
-                const std::vector<int>& plane_out, float2* planes, const ColArgs* final_col, bool fast, Timer& tm,
+                const int* plane_out, float2* planes, const ColArgs* final_col, bool fast, Timer& tm,
                 hipStream_t stream = nullptr) {
   hipStream_t s = stream ? stream : ctx().stream;
   const size_t N = (size_t)p.grid.nx * p.grid.ny;
@@ -673,7 +693,7 @@ void slice_loop(const sk_potential& p, int batch, float2* G, bool entry_full, fl
       r.in_full = full;
     } else {
       r.flags = ROW_IN_COMPACT | ROW_INV | ROW_MUL_T | ROW_FWD | ROW_OUT_COMPACT;
-      if (j > 0 && plane_out[j - 1] >= 0) {
+      if (j > 0 && plane_out && plane_out[j - 1] >= 0) {
         r.flags |= ROW_WRITE_PLANE;
         r.plane = planes + (size_t)plane_out[j - 1] * batch * N;
       }
@@ -698,13 +718,42 @@ void slice_loop(const sk_potential& p, int batch, float2* G, bool entry_full, fl
   }
   if (!final_col) {
     // exit: the real-space wave after the last slice (and any plane still pending)
-    const int q = plane_out[p.nslices - 1];
+    const int q = plane_out ? plane_out[p.nslices - 1] : -1;
     if (q >= 0) {
       r.flags = ROW_IN_COMPACT | ROW_INV | ROW_WRITE_PLANE;
       r.plane = planes + (size_t)q * batch * N;
       row_pass(r, batch, fast, s);
     }
   }
+}
+
+// One batch of nb scan positions starting at b0 (+ *pos_dev when replayed from a graph) through potential P: the
+// shift ramps, the first column pass (the probe, generated from the spectrum), every slice, the detector reduction of
+// the last column pass and the sum of its partials into acc. G, rx, ry and det.partial hold nb positions; `full`
+// (nb full-width waves) is needed only when the probe reaches past the bandlimit (fits = false).
+void scan_batch(const sk_potential& P, const float2* spec, const double* positions, bool fits, bool fast, float2* G,
+                float2* full, float2* rx, float2* ry, const DetPlan& det, int b0, const int* pos_dev, int nb,
+                double* acc, Timer& t, hipStream_t s) {
+  const sk_grid& g = P.grid;
+  const int nx = g.nx, ny = g.ny;
+  launch_ramps(nx, ny, g.extent_x / nx, g.extent_y / ny, positions, b0, pos_dev, nb, rx, ry, s);
+  ColArgs init = col_args(P, fits ? G : full);
+  init.spec = spec;
+  init.rx = rx;
+  init.ry = ry;
+  init.flags = COL_GENERATE | COL_INV | COL_OUT;
+  if (!fits) {
+    init.nc = ny;
+    init.keep = identity_map(ny);
+  }
+  col_pass(init, nb, fast && fits, s);
+  ColArgs fin = col_args(P, G);
+  fin.flags = COL_IN_G | COL_FWD | COL_MUL_P | COL_DETECT;
+  fin.pos0 = b0;
+  fin.pos_dev = pos_dev;
+  fin.det = det;
+  slice_loop(P, nb, G, false, full, fits ? 0 : 1, nullptr, nullptr, &fin, fast, t, s);
+  launch_reduce_partials(det, nb, b0, pos_dev, acc, s);
 }
 
 // The lane pipeline of a scan through several potentials (frozen-phonon configurations of any slice counts).
@@ -903,6 +952,24 @@ void run_pipeline(const sk_potential* const* pots, int npots, int npos, int W, i
 }
 
 }  // namespace
+
+// Pieces of one device buffer, each aligned to 256 bytes.
+struct Arena {
+  unsigned char* base;
+  size_t size, used;
+  size_t aligned(size_t off) const { return (off + 255) & ~size_t(255); }
+  template <class T>
+  T* take(size_t count) {
+    const size_t off = aligned(used), bytes = count * sizeof(T);
+    if (off > size || bytes > size - off) fail(SK_ERR_MEMORY, "sk_hip_scan_noalloc: scratch exhausted");
+    used = off + bytes;
+    return reinterpret_cast<T*>(base + off);
+  }
+  size_t left() const { return aligned(used) >= size ? 0 : size - aligned(used); }
+};
+// host staging of the annular detectors' pixel angles (static: the entry point allocates nothing)
+constexpr int kNoallocMaxLength = 4096;
+static double noalloc_sq[2 * kNoallocMaxLength];
 
 int default_scan_batch(const sk_potential& p, int count) { return default_batch(p, count, nullptr); }
 
@@ -1245,7 +1312,7 @@ static void run_waves(const sk_potential* p, int count, const double* in_re, con
     SK_HIP(hipMemcpyAsync(w.tmp_re.p, in_re + (size_t)b0 * N, nb * N * sizeof(double), hipMemcpyHostToDevice, s));
     SK_HIP(hipMemcpyAsync(w.tmp_im.p, in_im + (size_t)b0 * N, nb * N * sizeof(double), hipMemcpyHostToDevice, s));
     launch_real_to_complex(w.tmp_re.p, w.tmp_im.p, w.full.p, nb * N, s);
-    slice_loop(*p, nb, w.G.p, true, w.full.p, 0, plane_out, w.planes.p, nullptr, fast, tm);
+    slice_loop(*p, nb, w.G.p, true, w.full.p, 0, plane_out.data(), w.planes.p, nullptr, fast, tm);
     for (int q = 0, k = 0; q < nplanes; ++q) {
       double* ore = out_re + ((size_t)q * count + b0) * N;
       double* oim = out_im + ((size_t)q * count + b0) * N;
@@ -1291,6 +1358,19 @@ int sk_hip_run_batch(const sk_potential* p, int count, const double* in_re, cons
 }
 
 // probe spectrum Psi0 / ||Psi0|| * extra on the device; returns whether its support fits the kept columns
+// The probe's centred spectrum Psi_0 = A e^{-i chi} / ||A e^{-i chi}|| times `extra`, as complex64 in `out`
+// (Probe.centredSpectrum, Probe.swift:113-136), with its float64 re / im left in tmp_re / tmp_im; norm and
+// norm_partial (kSpectrumBlocks values) hold the float64 sum of squares.
+constexpr int kSpectrumBlocks = 256;
+static void spectrum_into(const sk_grid& g, const sk_probe& pr, double extra, double* tmp_re, double* tmp_im,
+                          double* norm, double* norm_partial, float2* out, hipStream_t s) {
+  const size_t N = (size_t)g.nx * g.ny;
+  launch_probe_spectrum(g.nx, g.ny, g.extent_x, g.extent_y, wavelength(pr.energy), pr.semiangle_mrad, pr.soft,
+                        pr.aberrations, tmp_re, tmp_im, s);
+  launch_sum_squares(tmp_re, tmp_im, N, norm_partial, kSpectrumBlocks, norm, s);
+  launch_normalise_spectrum(tmp_re, tmp_im, norm, extra, N, out, s);
+}
+
 static bool build_spectrum(const sk_grid& g, const sk_probe& pr, double extra, const std::vector<int>* keepinv,
                            float2* out) {
   Context& c = ctx();
@@ -1300,12 +1380,8 @@ static bool build_spectrum(const sk_grid& g, const sk_probe& pr, double extra, c
   w.tmp_re.ensure(N);
   w.tmp_im.ensure(N);
   w.norm.ensure(1);
-  const int blocks = 256;
-  w.norm_partial.ensure(blocks);
-  launch_probe_spectrum(g.nx, g.ny, g.extent_x, g.extent_y, wavelength(pr.energy), pr.semiangle_mrad, pr.soft,
-                        pr.aberrations, w.tmp_re.p, w.tmp_im.p, s);
-  launch_sum_squares(w.tmp_re.p, w.tmp_im.p, N, w.norm_partial.p, blocks, w.norm.p, s);
-  launch_normalise_spectrum(w.tmp_re.p, w.tmp_im.p, w.norm.p, extra, N, out, s);
+  w.norm_partial.ensure(kSpectrumBlocks);
+  spectrum_into(g, pr, extra, w.tmp_re.p, w.tmp_im.p, w.norm.p, w.norm_partial.p, out, s);
   if (!keepinv) return true;
   // support check: any non-zero amplitude on a pruned column?
   std::vector<double> re(N), im(N);
@@ -1326,6 +1402,28 @@ static void check_probe(const sk_probe* pr) {
   for (double a : pr->aberrations)
     if (!std::isfinite(a)) fail(SK_ERR_INVALID, "invalid aberration: every coefficient must be finite");
   if (!(std::isfinite(pr->tilt[0]) && std::isfinite(pr->tilt[1]))) fail(SK_ERR_INVALID, "invalid tilt: the tilt must be finite");
+}
+
+// The arguments of a scan (STEM.scan's validation, STEM.swift:49-81): potentials on one grid, at the probe's energy
+// and tilt, finite positions, an output per detector.
+static void check_scan(const sk_potential* const* pots, int npots, const sk_probe* probe, int npos,
+                       const double* positions, const sk_detector* dets, int ndets, void* const* outputs) {
+  if (!pots || npots < 1) fail(SK_ERR_INVALID, "invalid scan: no potential");
+  check_probe(probe);
+  if (npos < 1 || !positions) fail(SK_ERR_INVALID, "invalid scan: no positions");
+  if (ndets < 0 || (ndets > 0 && (!dets || !outputs))) fail(SK_ERR_INVALID, "invalid detectors");
+  if (!pots[0]) fail(SK_ERR_INVALID, "null potential");
+  const sk_grid g = pots[0]->grid;
+  for (int k = 0; k < npots; ++k) {
+    if (!pots[k]) fail(SK_ERR_INVALID, "null potential");
+    if (!same_grid(pots[k]->grid, g)) fail(SK_ERR_INVALID, "the waves and the potential are on different grids");
+    if (pots[k]->energy != probe->energy) fail(SK_ERR_INVALID, "the probe's energy differs from the potential's");
+    if (pots[k]->tilt[0] != probe->tilt[0] || pots[k]->tilt[1] != probe->tilt[1])
+      fail(SK_ERR_INVALID, "invalid tilt: the probe's tilt differs from the potential's");
+  }
+  for (int q = 0; q < npos; ++q)
+    if (!(std::isfinite(positions[2 * q]) && std::isfinite(positions[2 * q + 1])))
+      fail(SK_ERR_INVALID, "invalid position");
 }
 
 int sk_hip_probes(const sk_grid* grid, const sk_probe* probe, int npos, const double* positions, double* out_re,
@@ -1536,7 +1634,7 @@ int sk_hip_scan_waves(const sk_potential* p, const sk_probe* probe, int npos, co
       }
       col_pass(init, nb, fast && fits, s);
       // every slice, then the exit wave in real space (the probe's own normalisation: sum |psi|^2 = 1 / N)
-      slice_loop(*p, nb, w.G.p, false, w.full.p, fits ? 0 : 1, plane_out, w.planes.p, nullptr, fast, tm);
+      slice_loop(*p, nb, w.G.p, false, w.full.p, fits ? 0 : 1, plane_out.data(), w.planes.p, nullptr, fast, tm);
       const float2* result = w.planes.p;
       if (resample) {
         // WavesResample.swift:16-49: FFT, keep / pad the lowest frequencies, inverse FFT times (rows cols) / N:
@@ -1606,7 +1704,6 @@ int sk_hip_scan_intensities(const sk_potential* p, const sk_probe* probe, int np
         (void)hipEventDestroy(e[1]);
       }
     } evs{done};
-    std::vector<int> plane_out(p->nslices, -1);
     Timer tm;
     int pend_first = -1, pend_count = 0, pend_buf = 0;
     auto hand_over = [&]() {
@@ -1633,7 +1730,7 @@ int sk_hip_scan_intensities(const sk_potential* p, const sk_probe* probe, int np
       // the last slice's column pass stops after the propagator: G then holds FFT2(psi_exit) / N
       ColArgs fin = col_args(*p, w.G.p);
       fin.flags = COL_IN_G | COL_FWD | COL_MUL_P | COL_OUT;
-      slice_loop(*p, nb, w.G.p, false, w.full.p, fits ? 0 : 1, plane_out, nullptr, &fin, fast, tm);
+      slice_loop(*p, nb, w.G.p, false, w.full.p, fits ? 0 : 1, nullptr, nullptr, &fin, fast, tm);
       launch_compact_intensity(w.G.p, nx, ny, p->nc, p->keepinv_d.p, nb, double(N) * double(N), inten.p, s);
       SK_HIP(hipMemcpyAsync(pin.h[buf], inten.p, sizeof(double) * (size_t)nb * N, hipMemcpyDeviceToHost, s));
       SK_HIP(hipEventRecord(done[buf], s));
@@ -1670,23 +1767,9 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
     Context& c = require();
     hipStream_t s = c.stream;
     set_col_tile(c, opt);
-    if (!pots || npots < 1) fail(SK_ERR_INVALID, "invalid scan: no potential");
-    check_probe(probe);
-    if (npos < 1 || !positions) fail(SK_ERR_INVALID, "invalid scan: no positions");
-    if (ndets < 0 || (ndets > 0 && (!dets || !outputs))) fail(SK_ERR_INVALID, "invalid detectors");
+    check_scan(pots, npots, probe, npos, positions, dets, ndets, outputs);
     const sk_potential& p0 = *pots[0];
     const sk_grid g = p0.grid;
-    for (int k = 0; k < npots; ++k) {
-      if (!pots[k]) fail(SK_ERR_INVALID, "null potential");
-      if (!same_grid(pots[k]->grid, g)) fail(SK_ERR_INVALID, "the waves and the potential are on different grids");
-      if (pots[k]->energy != probe->energy)
-        fail(SK_ERR_INVALID, "the probe's energy differs from the potential's");
-      if (pots[k]->tilt[0] != probe->tilt[0] || pots[k]->tilt[1] != probe->tilt[1])
-        fail(SK_ERR_INVALID, "invalid tilt: the probe's tilt differs from the potential's");
-    }
-    for (int q = 0; q < npos; ++q)
-      if (!(std::isfinite(positions[2 * q]) && std::isfinite(positions[2 * q + 1])))
-        fail(SK_ERR_INVALID, "invalid position");
     const int nx = g.nx, ny = g.ny;
     const size_t N = (size_t)nx * ny;
     const double lambda = wavelength(probe->energy);
@@ -1909,31 +1992,11 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
     // `ss` and `off`: the stream and the first workspace position of a split scan's half (s and 0 otherwise).
     auto enqueue_batch = [&](const sk_potential& P, int b0, const int* pos_dev, int nb, Timer& t,
                              hipStream_t ss = nullptr, int off = 0) {
-      if (!ss) ss = s;
-      float2* G = w.G.p + (size_t)off * nx * p0.nc;
-      float2* rx = w.rx.p + (size_t)off * nx;
-      float2* ry = w.ry.p + (size_t)off * ny;
       DetPlan dph = dp;
       dph.partial = dp.partial + (size_t)off * nslots * bpw;
-      std::vector<int> plane_out(P.nslices, -1);
-      launch_ramps(nx, ny, g.extent_x / nx, g.extent_y / ny, w.positions.p, b0, pos_dev, nb, rx, ry, ss);
-      ColArgs init = col_args(P, fits ? G : w.full.p);
-      init.spec = w.spec.p;
-      init.rx = rx;
-      init.ry = ry;
-      init.flags = COL_GENERATE | COL_INV | COL_OUT;
-      if (!fits) {
-        init.nc = ny;
-        init.keep = identity_map(ny);
-      }
-      col_pass(init, nb, fast && fits, ss);
-      ColArgs fin = col_args(P, G);
-      fin.flags = COL_IN_G | COL_FWD | COL_MUL_P | COL_DETECT;
-      fin.pos0 = b0;
-      fin.pos_dev = pos_dev;
-      fin.det = dph;
-      slice_loop(P, nb, G, false, w.full.p, fits ? 0 : 1, plane_out, nullptr, &fin, fast, t, ss);
-      launch_reduce_partials(dph, nb, b0, pos_dev, w.acc.p, ss);
+      scan_batch(P, w.spec.p, w.positions.p, fits, fast, w.G.p + (size_t)off * nx * p0.nc, w.full.p,
+                 w.rx.p + (size_t)off * nx, w.ry.p + (size_t)off * ny, dph, b0, pos_dev, nb, w.acc.p, t,
+                 ss ? ss : s);
     };
     // the streams of a split scan's parts: the scan's own, then side streams created on first use
     auto part_streams = [&](int k) {
@@ -2112,6 +2175,137 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
     timing.batch = T;
     timing.group = B;
     c.last = timing;
+  });
+}
+
+// ---- the scan without allocation ------------------------------------------------------------------------------
+
+int sk_hip_scan_noalloc(const sk_potential* const* pots, int npots, const sk_probe* probe, int npos,
+                        const double* positions, const sk_detector* dets, int ndets, void* const* outputs,
+                        int outputs_on_device, const sk_options* opt) {
+  return guarded([&] {
+    Context& c = require();
+    hipStream_t s = c.stream;
+    if (!c.scratch_dev) fail(SK_ERR_DEVICE, "sk_hip_scan_noalloc: no device address for the scratch array: " +
+                                                c.scratch_why);
+    set_col_tile(c, opt);
+    if (c.precision != 0) fail(SK_ERR_INVALID, "sk_hip_scan_noalloc runs the float32 passes only (precision 0)");
+    if (opt && (opt->use_graph || opt->pipeline > 0 || opt->streams > 1 || opt->tile > 0))
+      fail(SK_ERR_INVALID, "sk_hip_scan_noalloc runs its batches on one stream, without graph replay, lanes or tiles");
+    check_scan(pots, npots, probe, npos, positions, dets, ndets, outputs);
+    if (ndets < 1) fail(SK_ERR_INVALID, "sk_hip_scan_noalloc: no detector");
+    const sk_potential& p0 = *pots[0];
+    const sk_grid g = p0.grid;
+    const int nx = g.nx, ny = g.ny;
+    if (nx > kNoallocMaxLength || ny > kNoallocMaxLength)
+      fail(SK_ERR_MEMORY, "sk_hip_scan_noalloc: grid lengths above " + std::to_string(kNoallocMaxLength) +
+                              " exceed its scratch; use sk_hip_scan");
+    const size_t N = (size_t)nx * ny;
+    const double lambda = wavelength(probe->energy);
+    // ---- detectors: annular sums (Detectors.swift:91-136), one slot each ----
+    DetPlan dp{};
+    dp.ny = ny;
+    if (ndets > kMaxSlots) fail(SK_ERR_INVALID, "too many detectors");
+    for (int d = 0; d < ndets; ++d) {
+      const sk_detector& D = dets[d];
+      if (D.kind != SK_DET_ANNULAR)
+        fail(SK_ERR_INVALID, "sk_hip_scan_noalloc reduces annular detectors only; use sk_hip_scan");
+      (void)detector_count(g, lambda, D, nullptr);  // validates
+      if (!outputs[d]) fail(SK_ERR_INVALID, "null detector output");
+      const double o = annular_outer(g, lambda, D);
+      dp.slot_kind[d] = 0;
+      dp.slot_arg[d] = d;
+      dp.slot_i2[d] = D.inner * D.inner;
+      dp.slot_o2[d] = o * o;
+    }
+    dp.nslots = ndets;
+    dp.npix = 0;
+    dp.scale = double(N) * double(N);
+    const int nslots = ndets;
+    const double rxs = 1 / (double(nx) * (g.extent_x / nx)), rys = 1 / (double(ny) * (g.extent_y / ny));
+    const double ax = rxs * lambda * 1e3, ay = rys * lambda * 1e3;
+    for (int ix = 0; ix < nx; ++ix) {
+      const double a = fftfreq_at(ix, nx, 1 / ax / double(nx));
+      noalloc_sq[ix] = a * a;
+    }
+    for (int iy = 0; iy < ny; ++iy) {
+      const double a = fftfreq_at(iy, ny, 1 / ay / double(ny));
+      noalloc_sq[nx + iy] = a * a;
+    }
+    // ---- the scratch: fixed part, then the batches' waves ----
+    const bool fast = !opt || opt->fast_path != 0;
+    const int gcols = col_generic_lines(slice_plan(nx, s));
+    int bpw = (p0.nc + gcols - 1) / gcols;
+    if (fast && fast_col_supported(nx, p0.nc, COL_IN_G | COL_FWD | COL_MUL_P | COL_DETECT))
+      bpw = fast_col_blocks_per_wave(nx, p0.nc, 0);
+    dp.blocks_per_wave = bpw;
+    Arena a{c.scratch_dev, kNoallocScratchBytes, 0};
+    const size_t fixed = N * sizeof(float2) + (2 + (size_t)nslots) * npos * sizeof(double) +
+                         (outputs_on_device ? 0 : (size_t)ndets * npos * sizeof(double)) +
+                         (size_t)(nx + ny) * sizeof(double) + (2 + kSpectrumBlocks) * sizeof(double) + 8 * 256;
+    const size_t per_pos = (size_t)nx * p0.nc * sizeof(float2) + (size_t)(nx + ny) * sizeof(float2) +
+                           (size_t)nslots * bpw * sizeof(double);
+    const size_t need = fixed + std::max(per_pos + 4 * 256, 2 * N * sizeof(double));
+    if (need > kNoallocScratchBytes)
+      fail(SK_ERR_MEMORY, "sk_hip_scan_noalloc: " + std::to_string(npos) + " positions on a " + std::to_string(nx) +
+                              " x " + std::to_string(ny) + " grid need at least " + std::to_string(need) +
+                              " bytes of scratch; it holds " + std::to_string(kNoallocScratchBytes) +
+                              " (use sk_hip_scan)");
+    float2* spec = a.take<float2>(N);
+    double* pos_d = a.take<double>(2 * (size_t)npos);
+    double* acc = a.take<double>((size_t)npos * nslots);
+    double* stage = outputs_on_device ? nullptr : a.take<double>((size_t)ndets * npos);
+    double* sq = a.take<double>((size_t)nx + ny);
+    double* norm = a.take<double>(1);
+    int* flag = reinterpret_cast<int*>(a.take<double>(1));
+    double* norm_partial = a.take<double>(kSpectrumBlocks);
+    // the rest holds T positions' compact waves, ramps and partial sums (and, before the first batch, the spectrum's
+    // float64 temporaries, which are dead once the spectrum is built)
+    const size_t rest = a.left();
+    const int tmax = (int)std::min<size_t>((size_t)npos, (rest - 4 * 256) / per_pos);
+    const int T = std::max(1, std::min(tmax, default_batch(p0, npos, opt)));
+    unsigned char* region = c.scratch_dev + a.aligned(a.used);
+    double* tmp_re = reinterpret_cast<double*>(region);
+    double* tmp_im = tmp_re + N;
+    float2* G = a.take<float2>((size_t)T * nx * p0.nc);
+    float2* rx = a.take<float2>((size_t)T * nx);
+    float2* ry = a.take<float2>((size_t)T * ny);
+    dp.partial = a.take<double>((size_t)T * nslots * bpw);
+    // ---- inputs ----
+    SK_HIP(hipMemcpyAsync(pos_d, positions, 2 * sizeof(double) * (size_t)npos, hipMemcpyHostToDevice, s));
+    SK_HIP(hipMemcpyAsync(sq, noalloc_sq, sizeof(double) * ((size_t)nx + ny), hipMemcpyHostToDevice, s));
+    dp.sqx = sq;
+    dp.sqy = sq + nx;
+    launch_fill_zero(acc, sizeof(double) * (size_t)npos * nslots, s);
+    launch_fill_zero(flag, sizeof(int), s);
+    spectrum_into(g, *probe, 1.0 / double(N), tmp_re, tmp_im, norm, norm_partial, spec, s);
+    launch_pruned_support(tmp_re, tmp_im, p0.keepinv_d.p, nx, ny, flag, s);
+    int outside = 0;
+    SK_HIP(hipMemcpyAsync(&outside, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    SK_HIP(hipStreamSynchronize(s));
+    if (outside)
+      fail(SK_ERR_INVALID, "sk_hip_scan_noalloc: the probe reaches past the bandlimit (it needs full-width waves); "
+                           "use sk_hip_scan");
+    // ---- the batches: potentials outer, positions inner, each batch through every slice ----
+    Timer off;
+    for (int k = 0; k < npots; ++k)
+      for (int b0 = 0; b0 < npos; b0 += T)
+        scan_batch(*pots[k], spec, pos_d, true, fast, G, nullptr, rx, ry, dp, b0, nullptr, std::min(T, npos - b0),
+                   acc, off, s);
+    // ---- outputs: averaged over the potentials (STEM.swift:146-150) ----
+    const double inv = 1.0 / double(npots);
+    for (int d = 0; d < ndets; ++d) {
+      const size_t esz = dets[d].out_f32 ? sizeof(float) : sizeof(double);
+      void* dst = outputs_on_device ? outputs[d] : static_cast<void*>(stage + (size_t)d * npos);
+      launch_gather_scalar(acc, nslots, d, npos, inv, dst, 1, dets[d].out_f32, s);
+      if (!outputs_on_device)
+        SK_HIP(hipMemcpyAsync(outputs[d], dst, esz * (size_t)npos, hipMemcpyDeviceToHost, s));
+    }
+    SK_HIP(hipStreamSynchronize(s));
+    sk_timing t{};
+    t.batch = T;
+    t.group = T;
+    c.last = t;
   });
 }
 

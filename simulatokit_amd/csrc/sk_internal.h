@@ -331,6 +331,11 @@ void launch_finite_slices(int nx, int ny, double ex, double ey, int pi_mode, con
 void launch_finalise_outputs(const double* acc, size_t n, double inv, void* out, int f32, hipStream_t s);
 void launch_gather_scalar(const double* acc, int nslots, int slot, int npos, double inv, void* out, int stride,
                           int f32, hipStream_t s);
+// clears `bytes` (a multiple of 4) of device memory with a kernel
+void launch_fill_zero(void* p, size_t bytes, hipStream_t s);
+// *flag = 1 when the float64 spectrum [nx][ny] is non-zero on a column with keepinv[iy] < 0 (*flag cleared first)
+void launch_pruned_support(const double* re, const double* im, const int* keepinv, int nx, int ny, int* flag,
+                           hipStream_t s);
 
 // ---- device complex helpers ------------------------------------------------------------------------------------
 __host__ __device__ inline float2 cmul(float2 a, float2 b) {

@@ -548,4 +548,32 @@ void launch_gather_scalar(const double* acc, int nslots, int slot, int npos, dou
   SK_HIP(hipGetLastError());
 }
 
+// words[0 .. n) = 0: clears device memory with a kernel (no memset node, so a captured stream replays it correctly)
+__global__ void fill_zero_kernel(uint32_t* words, size_t n) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) words[i] = 0u;
+}
+void launch_fill_zero(void* p, size_t bytes, hipStream_t s) {
+  const size_t n = bytes / sizeof(uint32_t);
+  if (bytes % sizeof(uint32_t)) fail(SK_ERR_INVALID, "fill_zero: size is not a multiple of 4 bytes");
+  if (!n) return;
+  hipLaunchKernelGGL(fill_zero_kernel, grid_for(n), dim3(kThreads), 0, s, static_cast<uint32_t*>(p), n);
+  SK_HIP(hipGetLastError());
+}
+
+// *flag = 1 when the float64 spectrum (re, im) [nx][ny] is non-zero on a column the bandlimit prunes
+// (keepinv[iy] < 0); the caller clears *flag first. Every writer stores the same value.
+__global__ void pruned_support_kernel(const double* re, const double* im, const int* keepinv, int nx, int ny,
+                                      int* flag) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= (size_t)nx * ny) return;
+  if (keepinv[i % ny] < 0 && (re[i] != 0.0 || im[i] != 0.0)) *flag = 1;
+}
+void launch_pruned_support(const double* re, const double* im, const int* keepinv, int nx, int ny, int* flag,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(pruned_support_kernel, grid_for((size_t)nx * ny), dim3(kThreads), 0, s, re, im, keepinv, nx,
+                     ny, flag);
+  SK_HIP(hipGetLastError());
+}
+
 }  // namespace sk
