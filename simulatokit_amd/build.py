@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "_lib")
 LIB_PATH = os.path.join(LIB_DIR, "libsimulato_hip.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
-SOURCES = ["sk_passes.hip", "sk_fast.hip", "sk_prepare.hip", "sk_api.hip", "sk_tune.hip", "sk_conv.hip"]
+SOURCES = ["sk_passes.hip", "sk_fast.hip", "sk_reg.hip", "sk_prepare.hip", "sk_api.hip", "sk_tune.hip", "sk_conv.hip"]
 ARCH = os.environ.get("SK_OFFLOAD_ARCH", "gfx950")
 
 # Packed FP32 instructions (v_pk_*_f32) are kept out of every device code object: the compiler would otherwise form

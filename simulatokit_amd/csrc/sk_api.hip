@@ -379,6 +379,10 @@ void row_pass(const RowArgs& a, int batch, bool fast, hipStream_t s) {
     launch_row_fast(a, batch, s);
     return;
   }
+  if (fast && reg_row_supported(a)) {
+    launch_row_reg(a, batch, s);
+    return;
+  }
   if (a.ev_start) SK_HIP(hipEventRecord(a.ev_start, s));
   launch_row_generic(a, batch, s);
   if (a.ev_stop) SK_HIP(hipEventRecord(a.ev_stop, s));
@@ -387,6 +391,10 @@ void row_pass(const RowArgs& a, int batch, bool fast, hipStream_t s) {
 void col_pass(const ColArgs& a, int batch, bool fast, hipStream_t s) {
   if (fast && fast_col_supported(a.nx, a.nc, a.flags)) {
     launch_col_fast(a, batch, s);
+    return;
+  }
+  if (fast && reg_col_supported(a)) {
+    launch_col_reg(a, batch, s);
     return;
   }
   if (a.ev_start) SK_HIP(hipEventRecord(a.ev_start, s));

@@ -265,6 +265,12 @@ constexpr size_t kCachedWaveBytes = size_t(240) << 20;
 // columns per block the fast column pass uses for a request of 0, 8 or 16 and a batch of `wave_bytes` compact waves
 int fast_col_tile(int nx, int requested, double wave_bytes);
 int fast_col_blocks_per_wave(int nx, int nc, int pipeline);  // blocks of the detecting column pass per wave
+// register-resident passes for lines of 384 and 1024 points and Bluestein's lengths 257..512 on 1024 points
+// (sk_reg.hip): the row modes of the fast passes, and the middle slices' column pass
+bool reg_row_supported(const RowArgs& a);
+bool reg_col_supported(const ColArgs& a);
+void launch_row_reg(const RowArgs& a, int batch, hipStream_t s);
+void launch_col_reg(const ColArgs& a, int batch, hipStream_t s);
 // lane pipeline launches (sk_fast.hip): the row pass and the column pass of one step, the partial-sum reduction of
 // the lanes that finished an item, and the step counter's increment
 bool pipeline_supported(int nx, int ny, int nc, int lane_waves);

@@ -1,0 +1,429 @@
+// sk_reg.hip — register-resident row and column passes of the slice loop for lines of 384 and 1024 points, and for
+// the grid lengths 257..512 that Bluestein's algorithm transforms on 1024 points (e.g. 509).
+//
+// A line of M points is held by T = M / V lanes, V values per lane: lane i holds elements i + T r, r < V (384: V = 24,
+// T = 16, four lines per wavefront; 1024: V = 16, T = 64, one line per wavefront). The transform is a three-stage
+// Stockham FFT (384 = 8 * 8 * 6, 1024 = 16 * 8 * 8): the first stage works from the registers, the last ends in the
+// registers in natural order, and the two exchanges between them go through the line's own LDS, which only its
+// wavefront touches (no block barrier). The line's LDS slots are XOR-swizzled, e -> e ^ (((e >> 3) ^ (e >> 7)) & 31),
+// which a bank model of every exchange (64 banks of 4 bytes, groups of 32 lanes) finds free of conflicts at both
+// lengths (with a line stride of 400 slots at 384). Stage twiddles come from per-block LDS tables of the line's
+// float64-derived roots.
+//
+// Bluestein (1970) for a logical length n <= 512 on M = 1024: X_k = c_k sum_j (x_j c_j) conj(c_{k - j}), c_k =
+// e^{-i pi k^2 / n} (conjugated for the inverse), the convolution done in the registers as the forward 1024-point
+// transform of x c (zero past n), the product with the kernel's transform, the inverse 1024-point transform, and the
+// closing chirp product.
+//
+//   row pass:    blocks of 2 wavefronts; each line is one row x of one position, IFFT_y -> * t(x, y) -> FFT_y (and
+//                the other row modes of the fast passes), the block's rows for 8 positions in turn with t in registers
+//   column pass: the middle slices (FFT_x -> * P -> IFFT_x); a block moves TC compact columns of one position between
+//                HBM and LDS in row segments of TC values, each column's lanes transform it inside the block's LDS,
+//                P in registers for the block's 8 positions
+// The probe-generating first and the detecting last column pass of a scan run the generic passes.
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
+
+#include "sk_dft.h"
+#include "sk_internal.h"
+
+namespace sk {
+namespace {
+
+template <int M>
+struct RegPlan;
+template <>
+struct RegPlan<1024> {
+  static constexpr int V = 16, R1 = 16, R2 = 8, R3 = 8;
+  static constexpr int LS = 1024;  // line stride (slots) in the row pass
+};
+template <>
+struct RegPlan<384> {
+  static constexpr int V = 24, R1 = 8, R2 = 8, R3 = 6;
+  static constexpr int LS = 400;   // 384 + 16: the four lines of a wavefront on different bank halves
+};
+
+__device__ __forceinline__ int rslot(int e) { return e ^ (((e >> 3) ^ (e >> 7)) & 31); }
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int R>
+__device__ __forceinline__ void dft_r(float2 (&u)[R], int sign) {
+  if constexpr (R == 6) dft6(u, sign);
+  else dft_pow2<R>(u, sign);
+}
+
+// The twiddles of stages 2 and 3 in LDS: tab[(j - 1) L2 + k] = w_{L2 R2}^{j k}, then tab[S2 + (j - 1) L3 + k] =
+// w_{L3 R3}^{j k}, from roots[m] = e^{-2 pi i m / M}.
+template <int M>
+struct RegTw {
+  using P = RegPlan<M>;
+  static constexpr int L2 = P::R1, L3 = P::R1 * P::R2;
+  static constexpr int S2 = (P::R2 - 1) * L2, kSize = S2 + (P::R3 - 1) * L3;
+  __device__ static void build(float2* tab, const float2* __restrict__ roots, int tid, int nthreads) {
+    for (int e = tid; e < kSize; e += nthreads) {
+      if (e < S2) {
+        const int j = e / L2 + 1, k = e % L2;
+        tab[e] = roots[j * k * (M / (L2 * P::R2))];
+      } else {
+        const int f = e - S2, j = f / L3 + 1, k = f % L3;
+        tab[e] = roots[j * k * (M / (L3 * P::R3))];
+      }
+    }
+  }
+};
+
+// One Stockham stage of radix R at sub-length L: butterfly bi = i + T q (q < V / R) reads x[bi + (M / R) j],
+// multiplies by w_{L R}^{j k} (k = bi mod L; tw = the stage's table), takes the DFT and writes y[(bi - k) R + k + L j].
+// FROM_REG: the inputs are the registers (element i + T r in v[r]); TO_REG: the outputs go to the registers.
+template <int M, int R, int L, int SIGN, bool FROM_REG, bool TO_REG, class Slot>
+__device__ __forceinline__ void vstage(float2 (&v)[RegPlan<M>::V], float2* __restrict__ buf, int i,
+                                      const float2* __restrict__ tw, Slot slot) {
+  constexpr int V = RegPlan<M>::V, T = M / V, NB = V / R, STRIDE = M / R;
+  static_assert(V % R == 0, "the lane's butterflies");
+  float2 u[NB][R];
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int bi = i + T * q;
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      if constexpr (FROM_REG) u[q][j] = v[q + NB * j];
+      else u[q][j] = buf[slot(bi + STRIDE * j)];
+    }
+  }
+  if constexpr (!FROM_REG) wave_sync();  // every lane has read the line before any lane overwrites it
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int bi = i + T * q;
+    const int k = (L == 1) ? 0 : (bi % L);
+    if constexpr (L > 1) {
+#pragma unroll
+      for (int j = 1; j < R; ++j) {
+        const float2 w = tw[(j - 1) * L + k];
+        u[q][j] = SIGN < 0 ? cmul(u[q][j], w) : cmulc(u[q][j], w);
+      }
+    }
+    dft_r<R>(u[q], SIGN);
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      if constexpr (TO_REG) v[q + NB * j] = u[q][j];
+      else buf[slot((bi - k) * R + k + j * L)] = u[q][j];
+    }
+  }
+  if constexpr (!TO_REG) wave_sync();
+}
+
+// Unscaled transform of the line in the registers (SIGN -1 forward, +1 inverse).
+template <int M, int SIGN, class Slot>
+__device__ __forceinline__ void vfft(float2 (&v)[RegPlan<M>::V], float2* buf, int i, const float2* tab, Slot slot) {
+  using P = RegPlan<M>;
+  vstage<M, P::R1, 1, SIGN, true, false>(v, buf, i, nullptr, slot);
+  vstage<M, P::R2, P::R1, SIGN, false, false>(v, buf, i, tab, slot);
+  vstage<M, P::R3, P::R1 * P::R2, SIGN, false, true>(v, buf, i, tab + RegTw<M>::S2, slot);
+}
+
+// The line transform of logical length n: the M-point transform itself, or (BLUE, M = 1024) Bluestein's for n <= 512
+// with the chirp cs[n] and the kernel transform ks[1024] (both in LDS; conjugated for the inverse). Values at
+// elements e >= n are ignored on entry and undefined on return.
+template <int M, int SIGN, bool BLUE, class Slot>
+__device__ __forceinline__ void line_fft(float2 (&v)[RegPlan<M>::V], float2* buf, int i, const float2* tab,
+                                         const float2* cs, const float2* ks, int n, Slot slot) {
+  if constexpr (!BLUE) {
+    vfft<M, SIGN>(v, buf, i, tab, slot);
+  } else {
+    constexpr int V = RegPlan<M>::V, T = M / V;
+#pragma unroll
+    for (int r = 0; r < V; ++r) {
+      const int e = i + T * r;
+      if (e < n) {
+        const float2 c = cs[e];
+        v[r] = SIGN < 0 ? cmul(v[r], c) : cmulc(v[r], c);
+      } else {
+        v[r] = make_float2(0.f, 0.f);
+      }
+    }
+    vfft<M, -1>(v, buf, i, tab, slot);
+#pragma unroll
+    for (int r = 0; r < V; ++r) {
+      const float2 kh = ks[i + T * r];
+      v[r] = SIGN < 0 ? cmul(v[r], kh) : cmulc(v[r], kh);
+    }
+    vfft<M, +1>(v, buf, i, tab, slot);
+#pragma unroll
+    for (int r = 0; r < V; ++r) {
+      const int e = i + T * r;
+      if (e < n) {
+        const float2 c = cs[e];
+        v[r] = SIGN < 0 ? cmul(v[r], c) : cmulc(v[r], c);
+      }
+    }
+  }
+}
+
+constexpr int kRegPB = 8;      // positions per block (t or P loaded once per 8 positions)
+constexpr int kRowWaves = 2;   // wavefronts per row-pass block
+
+__device__ __forceinline__ int compact_col(const RowArgs& a, int iy) {
+  return iy <= a.kept_hi ? iy : (iy >= a.kept_lo ? iy - a.kept_shift : -1);
+}
+
+// Row pass: LINES = (64 / T) x kRowWaves rows x per block, positions [b0, b0 + kRegPB). Rows past nx (a grid length
+// that is not a multiple of LINES) compute on the last row and store nothing.
+template <int M, int FLAGS, bool BLUE>
+__global__ void __launch_bounds__(64 * kRowWaves) row_reg_kernel(RowArgs a, const float2* __restrict__ roots,
+                                                                 const float2* __restrict__ chirp,
+                                                                 const float2* __restrict__ khat, int batch) {
+  using P = RegPlan<M>;
+  constexpr int V = P::V, T = M / V, LINES = (64 / T) * kRowWaves, LS = P::LS, NT = 64 * kRowWaves;
+  __shared__ float2 lds[LINES * LS];
+  __shared__ float2 tab[RegTw<M>::kSize];
+  __shared__ float2 cs[BLUE ? 512 : 1];
+  __shared__ float2 ks[BLUE ? 1024 : 1];
+  const int tid = threadIdx.x, line = tid / T, i = tid - line * T;
+  const int n = BLUE ? a.ny : M;
+  RegTw<M>::build(tab, roots, tid, NT);
+  if constexpr (BLUE) {
+    for (int e = tid; e < n; e += NT) cs[e] = chirp[e];
+    for (int e = tid; e < 1024; e += NT) ks[e] = khat[e];
+  }
+  __syncthreads();
+  const int x = blockIdx.x * LINES + line;
+  const bool live = x < a.nx;
+  const long long xs = live ? x : a.nx - 1;
+  const int b0 = blockIdx.y * kRegPB;
+  float2* buf = lds + line * LS;
+  auto slot = [](int e) { return rslot(e); };
+  const long long gw = (long long)a.nx * a.nc;
+  // the compact column of element e of the lane's line (or -1: pruned, or past n)
+  auto cidx = [&](int r) {
+    const int e = i + T * r;
+    return e < n ? compact_col(a, e) : -1;
+  };
+  float2 t[V];
+  if constexpr ((FLAGS & ROW_MUL_T) != 0) {
+    const float2* tr = a.t + xs * n;
+#pragma unroll
+    for (int r = 0; r < V; ++r) {
+      const int e = i + T * r;
+      t[r] = e < n ? tr[e] : make_float2(0.f, 0.f);
+    }
+  }
+  for (int p = 0; p < kRegPB; ++p) {
+    const long long b = b0 + p;
+    if (b >= batch) break;  // uniform over the block
+    float2 v[V];
+    if constexpr ((FLAGS & ROW_IN_COMPACT) != 0) {
+      const float2* g = a.G + b * gw + xs * a.nc;
+#pragma unroll
+      for (int r = 0; r < V; ++r) {
+        const int c = cidx(r);
+        v[r] = c >= 0 ? g[c] : make_float2(0.f, 0.f);
+      }
+    } else {
+      const float2* g = a.in_full + b * a.in_full_stride + xs * n;
+#pragma unroll
+      for (int r = 0; r < V; ++r) {
+        const int e = i + T * r;
+        v[r] = e < n ? g[e] : make_float2(0.f, 0.f);
+      }
+    }
+    if constexpr ((FLAGS & ROW_INV) != 0) line_fft<M, +1, BLUE>(v, buf, i, tab, cs, ks, n, slot);
+    if constexpr ((FLAGS & ROW_WRITE_PLANE) != 0) {
+      if (live) {
+        float2* pl = a.plane + b * a.plane_stride + xs * n;
+#pragma unroll
+        for (int r = 0; r < V; ++r) {
+          const int e = i + T * r;
+          if (e < n) pl[e] = cscale(v[r], a.out_scale);
+        }
+      }
+    }
+    if constexpr ((FLAGS & ROW_MUL_T) != 0) {
+#pragma unroll
+      for (int r = 0; r < V; ++r) v[r] = cmul(v[r], t[r]);
+    }
+    if constexpr ((FLAGS & ROW_FWD) != 0) line_fft<M, -1, BLUE>(v, buf, i, tab, cs, ks, n, slot);
+    if constexpr ((FLAGS & ROW_OUT_COMPACT) != 0) {
+      if (live) {
+        float2* g = a.G + b * gw + xs * a.nc;
+#pragma unroll
+        for (int r = 0; r < V; ++r) {
+          const int c = cidx(r);
+          if (c >= 0) g[c] = v[r];
+        }
+      }
+    }
+  }
+}
+
+// The column pass's tile: TC columns of one position, column c at slots [c CS, c CS + CL), element e at rslot(e);
+// CL = M (Bluestein: the 1024-point convolution buffer, which holds the column's n values before the transform).
+template <int M, bool BLUE>
+struct ColTile;
+template <>
+struct ColTile<1024, false> {
+  static constexpr int TC = 8, CS = 1028;  // 64-byte row segments; CS = 4 mod 32: conflict-free block phases
+};
+template <>
+struct ColTile<384, false> {
+  static constexpr int TC = 16, CS = 400;  // 128-byte row segments; conflict-free exchanges
+};
+template <>
+struct ColTile<1024, true> {
+  static constexpr int TC = 4, CS = 1032;  // 32-byte row segments (the arithmetic of Bluestein's dominates)
+};
+
+// Column pass of the middle slices (COL_IN_G | COL_FWD | COL_MUL_P | COL_INV | COL_OUT): block = TC compact columns
+// [TC blockIdx.x, + TC) of positions [b0, b0 + kRegPB); TC x T threads, the T lanes of column c = tid / T.
+template <int M, bool BLUE>
+__global__ void __launch_bounds__((ColTile<M, BLUE>::TC * (M / RegPlan<M>::V))) col_reg_kernel(
+    ColArgs a, const float2* __restrict__ roots, const float2* __restrict__ chirp, const float2* __restrict__ khat,
+    int batch) {
+  using P = RegPlan<M>;
+  constexpr int V = P::V, T = M / V, TC = ColTile<M, BLUE>::TC, CS = ColTile<M, BLUE>::CS, NT = TC * T;
+  constexpr int RSTEP = NT / TC;  // rows per block-phase round
+  __shared__ float2 tile[TC * CS];
+  __shared__ float2 tab[RegTw<M>::kSize];
+  __shared__ float2 cs[BLUE ? 512 : 1];
+  __shared__ float2 ks[BLUE ? 1024 : 1];
+  const int tid = threadIdx.x;
+  const int n = BLUE ? a.nx : M;
+  RegTw<M>::build(tab, roots, tid, NT);
+  if constexpr (BLUE) {
+    for (int e = tid; e < n; e += NT) cs[e] = chirp[e];
+    for (int e = tid; e < 1024; e += NT) ks[e] = khat[e];
+  }
+  const int c0 = blockIdx.x * TC;
+  // block phases: thread -> column cb of the tile, rows xi + RSTEP r
+  const int cb = tid % TC, xi = tid / TC;
+  // column lanes: column wc, lane i
+  const int wc = tid / T, i = tid - wc * T;
+  float2* col = tile + wc * CS;
+  auto slot = [](int e) { return rslot(e); };
+  const long long gw = (long long)a.nx * a.nc;
+  // P of the lane's elements of column c0 + wc, for every position of the block
+  float2 pv[V];
+  {
+    const float2* Pc = a.P + c0 + wc;
+#pragma unroll
+    for (int r = 0; r < V; ++r) {
+      const int e = i + T * r;
+      pv[r] = e < n ? Pc[(long long)e * a.nc] : make_float2(0.f, 0.f);
+    }
+  }
+  const bool kept = a.keep[c0 + cb] >= 0;  // padding columns are never read nor written
+  const int b0 = blockIdx.y * kRegPB;
+  for (int p = 0; p < kRegPB; ++p) {
+    const long long b = b0 + p;
+    if (b >= batch) break;  // uniform over the block
+    float2* g = a.G + b * gw + c0 + cb;
+    __syncthreads();  // the tile's previous position is stored (and, first, the tables are built)
+    for (int x = xi; x < n; x += RSTEP)
+      tile[cb * CS + rslot(x)] = kept ? g[(long long)x * a.nc] : make_float2(0.f, 0.f);
+    __syncthreads();
+    float2 v[V];
+#pragma unroll
+    for (int r = 0; r < V; ++r) {
+      const int e = i + T * r;
+      v[r] = e < n ? col[rslot(e)] : make_float2(0.f, 0.f);
+    }
+    wave_sync();  // every lane has read its column before the exchanges overwrite it
+    line_fft<M, -1, BLUE>(v, col, i, tab, cs, ks, n, slot);
+#pragma unroll
+    for (int r = 0; r < V; ++r) v[r] = cmul(v[r], pv[r]);
+    line_fft<M, +1, BLUE>(v, col, i, tab, cs, ks, n, slot);
+#pragma unroll
+    for (int r = 0; r < V; ++r) {
+      const int e = i + T * r;
+      if (e < n) col[rslot(e)] = v[r];
+    }
+    __syncthreads();
+    if (kept)
+      for (int x = xi; x < n; x += RSTEP) g[(long long)x * a.nc] = tile[cb * CS + rslot(x)];
+  }
+}
+
+template <class K, class... A>
+void launch_timed(K kernel, dim3 grid, dim3 block, hipStream_t s, hipEvent_t e0, hipEvent_t e1, A... args) {
+  if (e0) hipExtLaunchKernelGGL(kernel, grid, block, 0, s, e0, e1, 0, args...);
+  else hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
+}
+
+constexpr int ROW_MID = ROW_IN_COMPACT | ROW_INV | ROW_MUL_T | ROW_FWD | ROW_OUT_COMPACT;
+constexpr int ROW_MID_PLANE = ROW_MID | ROW_WRITE_PLANE;
+constexpr int ROW_ENTRY = ROW_IN_FULL | ROW_MUL_T | ROW_FWD | ROW_OUT_COMPACT;
+constexpr int ROW_FROM_FULL = ROW_IN_FULL | ROW_INV | ROW_MUL_T | ROW_FWD | ROW_OUT_COMPACT;
+constexpr int ROW_EXIT = ROW_IN_COMPACT | ROW_INV | ROW_WRITE_PLANE;
+constexpr int COL_MID = COL_IN_G | COL_FWD | COL_MUL_P | COL_INV | COL_OUT;
+
+// the transform length the register passes use for a grid length n: n itself (384, 1024), 1024 for Bluestein's
+// lengths 257..512 that are not 256 or 384 and have no fast pass, 0 for none
+int reg_length(const LineFFT& f) {
+  if (f.n == 384 || f.n == 1024) return f.blue ? 0 : f.n;
+  if (f.blue && f.m == 1024 && f.n >= 257 && f.n <= 512) return 1024;
+  return 0;
+}
+
+template <int M, int F, bool BLUE>
+void row_go(const RowArgs& a, int batch, hipStream_t s) {
+  constexpr int LINES = (64 / (M / RegPlan<M>::V)) * kRowWaves;
+  dim3 grid((a.nx + LINES - 1) / LINES, (batch + kRegPB - 1) / kRegPB);
+  launch_timed(row_reg_kernel<M, F, BLUE>, grid, dim3(64 * kRowWaves), s, a.ev_start, a.ev_stop, a, a.fy.roots,
+               a.fy.chirp, a.fy.khat, batch);
+}
+
+template <int M, bool BLUE>
+void row_launch(const RowArgs& a, int batch, hipStream_t s) {
+  switch (a.flags) {
+    case ROW_MID: row_go<M, ROW_MID, BLUE>(a, batch, s); break;
+    case ROW_MID_PLANE: row_go<M, ROW_MID_PLANE, BLUE>(a, batch, s); break;
+    case ROW_ENTRY: row_go<M, ROW_ENTRY, BLUE>(a, batch, s); break;
+    case ROW_FROM_FULL: row_go<M, ROW_FROM_FULL, BLUE>(a, batch, s); break;
+    default: row_go<M, ROW_EXIT, BLUE>(a, batch, s); break;
+  }
+  SK_HIP(hipGetLastError());
+}
+
+template <int M, bool BLUE>
+void col_launch(const ColArgs& a, int batch, hipStream_t s) {
+  constexpr int TC = ColTile<M, BLUE>::TC, NT = TC * (M / RegPlan<M>::V);
+  dim3 grid(a.nc / TC, (batch + kRegPB - 1) / kRegPB);
+  launch_timed(col_reg_kernel<M, BLUE>, grid, dim3(NT), s, a.ev_start, a.ev_stop, a, a.fx.roots, a.fx.chirp,
+               a.fx.khat, batch);
+  SK_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+bool reg_row_supported(const RowArgs& a) {
+  if (a.fy.use_mfma || reg_length(a.fy) == 0 || a.fy.n != a.ny) return false;
+  const int f = a.flags;
+  return f == ROW_MID || f == ROW_MID_PLANE || f == ROW_ENTRY || f == ROW_FROM_FULL || f == ROW_EXIT;
+}
+
+bool reg_col_supported(const ColArgs& a) {
+  if (a.fx.use_mfma || a.flags != COL_MID || a.fx.n != a.nx) return false;
+  const int m = reg_length(a.fx);
+  if (m == 0) return false;
+  const int tc = m == 384 ? ColTile<384, false>::TC : (a.fx.blue ? ColTile<1024, true>::TC : ColTile<1024, false>::TC);
+  return a.nc % tc == 0;
+}
+
+void launch_row_reg(const RowArgs& a, int batch, hipStream_t s) {
+  if (a.fy.blue) row_launch<1024, true>(a, batch, s);
+  else if (a.fy.n == 384) row_launch<384, false>(a, batch, s);
+  else row_launch<1024, false>(a, batch, s);
+}
+
+void launch_col_reg(const ColArgs& a, int batch, hipStream_t s) {
+  if (a.fx.blue) col_launch<1024, true>(a, batch, s);
+  else if (a.fx.n == 384) col_launch<384, false>(a, batch, s);
+  else col_launch<1024, false>(a, batch, s);
+}
+
+}  // namespace sk
