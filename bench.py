@@ -15,7 +15,8 @@ of two streams) whose waves stay in the 256 MiB Infinity Cache between passes.
 pass, the full complex64 wave read and written), the compact bytes it stores (the columns inside the bandlimit), and
 the L2-to-fabric bytes of the committed rocprofv3 counter summary; `step` checks the scan's algorithmic bytes per
 second against the peak. `llc_resident` gives the headline plan's kernels, whose algorithmic rate is an effective
-bandwidth above what HBM could deliver, not an HBM fraction. Kernel times come from HIP events attached to the
+bandwidth above what HBM could deliver, not an HBM fraction; `compare_no_graph` times the headline plan with its
+kernels launched one by one. Kernel times come from HIP events attached to the
 launches of one more scan of the same tiles run on one stream. The CPU baseline runs the float64 oracle on a bounded
 sample of positions on the host's cores (one warm-up, five runs, median and min-max), and the same sample gives the
 HIP path's error against float64.
@@ -198,6 +199,9 @@ def main():
         el = timed(o)
         rate = npos * S * args.steps / el
         result_plans["headline"] = (o, rate, el / args.steps * 1e3, kernel_times(o))
+        if graph and args.only == "all":
+            el = timed(plan(args.tile, use_graph=False))
+            result_plans["no_graph"] = (None, npos * S * args.steps / el, el / args.steps * 1e3)
     haadf = out.cpu().numpy()
     copy_gbs = copy_bandwidth(torch)
     if "headline" in result_plans:
@@ -246,8 +250,11 @@ def main():
             "kernel": llc["kernel"], "kernel_ms": llc["kernel_ms"], "row_ms": llc["row_ms"], "col_ms": llc["col_ms"],
             "effective_GBs": llc["achieved"], "effective_x_hbm_peak": llc["frac"],
             "step_effective_GBs": llc["step"]["GBs"], "stored": llc["stored"], "measured": llc["measured"]}
-        if "roofline" not in result:
-            result["roofline"] = llc
+        if "no_graph" in result_plans:
+            _, ng_rate, ng_ms = result_plans["no_graph"]
+            result["compare_no_graph"] = {
+                "what": "the headline plan with the slice loop's kernels launched one by one instead of replayed "
+                        "from a hipGraph", "value": ng_rate, "ms_per_step": ng_ms}
 
     if not args.no_cpu_baseline:
         result["cpu_baseline"], result["rel_err_vs_fp64"] = cpu_baseline(args, w, st, pot, positions, haadf, S)

@@ -12,9 +12,10 @@ Nyquist limit (a HAADF image is band-limited by the probe at 2 alpha / lambda, 4
 20 mrad and 200 keV), or periodic linear interpolation. Torch only holds device memory.
 
 Input: the cell's summed projected potential on a 128 x 128 grid and a thickness channel (the specified input), or,
-as a labelled variant (`depth_slabs` = K > 0), the projected potential summed over each of K equal depth ranges of
-[0, t_max) and the thickness channel: the probe spreads with depth, so the same projected atom scatters differently
-at the entrance and the exit of the cell, which the summed projection cannot tell apart.
+as labelled variants, the projected potential summed over each of K equal depth ranges of [0, t_max)
+(`depth_slabs` = K > 0), and/or split by element (`by_element`), each beside the thickness channel: the probe spreads
+with depth, so the same projected atom scatters differently at the entrance and the exit of the cell, and atoms that
+overlap in projection scatter by their atomic numbers, neither of which the summed projection can tell apart.
 """
 from __future__ import annotations
 
@@ -90,25 +91,34 @@ class SurrogateWorkload:
         prepared.release()
         return out.reshape(self.scan_n, self.scan_n) if positions is None else out
 
-    def features(self, st: AtomicStructure, depth_slabs: int = 0) -> np.ndarray:
+    def features(self, st: AtomicStructure, depth_slabs: int = 0, by_element: bool = False) -> np.ndarray:
         """The network's input: the summed projected potential on an input_n^2 grid and a thickness channel; with
         depth_slabs = K > 0 the projected potential summed over each of K equal depth ranges of [0, t_max) (a slice
-        counts in the range that holds its centre) instead of the one sum."""
-        pot = Potential(st, self.input_n, self.input_n, slice_thickness=self.slice_thickness)
-        prepared = pot.prepare(ElectronBeam(self.energy))
-        V = prepared.values()
-        prepared.release()
-        thick = np.full(V.shape[1:], st.box[2])
-        if depth_slabs <= 0:
-            return np.stack([V.sum(axis=0), thick]).astype(np.float32)
-        edges = np.cumsum(pot.slice_thicknesses)
-        centres = edges - 0.5 * np.asarray(pot.slice_thicknesses)
-        slab = np.minimum((centres / (self.t_max / depth_slabs)).astype(int), depth_slabs - 1)
-        out = np.zeros((depth_slabs + 1,) + V.shape[1:])
-        for j, k in enumerate(slab):
-            out[k] += V[j]
-        out[depth_slabs] = thick
-        return out.astype(np.float32)
+        counts in the range that holds its centre) instead of the one sum; with by_element, those channels once per
+        element of `elements` (the potential of that element's atoms alone)."""
+        groups = [st.numbers == z for z in self.elements] if by_element else [np.ones(len(st.numbers), bool)]
+        chans = []
+        for sel in groups:
+            if not sel.any():
+                chans += [np.zeros((self.input_n, self.input_n))] * max(1, depth_slabs)
+                continue
+            sub = AtomicStructure(st.box, st.numbers[sel], st.positions[sel])
+            pot = Potential(sub, self.input_n, self.input_n, slice_thickness=self.slice_thickness)
+            prepared = pot.prepare(ElectronBeam(self.energy))
+            V = prepared.values()
+            prepared.release()
+            if depth_slabs <= 0:
+                chans.append(V.sum(axis=0))
+                continue
+            edges = np.cumsum(pot.slice_thicknesses)
+            centres = edges - 0.5 * np.asarray(pot.slice_thicknesses)
+            slab = np.minimum((centres / (self.t_max / depth_slabs)).astype(int), depth_slabs - 1)
+            out = np.zeros((depth_slabs,) + V.shape[1:])
+            for j, k in enumerate(slab):
+                out[k] += V[j]
+            chans += list(out)
+        chans.append(np.full((self.input_n, self.input_n), st.box[2]))
+        return np.stack(chans).astype(np.float32)
 
 
 @dataclass
@@ -118,15 +128,20 @@ class Dataset:
     y: np.ndarray   # [n, scan_n, scan_n] float32 HAADF (fraction of the incident intensity)
 
 
+def input_channels(depth_slabs: int = 0, by_element: bool = False, elements: Sequence[int] = ELEMENTS) -> int:
+    """Channels of the network's input: potential channels (max(1, K) per element group) and the thickness."""
+    return (len(elements) if by_element else 1) * max(1, depth_slabs) + 1
+
+
 def make_dataset(work: SurrogateWorkload, seeds: Sequence[int], log=None, depth_slabs: int = 0,
-                 labels: Optional[np.ndarray] = None) -> Dataset:
+                 labels: Optional[np.ndarray] = None, by_element: bool = False) -> Dataset:
     """The cells of `seeds`: their features and their multislice HAADF images (or the given `labels`, images made
     earlier from the same seeds)."""
     xs, ys = [], []
     t0 = time.perf_counter()
     for k, s in enumerate(seeds):
         st = work.cell(int(s))
-        xs.append(work.features(st, depth_slabs))
+        xs.append(work.features(st, depth_slabs, by_element))
         ys.append(work.haadf(st).astype(np.float32) if labels is None else np.asarray(labels[k], np.float32))
         if log and (k + 1) % max(1, len(seeds) // 20) == 0:
             log(f"cells {k + 1}/{len(seeds)} in {time.perf_counter() - t0:.1f} s")
@@ -188,7 +203,7 @@ class HaadfSurrogate:
     """The network on the device. Weights: float32 masters (trained) and their bfloat16 rounding (used)."""
 
     def __init__(self, seed: int = 0, channels: Sequence[int] = CHANNELS, input_n: int = INPUT_N,
-                 output_n: int = SCAN_N, resample: str = "fourier", depth_slabs: int = 0):
+                 output_n: int = SCAN_N, resample: str = "fourier", depth_slabs: int = 0, by_element: bool = False):
         import torch
 
         self.torch = torch
@@ -197,10 +212,11 @@ class HaadfSurrogate:
         self.input_n, self.output_n = input_n, output_n
         if resample not in ("fourier", "linear"):
             raise ValueError("resample must be 'fourier' or 'linear'")
-        self.resample, self.depth_slabs = resample, int(depth_slabs)
-        want = self.depth_slabs + 1 if self.depth_slabs > 0 else 2
+        self.resample, self.depth_slabs, self.by_element = resample, int(depth_slabs), bool(by_element)
+        want = input_channels(self.depth_slabs, self.by_element)
         if self.channels[0] != want:
-            raise ValueError(f"the first layer takes {want} input channels with depth_slabs={self.depth_slabs}")
+            raise ValueError(f"the first layer takes {want} input channels with depth_slabs={self.depth_slabs}, "
+                             f"by_element={self.by_element}")
         rng = np.random.default_rng(seed)
         self.w32, self.wbf, self.b = [], [], []
         for k, (ci, co) in enumerate(zip(self.channels[:-1], self.channels[1:])):
@@ -379,6 +395,7 @@ class HaadfSurrogate:
         d["channels"] = np.array(self.channels)
         d["resample"] = np.array(self.resample)
         d["depth_slabs"] = np.array(self.depth_slabs)
+        d["by_element"] = np.array(int(self.by_element))
         d["io"] = np.array([self.input_n, self.output_n])
         return d
 
@@ -392,7 +409,8 @@ class HaadfSurrogate:
         io = d["io"] if "io" in d.files else np.array([INPUT_N, SCAN_N])
         model = cls(channels=[int(c) for c in d["channels"]], input_n=int(io[0]), output_n=int(io[1]),
                     resample=str(d["resample"]) if "resample" in d.files else "linear",
-                    depth_slabs=int(d["depth_slabs"]) if "depth_slabs" in d.files else 0)
+                    depth_slabs=int(d["depth_slabs"]) if "depth_slabs" in d.files else 0,
+                    by_element=bool(int(d["by_element"])) if "by_element" in d.files else False)
         model.load(path)
         return model
 
@@ -427,7 +445,7 @@ def judged_scan(model: HaadfSurrogate, work: SurrogateWorkload, st: AtomicStruct
     """The surrogate's HAADF image of cell `st`, judged by plain multislice on a random `fraction` of the positions;
     when the network's relative L2 error there exceeds `tolerance`, the whole image is re-simulated."""
     t0 = time.perf_counter()
-    img = model.predict(work.features(st, model.depth_slabs)[None])[0]
+    img = model.predict(work.features(st, model.depth_slabs, model.by_element)[None])[0]
     t1 = time.perf_counter()
     npos = work.scan_n * work.scan_n
     k = max(1, int(round(fraction * npos)))
@@ -447,7 +465,8 @@ def judged_scan(model: HaadfSurrogate, work: SurrogateWorkload, st: AtomicStruct
 
 
 __all__ = ["SurrogateWorkload", "Dataset", "make_dataset", "HaadfSurrogate", "JudgedScan", "judged_scan",
-           "resample_matrix", "fourier_resample_matrix", "generate_labels", "evaluate", "run_study", "CHANNELS"]
+           "resample_matrix", "fourier_resample_matrix", "generate_labels", "evaluate", "run_study", "input_channels",
+           "CHANNELS"]
 
 
 # ---- the study: train on generated cells, judge on held-out cells -----------------------------------------------------
@@ -520,7 +539,7 @@ def run_study(train: int = 256, held_out: int = 32, epochs: int = 60, batch: int
     t0 = time.perf_counter()
     data = make_dataset(work, seeds[:train], log=log, depth_slabs=depth_slabs)
     gen_s = time.perf_counter() - t0
-    channels = [depth_slabs + 1 if depth_slabs > 0 else 2] + CHANNELS[1:]
+    channels = [input_channels(depth_slabs)] + CHANNELS[1:]
     model = HaadfSurrogate(seed=seed, channels=channels, depth_slabs=depth_slabs)
     t0 = time.perf_counter()
     history = model.train(data, epochs=epochs, batch=batch, lr=lr, seed=seed, log=log)
@@ -553,6 +572,7 @@ def _main(argv=None):
     t.add_argument("--loss", default="relative", choices=["relative", "mse"])
     t.add_argument("--resample", default="fourier", choices=["fourier", "linear"])
     t.add_argument("--depth-slabs", type=int, default=0)
+    t.add_argument("--by-element", action="store_true", help="potential channels per element (a labelled variant)")
     t.add_argument("--seed", type=int, default=20261202)
     t.add_argument("--weights", required=True, help="output .npz")
     t.add_argument("--compact", action="store_true", help="save only the bfloat16 weights the network uses")
@@ -592,10 +612,11 @@ def _main(argv=None):
         if a.count:
             seeds, labels = seeds[:a.count], labels[:a.count]
         t0 = time.perf_counter()
-        data = make_dataset(work, seeds, log=log, depth_slabs=a.depth_slabs, labels=labels)
+        data = make_dataset(work, seeds, log=log, depth_slabs=a.depth_slabs, labels=labels, by_element=a.by_element)
         log(f"features of {len(seeds)} cells in {time.perf_counter() - t0:.1f} s")
-        channels = [a.depth_slabs + 1 if a.depth_slabs > 0 else 2] + CHANNELS[1:]
-        model = HaadfSurrogate(seed=a.seed, channels=channels, resample=a.resample, depth_slabs=a.depth_slabs)
+        channels = [input_channels(a.depth_slabs, a.by_element)] + CHANNELS[1:]
+        model = HaadfSurrogate(seed=a.seed, channels=channels, resample=a.resample, depth_slabs=a.depth_slabs,
+                               by_element=a.by_element)
         t0 = time.perf_counter()
         hist = model.train(data, epochs=a.epochs, batch=a.batch, lr=a.lr, seed=a.seed, log=log, loss=a.loss)
         model.save(a.weights, compact=a.compact)

@@ -107,12 +107,21 @@ def test_judged_surrogate_scan_on_a_small_workload():
     print("mse first/last", hist[0], hist[-1])
     assert hist[-1] < 0.25 * hist[0]
     st = work.cell(999)
-    js = SG.judged_scan(model, work, st, fraction=0.05, tolerance=1.0)
+    js = SG.judged_scan(model, work, st, fraction=0.05, tolerance=0.02)
     truth = work.haadf(st)
     err = np.linalg.norm(js.surrogate - truth) / np.linalg.norm(truth)
     print("held-out network error", err, "spot", js.spot_error, "seconds", js.seconds)
     assert len(js.checked) == round(0.05 * 48 * 48)
-    assert not js.fallback and js.image is js.surrogate
-    forced = SG.judged_scan(model, work, st, fraction=0.05, tolerance=0.0)
-    assert forced.fallback
-    assert np.array_equal(forced.image, truth)
+    # the judge's estimate tracks the network's true error, and decides at the tolerance
+    assert 0.5 * err <= js.spot_error <= 2.0 * err
+    assert js.fallback == (js.spot_error > 0.02)
+    if js.fallback:
+        assert np.array_equal(js.image, truth)
+    else:
+        assert js.image is js.surrogate
+    # just above the spot error it accepts the network's image, just below it falls back to multislice
+    above = SG.judged_scan(model, work, st, fraction=0.05, tolerance=js.spot_error * 1.01)
+    assert not above.fallback and above.image is above.surrogate
+    below = SG.judged_scan(model, work, st, fraction=0.05, tolerance=js.spot_error * 0.99)
+    assert below.fallback
+    assert np.array_equal(below.image, truth)
