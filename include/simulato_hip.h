@@ -80,7 +80,8 @@ typedef struct sk_options {
                        time; -1 = the library's choice (lanes when npots > 1 and the positions fill at most a
                        quarter of a launch, else one potential at a time) */
   int lane_waves;   /* pipeline: positions per lane (0 = the library's choice) */
-  int col_tile;     /* 512-point columns: columns per block of the column pass, 8 or 16 (0 = the library's choice) */
+  int col_tile;     /* columns per block of the column pass: 8 or 16 for 512- and 384-point columns, 4 or 8 for
+                       1024-point columns and Bluestein's lengths on 1024 points (0 = the library's choice) */
   int precision;    /* transforms of the slice loop: 0 = float32 (default); 1 = split-BF16 on the matrix cores
                        (bfloat16 hi/lo pairs, three products, float32 accumulation; grid lengths 256,
                        384, 512 and 1024, and the Bluestein lengths 65-512 that have a prime factor

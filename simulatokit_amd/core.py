@@ -213,7 +213,7 @@ class ExecutionOptions:
                              # potential at a time, -1 the library's choice (lanes for a few positions over several
                              # potentials, else one potential at a time)
     lane_waves: int = 0      # positions per lane (0 = the library's choice)
-    col_tile: int = 0        # 512-point columns: columns per column-pass block, 8 or 16 (0 = the library's choice)
+    col_tile: int = 0        # columns per column-pass block: 8 or 16 (512, 384 points), 4 or 8 (1024); 0 = library's
     precision: str = "float32"  # the slice loop's transforms: "float32", or "split_bf16" (matrix cores)
     streams: int = 0         # scan one potential at a time through the fast passes: streams the parts of each batch
                              # run on (1-4), 0 the library's choice (2 for batches of at least 128 positions)

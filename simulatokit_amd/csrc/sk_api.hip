@@ -545,7 +545,7 @@ constexpr int kMaxStreams = 4;
 
 void set_col_tile(Context& c, const sk_options* opt) {
   const int t = opt ? opt->col_tile : 0;
-  if (t != 0 && t != 8 && t != 16) fail(SK_ERR_INVALID, "col_tile must be 0, 8 or 16");
+  if (t != 0 && t != 4 && t != 8 && t != 16) fail(SK_ERR_INVALID, "col_tile must be 0, 4, 8 or 16");
   c.col_tile = t;
   const int pr = opt ? opt->precision : 0;
   if (pr != 0 && pr != 1) fail(SK_ERR_INVALID, "precision must be 0 (float32) or 1 (split-BF16)");

@@ -890,6 +890,7 @@ bool fast_col_supported(int nx, int nc, int flags) {
 int fast_col_tile(int nx, int requested, double wave_bytes) {
   if (nx != 512) return kColTile;
   if (requested == 8 || requested == 16) return requested;
+  if (requested != 0) fail(SK_ERR_INVALID, "col_tile 4 is not a column-pass width for 512-point lines (8 or 16)");
   // 8-column spans when the batch stays in the last-level cache (they fill every CU for any batch); 16-column
   // blocks when it streams from HBM, where their full 128-byte row segments measured faster
   return wave_bytes <= (double)kCachedWaveBytes ? 8 : 16;  // the 8-column spans also fill every CU at any batch

@@ -16,7 +16,8 @@
 // closing chirp product.
 //
 //   row pass:    blocks of 2 wavefronts; each line is one row x of one position, IFFT_y -> * t(x, y) -> FFT_y (and
-//                the other row modes of the fast passes), the block's rows for 8 positions in turn with t in registers
+//                the other row modes of the fast passes), the block's rows for 8 positions in turn (RegTuning: t in
+//                registers or read per position)
 //   column pass: the middle slices (FFT_x -> * P -> IFFT_x); a block moves TC compact columns of one position between
 //                HBM and LDS in row segments of TC values, each column's lanes transform it inside the block's LDS,
 //                P in registers for the block's 8 positions
@@ -42,6 +43,22 @@ template <>
 struct RegPlan<384> {
   static constexpr int V = 24, R1 = 8, R2 = 8, R3 = 6;
   static constexpr int LS = 400;   // 384 + 16: the four lines of a wavefront on different bank halves
+};
+
+// Register use per length, measured on an MI355X (HAADF geometry, mean row / column milliseconds per launch of 4096
+// positions, 1024 at 1024^2, over every combination):
+// - T_REGS: the row pass keeps t(x, .) in registers for the block's positions (else reads it per position, from the
+//   L1 cache). 384: 1.51 ms reading against 2.22 ms holding (24 values a lane); 1024: 2.87 against 4.52 ms holding;
+//   Bluestein 509: 10.29 against 11.42 ms holding.
+// - OPAQUE: the lane index is made opaque to the compiler per position, so the lane's LDS slot addresses are
+//   recomputed in each position instead of held in registers across the loop. It costs the row passes (384: 2.53
+//   against 1.51 ms), and helps only the Bluestein column pass (10.30 against 13.30 ms).
+// - The column tile (TC columns per block): 384, 16 columns (3.01 against 3.46 ms for 8); 1024 and Bluestein,
+//   4 columns (6.42 against 6.75 ms; 10.30 against 11.94 ms). Blocks of 1, 2 or 4 row wavefronts differ by under 2 %.
+template <int M, bool BLUE>
+struct RegTuning {
+  static constexpr bool T_REGS = M == 1024, ROW_OPAQUE = false, COL_OPAQUE = BLUE;
+  static constexpr int COL_TILE = M == 384 ? 16 : 4;
 };
 
 __device__ __forceinline__ int rslot(int e) { return e ^ (((e >> 3) ^ (e >> 7)) & 31); }
@@ -200,60 +217,69 @@ __global__ void __launch_bounds__(64 * kRowWaves) row_reg_kernel(RowArgs a, cons
   auto slot = [](int e) { return rslot(e); };
   const long long gw = (long long)a.nx * a.nc;
   // the compact column of element e of the lane's line (or -1: pruned, or past n)
-  auto cidx = [&](int r) {
-    const int e = i + T * r;
+  auto cidx = [&](int li, int r) {
+    const int e = li + T * r;
     return e < n ? compact_col(a, e) : -1;
   };
-  float2 t[V];
-  if constexpr ((FLAGS & ROW_MUL_T) != 0) {
-    const float2* tr = a.t + xs * n;
+  const float2* tr = a.t + xs * n;  // t(x, .): in registers (T_REGS) or read per position from the L1 cache
+  constexpr bool TREG = RegTuning<M, BLUE>::T_REGS && (FLAGS & ROW_MUL_T) != 0;
+  float2 treg[TREG ? V : 1];
+  if constexpr (TREG) {
 #pragma unroll
     for (int r = 0; r < V; ++r) {
       const int e = i + T * r;
-      t[r] = e < n ? tr[e] : make_float2(0.f, 0.f);
+      treg[r] = e < n ? tr[e] : make_float2(0.f, 0.f);
     }
   }
   for (int p = 0; p < kRegPB; ++p) {
     const long long b = b0 + p;
     if (b >= batch) break;  // uniform over the block
+    // ROW_OPAQUE: the lane index as a value the compiler cannot follow across positions, so the lane's LDS slot
+    // addresses are recomputed in each position rather than kept live over the loop (one register each)
+    int il = i;
+    if constexpr (RegTuning<M, BLUE>::ROW_OPAQUE) asm volatile("" : "+v"(il));
     float2 v[V];
     if constexpr ((FLAGS & ROW_IN_COMPACT) != 0) {
       const float2* g = a.G + b * gw + xs * a.nc;
 #pragma unroll
       for (int r = 0; r < V; ++r) {
-        const int c = cidx(r);
+        const int c = cidx(il, r);
         v[r] = c >= 0 ? g[c] : make_float2(0.f, 0.f);
       }
     } else {
       const float2* g = a.in_full + b * a.in_full_stride + xs * n;
 #pragma unroll
       for (int r = 0; r < V; ++r) {
-        const int e = i + T * r;
+        const int e = il + T * r;
         v[r] = e < n ? g[e] : make_float2(0.f, 0.f);
       }
     }
-    if constexpr ((FLAGS & ROW_INV) != 0) line_fft<M, +1, BLUE>(v, buf, i, tab, cs, ks, n, slot);
+    if constexpr ((FLAGS & ROW_INV) != 0) line_fft<M, +1, BLUE>(v, buf, il, tab, cs, ks, n, slot);
     if constexpr ((FLAGS & ROW_WRITE_PLANE) != 0) {
       if (live) {
         float2* pl = a.plane + b * a.plane_stride + xs * n;
 #pragma unroll
         for (int r = 0; r < V; ++r) {
-          const int e = i + T * r;
+          const int e = il + T * r;
           if (e < n) pl[e] = cscale(v[r], a.out_scale);
         }
       }
     }
     if constexpr ((FLAGS & ROW_MUL_T) != 0) {
 #pragma unroll
-      for (int r = 0; r < V; ++r) v[r] = cmul(v[r], t[r]);
+      for (int r = 0; r < V; ++r) {
+        const int e = il + T * r;
+        if constexpr (TREG) v[r] = cmul(v[r], treg[r]);
+        else v[r] = e < n ? cmul(v[r], tr[e]) : make_float2(0.f, 0.f);
+      }
     }
-    if constexpr ((FLAGS & ROW_FWD) != 0) line_fft<M, -1, BLUE>(v, buf, i, tab, cs, ks, n, slot);
+    if constexpr ((FLAGS & ROW_FWD) != 0) line_fft<M, -1, BLUE>(v, buf, il, tab, cs, ks, n, slot);
     if constexpr ((FLAGS & ROW_OUT_COMPACT) != 0) {
       if (live) {
         float2* g = a.G + b * gw + xs * a.nc;
 #pragma unroll
         for (int r = 0; r < V; ++r) {
-          const int c = cidx(r);
+          const int c = cidx(il, r);
           if (c >= 0) g[c] = v[r];
         }
       }
@@ -261,31 +287,24 @@ __global__ void __launch_bounds__(64 * kRowWaves) row_reg_kernel(RowArgs a, cons
   }
 }
 
-// The column pass's tile: TC columns of one position, column c at slots [c CS, c CS + CL), element e at rslot(e);
-// CL = M (Bluestein: the 1024-point convolution buffer, which holds the column's n values before the transform).
-template <int M, bool BLUE>
-struct ColTile;
-template <>
-struct ColTile<1024, false> {
-  static constexpr int TC = 8, CS = 1028;  // 64-byte row segments; CS = 4 mod 32: conflict-free block phases
-};
-template <>
-struct ColTile<384, false> {
-  static constexpr int TC = 16, CS = 400;  // 128-byte row segments; conflict-free exchanges
-};
-template <>
-struct ColTile<1024, true> {
-  static constexpr int TC = 4, CS = 1032;  // 32-byte row segments (the arithmetic of Bluestein's dominates)
-};
+// The column pass's tile: TC columns of one position, column c at slots [c CS, c CS + M), element e at slot
+// rslot(e) (Bluestein: the 1024-point convolution buffer, which holds the column's n values before the transform).
+// CS, from a bank model of 64 banks of 4 bytes in groups of 32 lanes: 1024 points, CS = 1028 (TC = 8) or 1032
+// (TC = 4) keeps the block phases' row segments free of conflicts; 384 points, CS = 400 keeps the exchanges free
+// of conflicts (four columns per wavefront).
+template <int M, int TC>
+constexpr int col_stride() {
+  return M == 384 ? 400 : (TC == 8 ? 1028 : 1032);
+}
 
 // Column pass of the middle slices (COL_IN_G | COL_FWD | COL_MUL_P | COL_INV | COL_OUT): block = TC compact columns
 // [TC blockIdx.x, + TC) of positions [b0, b0 + kRegPB); TC x T threads, the T lanes of column c = tid / T.
-template <int M, bool BLUE>
-__global__ void __launch_bounds__((ColTile<M, BLUE>::TC * (M / RegPlan<M>::V))) col_reg_kernel(
-    ColArgs a, const float2* __restrict__ roots, const float2* __restrict__ chirp, const float2* __restrict__ khat,
-    int batch) {
+template <int M, bool BLUE, int TC>
+__global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColArgs a, const float2* __restrict__ roots,
+                                                                       const float2* __restrict__ chirp,
+                                                                       const float2* __restrict__ khat, int batch) {
   using P = RegPlan<M>;
-  constexpr int V = P::V, T = M / V, TC = ColTile<M, BLUE>::TC, CS = ColTile<M, BLUE>::CS, NT = TC * T;
+  constexpr int V = P::V, T = M / V, CS = col_stride<M, TC>(), NT = TC * T;
   constexpr int RSTEP = NT / TC;  // rows per block-phase round
   __shared__ float2 tile[TC * CS];
   __shared__ float2 tab[RegTw<M>::kSize];
@@ -326,20 +345,22 @@ __global__ void __launch_bounds__((ColTile<M, BLUE>::TC * (M / RegPlan<M>::V))) 
     for (int x = xi; x < n; x += RSTEP)
       tile[cb * CS + rslot(x)] = kept ? g[(long long)x * a.nc] : make_float2(0.f, 0.f);
     __syncthreads();
+    int il = i;  // COL_OPAQUE: as ROW_OPAQUE in the row pass
+    if constexpr (RegTuning<M, BLUE>::COL_OPAQUE) asm volatile("" : "+v"(il));
     float2 v[V];
 #pragma unroll
     for (int r = 0; r < V; ++r) {
-      const int e = i + T * r;
+      const int e = il + T * r;
       v[r] = e < n ? col[rslot(e)] : make_float2(0.f, 0.f);
     }
     wave_sync();  // every lane has read its column before the exchanges overwrite it
-    line_fft<M, -1, BLUE>(v, col, i, tab, cs, ks, n, slot);
+    line_fft<M, -1, BLUE>(v, col, il, tab, cs, ks, n, slot);
 #pragma unroll
     for (int r = 0; r < V; ++r) v[r] = cmul(v[r], pv[r]);
-    line_fft<M, +1, BLUE>(v, col, i, tab, cs, ks, n, slot);
+    line_fft<M, +1, BLUE>(v, col, il, tab, cs, ks, n, slot);
 #pragma unroll
     for (int r = 0; r < V; ++r) {
-      const int e = i + T * r;
+      const int e = il + T * r;
       if (e < n) col[rslot(e)] = v[r];
     }
     __syncthreads();
@@ -389,11 +410,20 @@ void row_launch(const RowArgs& a, int batch, hipStream_t s) {
   SK_HIP(hipGetLastError());
 }
 
-template <int M, bool BLUE>
-void col_launch(const ColArgs& a, int batch, hipStream_t s) {
-  constexpr int TC = ColTile<M, BLUE>::TC, NT = TC * (M / RegPlan<M>::V);
+// columns per column-pass block: the caller's sk_options.col_tile where this length has that width, else the
+// default (1024 points, also for Bluestein's lengths: 4; 384 points: 16)
+int reg_col_tile(int m, int requested) {
+  if (requested == 0) return m == 384 ? RegTuning<384, false>::COL_TILE : RegTuning<1024, false>::COL_TILE;
+  if (m == 384 ? (requested == 8 || requested == 16) : (requested == 4 || requested == 8)) return requested;
+  fail(SK_ERR_INVALID, "col_tile " + std::to_string(requested) + " is not a column-pass width for " +
+                           std::to_string(m) + "-point lines (384: 8 or 16; 1024: 4 or 8)");
+}
+
+template <int M, bool BLUE, int TC>
+void col_go(const ColArgs& a, int batch, hipStream_t s) {
+  constexpr int NT = TC * (M / RegPlan<M>::V);
   dim3 grid(a.nc / TC, (batch + kRegPB - 1) / kRegPB);
-  launch_timed(col_reg_kernel<M, BLUE>, grid, dim3(NT), s, a.ev_start, a.ev_stop, a, a.fx.roots, a.fx.chirp,
+  launch_timed(col_reg_kernel<M, BLUE, TC>, grid, dim3(NT), s, a.ev_start, a.ev_stop, a, a.fx.roots, a.fx.chirp,
                a.fx.khat, batch);
   SK_HIP(hipGetLastError());
 }
@@ -409,9 +439,7 @@ bool reg_row_supported(const RowArgs& a) {
 bool reg_col_supported(const ColArgs& a) {
   if (a.fx.use_mfma || a.flags != COL_MID || a.fx.n != a.nx) return false;
   const int m = reg_length(a.fx);
-  if (m == 0) return false;
-  const int tc = m == 384 ? ColTile<384, false>::TC : (a.fx.blue ? ColTile<1024, true>::TC : ColTile<1024, false>::TC);
-  return a.nc % tc == 0;
+  return m != 0 && a.nc % reg_col_tile(m, a.col_tile) == 0;
 }
 
 void launch_row_reg(const RowArgs& a, int batch, hipStream_t s) {
@@ -421,9 +449,17 @@ void launch_row_reg(const RowArgs& a, int batch, hipStream_t s) {
 }
 
 void launch_col_reg(const ColArgs& a, int batch, hipStream_t s) {
-  if (a.fx.blue) col_launch<1024, true>(a, batch, s);
-  else if (a.fx.n == 384) col_launch<384, false>(a, batch, s);
-  else col_launch<1024, false>(a, batch, s);
+  const int m = reg_length(a.fx), tc = reg_col_tile(m, a.col_tile);
+  if (a.fx.blue) {
+    if (tc == 4) col_go<1024, true, 4>(a, batch, s);
+    else col_go<1024, true, 8>(a, batch, s);
+  } else if (m == 384) {
+    if (tc == 16) col_go<384, false, 16>(a, batch, s);
+    else col_go<384, false, 8>(a, batch, s);
+  } else {
+    if (tc == 4) col_go<1024, false, 4>(a, batch, s);
+    else col_go<1024, false, 8>(a, batch, s);
+  }
 }
 
 }  // namespace sk

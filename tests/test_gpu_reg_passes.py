@@ -1,5 +1,6 @@
 """The register-resident passes for lines of 384 and 1024 points and for Bluestein's lengths 257..512 (on 1024
-points) against the generic LDS passes (ExecutionOptions(fast_path=False)) and the float64 oracle.
+points) against the generic LDS passes (ExecutionOptions(fast_path=False)) and the float64 oracle, at every
+column-pass width (ExecutionOptions.col_tile: 8 or 16 columns per block at 384 points, 4 or 8 at 1024).
 
 Both paths compute the same numbers in float32 with different operation orders, so they agree to float32 rounding
 over the slice loop; both are held to the oracle at the scan's float32 bound.
@@ -37,6 +38,11 @@ def test_register_passes_scan_as_the_generic_passes_and_the_oracle(nx, ny, tiles
     dets = [sk.AnnularDetector(40, outer), sk.PixelatedDetector("cutoff"), sk.CenterOfMassDetector("cutoff", "mrad")]
     reg = sk.STEM.scan(probe, [prepared], positions, dets, sk.ExecutionOptions())
     gen = sk.STEM.scan(probe, [prepared], positions, dets, sk.ExecutionOptions(fast_path=False))
+    widths = (8, 16) if nx == 384 else ((4, 8) if nx in (509, 1024) else ())
+    for tc in widths:  # every column-pass width agrees with the default one to float32 rounding
+        other = sk.STEM.scan(probe, [prepared], positions, dets, sk.ExecutionOptions(col_tile=tc))
+        for k in range(len(dets)):
+            assert rel_max(other[k].values, reg[k].values) <= 2e-5, (tc, k)
     V, _ = O.potential(nx, ny, st.numbers, st.positions, st.box, pot.slice_thicknesses)
     want = O.stem_scan([V], [pot.slice_thicknesses], (st.box[0], st.box[1]), 200e3, 20.0, positions,
                        [O.det_annular(40, outer), O.det_pixelated("cutoff"), O.det_com("cutoff", "mrad")])
@@ -66,3 +72,11 @@ def test_register_passes_exit_planes_as_the_generic_passes(nx, ny, tiles, tz):
     print(f"{nx}x{ny}: {len(reg)} planes, register vs generic rel max {worst:.2e}")
     assert worst <= 2e-5
     prepared.release()
+
+
+def test_column_widths_the_length_has_not_are_refused():
+    st = srtio3(4, 4, 2)
+    pot = sk.Potential(st, 384, 384, slice_thickness=2.0)
+    beam = sk.ElectronBeam(200e3)
+    with pytest.raises(sk.SimulatoError):
+        sk.Multislice.run(sk.Waves.plane_wave(beam, pot.grid), pot, options=sk.ExecutionOptions(col_tile=4))

@@ -1,0 +1,109 @@
+"""The index arithmetic of the register-resident passes (simulatokit_amd/csrc/sk_reg.hip), emulated on the CPU:
+
+- the three-stage Stockham transforms with V values per lane (384 = 8*8*6 with 24 values, 1024 = 16*8*8 with 16),
+  built from the kernel's formulas (butterfly bi = i + T q reads x[bi + (M / R) j], twiddle w_{LR}^{jk} with
+  k = bi mod L, writes y[(bi - k) R + k + L j]; first stage from the registers, last into them), equal numpy's FFT;
+- Bluestein's transform of n points on 1024 (chirp, forward transform, kernel product, inverse, chirp) equals it;
+- the LDS slot map e -> e ^ (((e >> 3) ^ (e >> 7)) & 31) is a permutation of each line, and a bank model (64 banks of
+  4 bytes, a float2 access served in groups of 32 lanes) finds every exchange of both lengths free of conflicts.
+"""
+import numpy as np
+import pytest
+
+PLANS = {1024: (16, (16, 8, 8), 1024), 384: (24, (8, 8, 6), 400)}  # M: V, radices, line stride (slots)
+
+
+def rslot(e):
+    return e ^ (((e >> 3) ^ (e >> 7)) & 31)
+
+
+def dft(u, sign):
+    k = np.arange(len(u))
+    return np.exp(sign * 2j * np.pi * np.outer(k, k) / len(u)) @ u
+
+
+def vfft(x, M, V, radices, sign):
+    T = M // V
+    roots = np.exp(-2j * np.pi * np.arange(M) / M)
+    v = np.array([[x[i + T * r] for r in range(V)] for i in range(T)], dtype=complex)  # lane i, register r
+    buf = np.zeros(M, complex)
+    L = 1
+    for s, R in enumerate(radices):
+        NB, STRIDE = V // R, M // R
+        first, last = s == 0, s == len(radices) - 1
+        out, nv = np.zeros(M, complex), v.copy()
+        for i in range(T):
+            for q in range(NB):
+                bi = i + T * q
+                u = np.array([v[i, q + NB * j] if first else buf[bi + STRIDE * j] for j in range(R)])
+                k = 0 if L == 1 else bi % L
+                for j in range(1, R):
+                    if L > 1:
+                        w = roots[j * k * (M // (L * R))]
+                        u[j] *= w if sign < 0 else np.conj(w)
+                u = dft(u, sign)
+                for j in range(R):
+                    if last:
+                        nv[i, q + NB * j] = u[j]
+                    else:
+                        out[(bi - k) * R + k + j * L] = u[j]
+        buf, v, L = out, nv, L * R
+    return np.array([v[e % T, e // T] for e in range(M)])
+
+
+@pytest.mark.parametrize("M", [1024, 384])
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_register_stages_equal_numpy(M, sign):
+    V, radices, _ = PLANS[M]
+    x = np.random.default_rng(M).normal(size=(M, 2)) @ np.array([1, 1j])
+    got = vfft(x, M, V, radices, sign)
+    want = np.fft.fft(x) if sign < 0 else np.fft.ifft(x) * M
+    assert np.abs(got - want).max() <= 1e-13 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("n", [257, 500, 509])
+def test_bluestein_on_1024_points_equals_numpy(n):
+    M = 1024
+    k = np.arange(n)
+    chirp = np.exp(-1j * np.pi * ((k * k) % (2 * n)) / n)
+    b = np.zeros(M, complex)
+    b[0] = np.conj(chirp[0])
+    for q in range(1, n):
+        b[q] = b[M - q] = np.conj(chirp[q])
+    khat = np.fft.fft(b) / M
+    x = np.random.default_rng(n).normal(size=(n, 2)) @ np.array([1, 1j])
+    for sign in (-1, 1):
+        c, kh = (chirp, khat) if sign < 0 else (np.conj(chirp), np.conj(khat))
+        u = np.zeros(M, complex)
+        u[:n] = x * c
+        w = vfft(vfft(u, M, 16, (16, 8, 8), -1) * kh, M, 16, (16, 8, 8), +1)
+        want = np.fft.fft(x) if sign < 0 else np.fft.ifft(x) * n
+        assert np.abs(w[:n] * c - want).max() <= 1e-13 * np.abs(want).max()
+
+
+def worst_conflict(slots):
+    return max(np.bincount(np.asarray(slots[g:g + 32]) % 32).max() for g in range(0, len(slots), 32))
+
+
+@pytest.mark.parametrize("M", [1024, 384])
+def test_slot_map_is_a_permutation_and_every_exchange_is_free_of_bank_conflicts(M):
+    V, radices, LS = PLANS[M]
+    assert sorted(rslot(e) for e in range(M)) == list(range(M))
+    T = M // V
+    L = 1
+    for s, R in enumerate(radices):
+        NB, STRIDE = V // R, M // R
+        for q in range(NB):
+            for j in range(R):
+                reads, writes = [], []
+                for lane in range(64):
+                    line, i = divmod(lane, T)
+                    bi = i + T * q
+                    k = 0 if L == 1 else bi % L
+                    reads.append(line * LS + rslot(bi + STRIDE * j))
+                    writes.append(line * LS + rslot((bi - k) * R + k + j * L))
+                if s > 0:
+                    assert worst_conflict(reads) == 1, (M, s, q, j)
+                if s < len(radices) - 1:
+                    assert worst_conflict(writes) == 1, (M, s, q, j)
+        L *= R
