@@ -53,12 +53,13 @@ struct RegPlan<384> {
 // - OPAQUE: the lane index is made opaque to the compiler per position, so the lane's LDS slot addresses are
 //   recomputed in each position instead of held in registers across the loop. It costs the row passes (384: 2.53
 //   against 1.51 ms), and helps only the Bluestein column pass (10.30 against 13.30 ms).
-// - The column tile (TC columns per block): 384, 16 columns (3.01 against 3.46 ms for 8); 1024 and Bluestein,
-//   4 columns (6.42 against 6.75 ms; 10.30 against 11.94 ms). Blocks of 1, 2 or 4 row wavefronts differ by under 2 %.
+// - The column tile (TC columns per block), with the next position's loads in flight during the transforms: 384,
+//   8 columns (2.28 against 2.32 ms for 16); 1024, 8 columns (3.95 against 5.36 ms for 4); Bluestein, 4 columns
+//   (9.27 against 10.61 ms for 8). Blocks of 1, 2 or 4 row wavefronts differ by under 2 %.
 template <int M, bool BLUE>
 struct RegTuning {
   static constexpr bool T_REGS = M == 1024, ROW_OPAQUE = false, COL_OPAQUE = BLUE;
-  static constexpr int COL_TILE = M == 384 ? 16 : 4;
+  static constexpr int COL_TILE = BLUE ? 4 : 8;
 };
 
 __device__ __forceinline__ int rslot(int e) { return e ^ (((e >> 3) ^ (e >> 7)) & 31); }
@@ -297,6 +298,17 @@ constexpr int col_stride() {
   return M == 384 ? 400 : (TC == 8 ? 1028 : 1032);
 }
 
+// The tile slot of element e of column c. At 384 points the element slots of column c are XORed with c (16-column
+// tiles) or 4 c (8-column tiles): the block phases' row segments then fall on distinct banks, which CS = 400 alone
+// does not give (a model of 64 banks finds the 16-column row segments 8-way conflicted without it, the 8-column ones
+// 4-way, and both free with it, the exchanges free either way; measured 4.7e8 bank-conflict cycles per 4096-position
+// launch of 16-column tiles without it).
+template <int M, int TC>
+__device__ __forceinline__ int tile_elem(int e, int c) {
+  if constexpr (M == 384) return rslot(e) ^ ((TC == 8 ? 4 * c : c) & 31);
+  else return rslot(e);
+}
+
 // Column pass of the middle slices (COL_IN_G | COL_FWD | COL_MUL_P | COL_INV | COL_OUT): block = TC compact columns
 // [TC blockIdx.x, + TC) of positions [b0, b0 + kRegPB); TC x T threads, the T lanes of column c = tid / T.
 template <int M, bool BLUE, int TC>
@@ -323,7 +335,7 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
   // column lanes: column wc, lane i
   const int wc = tid / T, i = tid - wc * T;
   float2* col = tile + wc * CS;
-  auto slot = [](int e) { return rslot(e); };
+  auto slot = [wc](int e) { return tile_elem<M, TC>(e, wc); };
   const long long gw = (long long)a.nx * a.nc;
   // P of the lane's elements of column c0 + wc, for every position of the block
   float2 pv[V];
@@ -337,13 +349,31 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
   }
   const bool kept = a.keep[c0 + cb] >= 0;  // padding columns are never read nor written
   const int b0 = blockIdx.y * kRegPB;
+  // The block phase's values of the next position are loaded into registers while this position transforms: the
+  // loads' latency then overlaps the arithmetic (measured: 1024 points, 8 columns, 6.76 -> 3.95 ms per launch of
+  // 1024 positions; 384 points 3.03 -> 2.28 ms per 4096).
+  constexpr int NL = (M + RSTEP - 1) / RSTEP;
+  float2 pre[NL];
+  auto prefetch = [&](long long bb) {
+    const float2* gg = a.G + bb * gw + c0 + cb;
+#pragma unroll
+    for (int r = 0; r < NL; ++r) {
+      const int x = xi + RSTEP * r;
+      pre[r] = (kept && x < n) ? gg[(long long)x * a.nc] : make_float2(0.f, 0.f);
+    }
+  };
+  if (b0 < batch) prefetch(b0);
   for (int p = 0; p < kRegPB; ++p) {
     const long long b = b0 + p;
     if (b >= batch) break;  // uniform over the block
     float2* g = a.G + b * gw + c0 + cb;
     __syncthreads();  // the tile's previous position is stored (and, first, the tables are built)
-    for (int x = xi; x < n; x += RSTEP)
-      tile[cb * CS + rslot(x)] = kept ? g[(long long)x * a.nc] : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int r = 0; r < NL; ++r) {
+      const int x = xi + RSTEP * r;
+      if (x < n) tile[cb * CS + tile_elem<M, TC>(x, cb)] = pre[r];
+    }
+    if (p + 1 < kRegPB && b + 1 < batch) prefetch(b + 1);
     __syncthreads();
     int il = i;  // COL_OPAQUE: as ROW_OPAQUE in the row pass
     if constexpr (RegTuning<M, BLUE>::COL_OPAQUE) asm volatile("" : "+v"(il));
@@ -351,7 +381,7 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
 #pragma unroll
     for (int r = 0; r < V; ++r) {
       const int e = il + T * r;
-      v[r] = e < n ? col[rslot(e)] : make_float2(0.f, 0.f);
+      v[r] = e < n ? col[slot(e)] : make_float2(0.f, 0.f);
     }
     wave_sync();  // every lane has read its column before the exchanges overwrite it
     line_fft<M, -1, BLUE>(v, col, il, tab, cs, ks, n, slot);
@@ -361,11 +391,11 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
 #pragma unroll
     for (int r = 0; r < V; ++r) {
       const int e = il + T * r;
-      if (e < n) col[rslot(e)] = v[r];
+      if (e < n) col[slot(e)] = v[r];
     }
     __syncthreads();
     if (kept)
-      for (int x = xi; x < n; x += RSTEP) g[(long long)x * a.nc] = tile[cb * CS + rslot(x)];
+      for (int x = xi; x < n; x += RSTEP) g[(long long)x * a.nc] = tile[cb * CS + tile_elem<M, TC>(x, cb)];
   }
 }
 
@@ -411,9 +441,11 @@ void row_launch(const RowArgs& a, int batch, hipStream_t s) {
 }
 
 // columns per column-pass block: the caller's sk_options.col_tile where this length has that width, else the
-// default (1024 points, also for Bluestein's lengths: 4; 384 points: 16)
-int reg_col_tile(int m, int requested) {
-  if (requested == 0) return m == 384 ? RegTuning<384, false>::COL_TILE : RegTuning<1024, false>::COL_TILE;
+// default (RegTuning::COL_TILE)
+int reg_col_tile(int m, bool blue, int requested) {
+  if (requested == 0)
+    return m == 384 ? RegTuning<384, false>::COL_TILE
+                    : (blue ? RegTuning<1024, true>::COL_TILE : RegTuning<1024, false>::COL_TILE);
   if (m == 384 ? (requested == 8 || requested == 16) : (requested == 4 || requested == 8)) return requested;
   fail(SK_ERR_INVALID, "col_tile " + std::to_string(requested) + " is not a column-pass width for " +
                            std::to_string(m) + "-point lines (384: 8 or 16; 1024: 4 or 8)");
@@ -439,7 +471,7 @@ bool reg_row_supported(const RowArgs& a) {
 bool reg_col_supported(const ColArgs& a) {
   if (a.fx.use_mfma || a.flags != COL_MID || a.fx.n != a.nx) return false;
   const int m = reg_length(a.fx);
-  return m != 0 && a.nc % reg_col_tile(m, a.col_tile) == 0;
+  return m != 0 && a.nc % reg_col_tile(m, a.fx.blue, a.col_tile) == 0;
 }
 
 void launch_row_reg(const RowArgs& a, int batch, hipStream_t s) {
@@ -449,7 +481,7 @@ void launch_row_reg(const RowArgs& a, int batch, hipStream_t s) {
 }
 
 void launch_col_reg(const ColArgs& a, int batch, hipStream_t s) {
-  const int m = reg_length(a.fx), tc = reg_col_tile(m, a.col_tile);
+  const int m = reg_length(a.fx), tc = reg_col_tile(m, a.fx.blue, a.col_tile);
   if (a.fx.blue) {
     if (tc == 4) col_go<1024, true, 4>(a, batch, s);
     else col_go<1024, true, 8>(a, batch, s);

@@ -107,3 +107,43 @@ def test_slot_map_is_a_permutation_and_every_exchange_is_free_of_bank_conflicts(
                 if s < len(radices) - 1:
                     assert worst_conflict(writes) == 1, (M, s, q, j)
         L *= R
+
+
+@pytest.mark.parametrize("M,TC,CS", [(384, 16, 400), (384, 8, 400), (1024, 4, 1032), (1024, 8, 1028)])
+def test_column_tiles_are_free_of_bank_conflicts(M, TC, CS):
+    """The column pass's tile: column c at slots [c CS, c CS + M), element e at rslot(e), XORed with c (16-column
+    tiles) or 4 c (8-column tiles) at 384 points.
+    The block phases (thread t -> column t mod TC, rows t / TC + (threads / TC) r) and, at 384 points, the exchanges
+    of the four columns a wavefront holds fall on distinct banks."""
+    V, radices, _ = PLANS[M]
+    T = M // V
+    threads = TC * T
+
+    def elem(e, c):
+        return rslot(e) ^ ((4 * c if TC == 8 else c) & 31) if M == 384 else rslot(e)
+
+    assert all(sorted(elem(e, c) for e in range(M)) == list(range(M)) for c in range(TC))
+    step = threads // TC
+    for r in range(M // step):
+        for w in range(threads // 64):
+            slots = [(t % TC) * CS + elem(t // TC + step * r, t % TC) for t in range(64 * w, 64 * w + 64)]
+            assert worst_conflict(slots) == 1, (r, w)
+    if M == 384:
+        L = 1
+        for s, R in enumerate(radices):
+            NB, STRIDE = V // R, M // R
+            for w in range(threads // 64):
+                for q in range(NB):
+                    for j in range(R):
+                        reads, writes = [], []
+                        for lane in range(64):
+                            c, i = w * (64 // T) + lane // T, lane % T
+                            bi = i + T * q
+                            k = 0 if L == 1 else bi % L
+                            reads.append(c * CS + elem(bi + STRIDE * j, c))
+                            writes.append(c * CS + elem((bi - k) * R + k + j * L, c))
+                        if s > 0:
+                            assert worst_conflict(reads) == 1
+                        if s < len(radices) - 1:
+                            assert worst_conflict(writes) == 1
+            L *= R
