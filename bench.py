@@ -84,9 +84,14 @@ def main():
     ap.add_argument("--cpu-runs", type=int, default=5, help="timed runs of the CPU baseline (after one warm-up)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(8, cores), the reference's default")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="haadf", choices=["haadf", "frozen_phonons"],
-                    help="haadf (the headline scan) or frozen_phonons (16 configurations of uneven slice counts, "
-                         "pixelated 128 x 128 float32 crop + centre of mass)")
+    ap.add_argument("--workload", default="haadf", choices=["haadf", "frozen_phonons", "precision_study"],
+                    help="haadf (the headline scan), frozen_phonons (16 configurations of uneven slice counts, "
+                         "pixelated 128 x 128 float32 crop + centre of mass) or precision_study (the HAADF geometry "
+                         "at grid --grid, 4096 positions in one launch)")
+    ap.add_argument("--grid", type=int, default=384, help="precision_study: grid points per axis (256, 384, 509, "
+                    "512, 1024)")
+    ap.add_argument("--precision", default="float32", choices=["float32", "split_bf16"],
+                    help="precision_study: transforms in float32 or split-bfloat16 on the matrix cores")
     ap.add_argument("--fp-scan", type=int, default=128, help="frozen_phonons: scan points per axis")
     ap.add_argument("--pipeline", type=int, default=-1, help="frozen_phonons: 1 lane pipeline, 0 one potential at a time")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -118,6 +123,9 @@ def main():
         args.batch = 1024 if args.workload == "haadf" else 0
     if args.workload == "frozen_phonons":
         print(json.dumps(frozen_phonon_bench(args, sk, torch)), flush=True)
+        return
+    if args.workload == "precision_study":
+        print(json.dumps(precision_study_bench(args, sk, torch)), flush=True)
         return
 
     w = HaadfScan(batch=args.batch)
@@ -306,6 +314,54 @@ def frozen_phonon_bench(args, sk, torch):
                        "streams": args.streams or "library",
                        "detectors": "pixelated 128x128 float32 + centre of mass"},
             "prepare_ms": prepare_ms, "scan_pos_slices": work}
+
+
+def precision_study_bench(args, sk, torch):
+    """The HAADF geometry at one grid length: SrTiO3 8 x 8 cells (40 slices of 2 A; 20 at 509, whose Bluestein
+    transforms cost 5x), 200 keV, 20 mrad, a 64 x 64 scan over one cell, all 4096 positions in ONE launch, annular
+    50 mrad to 200 mrad (to 100 and 150 mrad at 256 and 384 points, where 200 mrad exceeds the largest simulated
+    angle). One step = one scan of the 4096 positions, potential prepared beforehand; the line also carries the
+    device time of one row and one column launch (averaged over the slices of a timed scan)."""
+    from simulatokit_amd.workloads import SRTIO3_A, srtio3
+
+    n = args.grid
+    st = srtio3(8, 8, 10 if n == 509 else 20)
+    pot = sk.Potential(st, n, n, slice_thickness=2.0)
+    probe = sk.Probe(sk.ElectronBeam(200e3), pot.grid, sk.Aperture(20.0))
+    prepared = pot.prepare(probe.beam)
+    scan = sk.GridScan((0.0, 0.0), (SRTIO3_A, SRTIO3_A), 64, 64)
+    outer = {256: 100.0, 384: 150.0}.get(n, 200.0)
+    det = sk.AnnularDetector(50.0, outer)
+    npos = 64 * 64
+    out = torch.zeros(npos, dtype=torch.float64, device="cuda:0")
+    opt = sk.ExecutionOptions(batch=npos, tile=npos, col_tile=args.col_tile, precision=args.precision)
+
+    def step():
+        sk.STEM.scan(probe, [prepared], scan, [det], opt, device_outputs=[out.data_ptr()])
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    sk.set_timing(True)
+    step()
+    sk.set_timing(False)
+    tm = sk.last_timing()
+    S = prepared.slice_count
+    prepared.release()
+    return {"metric": "probe-positions*slices/sec (HAADF scan, precision-study geometry)",
+            "value": npos * S * args.steps / elapsed, "unit": "probe-positions*slices/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": max(1, args.warmup), "ms_per_step": elapsed / args.steps * 1e3,
+            "higher_is_better": True, "dtype": "c64" if args.precision == "float32" else "split-bf16/f32",
+            "data": "synthetic",
+            "config": {"workload": f"precision_study_srtio3_{n}sq_64x64", "grid": n, "slices": S, "positions": npos,
+                       "batch": npos, "precision": args.precision, "col_tile": args.col_tile or "library",
+                       "detector": f"annular 50-{outer:.0f} mrad"},
+            "row_ms": tm["row_ms"], "col_ms": tm["col_ms"]}
 
 
 def copy_bandwidth(torch, nbytes: int = 1 << 30, reps: int = 5) -> float:

@@ -19,6 +19,31 @@ extern "C" {
 int sk_conv5_forward(const float* x, int batch, int cin, int h, int w, const uint16_t* wt, const float* bias, int cout,
                      int relu, float* y, void* stream);
 
+/* The weights in the tap-major layout of the matrix-core convolution, out[tap][32][kp] with kp = 16 when the
+ * convolution it serves has at most 16 input channels, else 32: out[tap][co][ci] = wt[co][ci][tap] (flip = 0; kp from
+ * cin), or, for the backward pass over the data, the flipped transposed kernel out[tap][ci][co] = wt[co][ci][24 - tap]
+ * (flip = 1; kp from cout); the rows and columns past the real channels are zero. cin, cout <= 32; out holds
+ * 25 * 32 * kp bfloat16 values (at most 25,600). */
+int sk_conv5_weights_tap_major(const uint16_t* wt, int cin, int cout, int flip, uint16_t* out, void* stream);
+
+/* y = bias + conv5(x, w) on the matrix cores (v_mfma_f32_32x32x16_bf16: the bfloat16 weights exact, each activation
+ * as a bfloat16 hi + lo pair, float32 accumulation), ReLU applied when relu != 0; wtm = the tap-major weights
+ * (sk_conv5_weights_tap_major with flip = 0; with flip = 1 and dy as x, cin and cout swapped, this is the backward
+ * pass over the data). cin, cout <= 32, w a multiple of 32, h a multiple of 16. */
+int sk_conv5_forward_tap_major(const float* x, int batch, int cin, int h, int w, const uint16_t* wtm,
+                               const float* bias, int cout, int relu, float* y, void* stream);
+
+/* The floats of the workspace sk_conv5_backward_weights_mfma takes: 512 blocks' partial sums of 25 x 32 x 32, and
+ * 32 x 64 float64 partial sums of the bias gradient */
+#define SK_CONV5_MFMA_WORKSPACE_FLOATS (512 * 25 * 32 * 32 + 2 * 32 * 64)
+
+/* dw [cout][cin][5][5] and db [cout] (db may be NULL) as sk_conv5_backward_weights computes them, on the matrix
+ * cores: both operands as bfloat16 hi + lo pairs, three products (a relative 2^-17 per product against float32's
+ * 2^-24), float32 sums per block of pixels, the blocks' sums added in a fixed order in float64. cin, cout <= 32, w a
+ * multiple of 32, h a multiple of 16; workspace holds SK_CONV5_MFMA_WORKSPACE_FLOATS floats. */
+int sk_conv5_backward_weights_mfma(const float* x, const float* dy, int batch, int cin, int h, int w, int cout,
+                                   float* dw, float* db, float* workspace, void* stream);
+
 /* dx = the transpose of conv5 applied to dy; wt_scratch holds cin * cout * 25 bfloat16 values */
 int sk_conv5_backward_data(const float* dy, int batch, int cin, int h, int w, const uint16_t* wt, int cout, float* dx,
                            uint16_t* wt_scratch, void* stream);

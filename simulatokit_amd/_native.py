@@ -143,4 +143,5 @@ EXPORTED = ("sk_hip_available", "sk_hip_last_error", "sk_hip_build_id", "sk_hip_
             "sk_hip_ctf_kernel", "sk_hip_apply_ctf", "sk_hip_scan_noalloc",
             # include/simulato_surrogate.h
             "sk_conv5_forward", "sk_conv5_backward_data", "sk_conv5_backward_weights", "sk_relu_backward",
-            "sk_separable_map", "sk_mse_grad", "sk_weighted_mse_grad", "sk_adam_step", "sk_to_bf16")
+            "sk_separable_map", "sk_mse_grad", "sk_weighted_mse_grad", "sk_adam_step", "sk_to_bf16",
+            "sk_conv5_weights_tap_major", "sk_conv5_forward_tap_major", "sk_conv5_backward_weights_mfma")

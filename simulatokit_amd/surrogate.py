@@ -35,6 +35,7 @@ from .probe import Aperture, GridScan, Probe
 from .workloads import random_cell
 
 CHANNELS = [2, 32, 32, 32, 32, 32, 1]
+WGRAD_WORKSPACE_FLOATS = 512 * 25 * 32 * 32 + 2 * 32 * 64  # SK_CONV5_MFMA_WORKSPACE_FLOATS
 INPUT_N = 128
 SCAN_N = 96
 ELEMENTS = (6, 8, 14, 22, 38, 79)
@@ -167,6 +168,9 @@ def _bind():
     L.sk_weighted_mse_grad.argtypes = [vp, vp, vp, ll, ll, vp, vp, vp]
     L.sk_adam_step.argtypes = [vp, vp, vp, vp, vp, ll, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, vp]
     L.sk_to_bf16.argtypes = [vp, vp, ll, vp]
+    L.sk_conv5_weights_tap_major.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp]
+    L.sk_conv5_forward_tap_major.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, C.c_int, C.c_int, vp, vp]
+    L.sk_conv5_backward_weights_mfma.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, vp]
     L._surrogate_bound = True
     return L
 
@@ -200,10 +204,17 @@ def resample_matrix(m: int, n: int) -> np.ndarray:
 
 
 class HaadfSurrogate:
-    """The network on the device. Weights: float32 masters (trained) and their bfloat16 rounding (used)."""
+    """The network on the device. Weights: float32 masters (trained) and their bfloat16 rounding (used).
+
+    With matrix_cores true, an input size that is a multiple of 32 and at most 32 channels a layer, every layer runs
+    on the matrix cores: the forward pass and the backward pass over the data through sk_conv5_forward_tap_major (from
+    tap-major copies of the bfloat16 weights, refreshed after every update), the gradient of the weights through
+    sk_conv5_backward_weights_mfma. Otherwise, or with matrix_cores=False, on the vector units (sk_conv5_forward,
+    sk_conv5_backward_data, sk_conv5_backward_weights)."""
 
     def __init__(self, seed: int = 0, channels: Sequence[int] = CHANNELS, input_n: int = INPUT_N,
-                 output_n: int = SCAN_N, resample: str = "fourier", depth_slabs: int = 0, by_element: bool = False):
+                 output_n: int = SCAN_N, resample: str = "fourier", depth_slabs: int = 0, by_element: bool = False,
+                 matrix_cores: bool = True):
         import torch
 
         self.torch = torch
@@ -229,6 +240,11 @@ class HaadfSurrogate:
         self.A = torch.tensor(A, dtype=torch.float32, device=self.dev)
         self.At = self.A.t().contiguous()
         self.norm = {"v": 1.0, "t": 100.0, "y": 1.0}
+        # tap-major bfloat16 copies of every layer: [forward, flipped for the backward pass over the data]
+        self.tap_major, self._wgrad_ws = {}, None
+        if matrix_cores and input_n % 32 == 0 and max(self.channels) <= 32:
+            for k in range(len(self.w32)):
+                self.tap_major[k] = [torch.empty(25 * 32 * 32, dtype=torch.int16, device=self.dev) for _ in range(2)]
         self._publish()
 
     # -- plumbing
@@ -240,6 +256,15 @@ class HaadfSurrogate:
         L = _bind()
         for w, wb in zip(self.w32, self.wbf):
             _check(L.sk_to_bf16(w.data_ptr(), wb.data_ptr(), w.numel(), self._stream), "to_bf16")
+        self._refresh_tap_major()
+
+    def _refresh_tap_major(self):
+        L = _bind()
+        for k, (fwd, bwd) in self.tap_major.items():
+            ci, co = self.channels[k], self.channels[k + 1]
+            for flip, out in ((0, fwd), (1, bwd)):
+                _check(L.sk_conv5_weights_tap_major(self.wbf[k].data_ptr(), ci, co, flip, out.data_ptr(),
+                                                    self._stream), "conv5_weights_tap_major")
 
     def parameters(self) -> List:
         return [t for pair in zip(self.w32, self.b) for t in pair]
@@ -264,9 +289,15 @@ class HaadfSurrogate:
         for k in range(len(self.w32)):
             ci, co = self.channels[k], self.channels[k + 1]
             y = torch.empty((B, co, self.input_n, self.input_n), dtype=torch.float32, device=self.dev)
-            _check(L.sk_conv5_forward(acts[-1].data_ptr(), B, ci, self.input_n, self.input_n,
-                                      self.wbf[k].data_ptr(), self.b[k].data_ptr(), co,
-                                      1 if k < len(self.w32) - 1 else 0, y.data_ptr(), self._stream), "conv5_forward")
+            relu = 1 if k < len(self.w32) - 1 else 0
+            if k in self.tap_major:
+                _check(L.sk_conv5_forward_tap_major(acts[-1].data_ptr(), B, ci, self.input_n, self.input_n,
+                                                    self.tap_major[k][0].data_ptr(), self.b[k].data_ptr(), co, relu,
+                                                    y.data_ptr(), self._stream), "conv5_forward_tap_major")
+            else:
+                _check(L.sk_conv5_forward(acts[-1].data_ptr(), B, ci, self.input_n, self.input_n,
+                                          self.wbf[k].data_ptr(), self.b[k].data_ptr(), co, relu, y.data_ptr(),
+                                          self._stream), "conv5_forward")
             acts.append(y)
         tmp = torch.empty((B, self.input_n, self.output_n), dtype=torch.float32, device=self.dev)
         out = torch.empty((B, self.output_n, self.output_n), dtype=torch.float32, device=self.dev)
@@ -289,15 +320,29 @@ class HaadfSurrogate:
                 _check(L.sk_relu_backward(g.data_ptr(), acts[k + 1].data_ptr(), g.numel(), self._stream), "relu")
             dw = torch.empty((co, ci, 5, 5), dtype=torch.float32, device=self.dev)
             db = torch.empty(co, dtype=torch.float32, device=self.dev)
-            _check(L.sk_conv5_backward_weights(acts[k].data_ptr(), g.data_ptr(), B, ci, self.input_n, self.input_n,
-                                               co, dw.data_ptr(), db.data_ptr(), self._stream), "backward_weights")
+            if k in self.tap_major:
+                if self._wgrad_ws is None:
+                    self._wgrad_ws = torch.empty(WGRAD_WORKSPACE_FLOATS, dtype=torch.float32, device=self.dev)
+                _check(L.sk_conv5_backward_weights_mfma(acts[k].data_ptr(), g.data_ptr(), B, ci, self.input_n,
+                                                        self.input_n, co, dw.data_ptr(), db.data_ptr(),
+                                                        self._wgrad_ws.data_ptr(), self._stream),
+                       "backward_weights_mfma")
+            else:
+                _check(L.sk_conv5_backward_weights(acts[k].data_ptr(), g.data_ptr(), B, ci, self.input_n,
+                                                   self.input_n, co, dw.data_ptr(), db.data_ptr(), self._stream),
+                       "backward_weights")
             grads[k] = (dw, db)
             if k > 0:
                 gx = torch.empty((B, ci, self.input_n, self.input_n), dtype=torch.float32, device=self.dev)
-                scratch = torch.empty(co * ci * 25, dtype=torch.int16, device=self.dev)
-                _check(L.sk_conv5_backward_data(g.data_ptr(), B, ci, self.input_n, self.input_n,
-                                                self.wbf[k].data_ptr(), co, gx.data_ptr(), scratch.data_ptr(),
-                                                self._stream), "backward_data")
+                if k in self.tap_major:  # the forward convolution of dy with the flipped, transposed kernel
+                    _check(L.sk_conv5_forward_tap_major(g.data_ptr(), B, co, self.input_n, self.input_n,
+                                                        self.tap_major[k][1].data_ptr(), None, ci, 0, gx.data_ptr(),
+                                                        self._stream), "conv5_forward_tap_major")
+                else:
+                    scratch = torch.empty(co * ci * 25, dtype=torch.int16, device=self.dev)
+                    _check(L.sk_conv5_backward_data(g.data_ptr(), B, ci, self.input_n, self.input_n,
+                                                    self.wbf[k].data_ptr(), co, gx.data_ptr(), scratch.data_ptr(),
+                                                    self._stream), "backward_data")
                 g = gx
         return grads
 
@@ -367,6 +412,7 @@ class HaadfSurrogate:
                         _check(L.sk_adam_step(p.data_ptr(), gp.data_ptr(), mm.data_ptr(), vv.data_ptr(),
                                               pub.data_ptr() if pub is not None else None, p.numel(), rate, 0.9,
                                               0.999, 1e-8, step, self._stream), "adam")
+                self._refresh_tap_major()
                 total += float(partial.sum().item())
                 count += nb
             history.append(total / max(1, count))

@@ -96,6 +96,122 @@ def test_network_gradients_equal_the_oracle():
         assert rel(db.cpu().numpy(), odb) <= 1e-4, k
 
 
+def test_matrix_core_convolution_equals_the_oracle():
+    # the 32 -> 32 layers on v_mfma_f32_32x32x16_bf16: two 8-row and two 32-column tiles per image, so that the
+    # circular halo crosses tile edges both ways; forward with bias and ReLU, and the backward pass over the data as
+    # the forward convolution of dy with the flipped tap-major weights
+    import torch
+
+    L = SG._bind()
+    rng = np.random.default_rng(15)
+    B, c, h, w = 2, 32, 16, 64
+    x = rng.normal(size=(B, c, h, w)).astype(np.float32)
+    wt = S.bf16_round(rng.normal(size=(c, c, 5, 5)) * 0.05)
+    bias = rng.normal(size=c).astype(np.float32)
+    dev = torch.device("cuda:0")
+    W = torch.tensor(bf16_bits(wt).astype(np.int16), device=dev)
+    fwd, bwd = (torch.empty(c * c * 25, dtype=torch.int16, device=dev) for _ in range(2))
+    assert L.sk_conv5_weights_tap_major(W.data_ptr(), c, c, 0, fwd.data_ptr(), None) == 0
+    assert L.sk_conv5_weights_tap_major(W.data_ptr(), c, c, 1, bwd.data_ptr(), None) == 0
+    X, Bv = torch.tensor(x, device=dev), torch.tensor(bias, device=dev)
+    Y = torch.empty((B, c, h, w), device=dev)
+    assert L.sk_conv5_forward_tap_major(X.data_ptr(), B, c, h, w, fwd.data_ptr(), Bv.data_ptr(), c, 1, Y.data_ptr(),
+                                        None) == 0
+    assert rel(Y.cpu().numpy(), S.conv5_forward(x, wt, bias, relu=True)) <= 1e-5
+    dy = rng.normal(size=(B, c, h, w)).astype(np.float32)
+    DX = torch.empty((B, c, h, w), device=dev)
+    assert L.sk_conv5_forward_tap_major(torch.tensor(dy, device=dev).data_ptr(), B, c, h, w, bwd.data_ptr(), None, c,
+                                        0, DX.data_ptr(), None) == 0
+    assert rel(DX.cpu().numpy(), S.conv5_backward_data(dy, wt)) <= 1e-5
+    # the gradient of the weights: both operands split, three products (3.6e-6 of the largest value in float64 for
+    # these inputs), float32 sums per block
+    ws = torch.empty(SG.WGRAD_WORKSPACE_FLOATS, dtype=torch.float32, device=dev)
+    DW, DB = torch.empty((c, c, 5, 5), device=dev), torch.empty(c, device=dev)
+    assert L.sk_conv5_backward_weights_mfma(X.data_ptr(), torch.tensor(dy, device=dev).data_ptr(), B, c, h, w, c,
+                                            DW.data_ptr(), DB.data_ptr(), ws.data_ptr(), None) == 0
+    dw, db = S.conv5_backward_weights(x, dy)
+    assert rel(DW.cpu().numpy(), dw) <= 1e-5
+    assert rel(DB.cpu().numpy(), db) <= 1e-6
+    # shapes outside the kernels' tiling are refused, not computed
+    assert L.sk_conv5_forward_tap_major(X.data_ptr(), B, c, h, 48, fwd.data_ptr(), None, c, 0, Y.data_ptr(),
+                                        None) != 0
+    assert L.sk_conv5_forward_tap_major(X.data_ptr(), B, 33, h, w, fwd.data_ptr(), None, c, 0, Y.data_ptr(),
+                                        None) != 0
+    assert L.sk_conv5_backward_weights_mfma(X.data_ptr(), X.data_ptr(), B, c, h, 48, c, DW.data_ptr(), None,
+                                            ws.data_ptr(), None) != 0
+
+
+def test_matrix_core_layers_with_fewer_channels_equal_the_oracle():
+    # the first layer (2 -> 32), the last (32 -> 1) and its backward pass over the data (1 -> 32): channels padded
+    # with zeros to 16 or 32 inside the kernels, only the real ones stored
+    import torch
+
+    L = SG._bind()
+    rng = np.random.default_rng(18)
+    dev = torch.device("cuda:0")
+    B, h, w = 3, 16, 32
+    ws = torch.empty(SG.WGRAD_WORKSPACE_FLOATS, dtype=torch.float32, device=dev)
+    for ci, co in ((2, 32), (32, 1), (5, 17)):
+        x = rng.normal(size=(B, ci, h, w)).astype(np.float32)
+        wt = S.bf16_round(rng.normal(size=(co, ci, 5, 5)) * 0.1)
+        bias = rng.normal(size=co).astype(np.float32)
+        W = torch.tensor(bf16_bits(wt).astype(np.int16), device=dev)
+        fwd, bwd = (torch.empty(25 * 32 * 32, dtype=torch.int16, device=dev) for _ in range(2))
+        assert L.sk_conv5_weights_tap_major(W.data_ptr(), ci, co, 0, fwd.data_ptr(), None) == 0
+        assert L.sk_conv5_weights_tap_major(W.data_ptr(), ci, co, 1, bwd.data_ptr(), None) == 0
+        X = torch.tensor(x, device=dev)
+        Y = torch.empty((B, co, h, w), device=dev)
+        assert L.sk_conv5_forward_tap_major(X.data_ptr(), B, ci, h, w, fwd.data_ptr(),
+                                            torch.tensor(bias, device=dev).data_ptr(), co, 1, Y.data_ptr(), None) == 0
+        assert rel(Y.cpu().numpy(), S.conv5_forward(x, wt, bias, relu=True)) <= 1e-5, (ci, co)
+        dy = rng.normal(size=(B, co, h, w)).astype(np.float32)
+        DY = torch.tensor(dy, device=dev)
+        DX = torch.empty((B, ci, h, w), device=dev)
+        assert L.sk_conv5_forward_tap_major(DY.data_ptr(), B, co, h, w, bwd.data_ptr(), None, ci, 0, DX.data_ptr(),
+                                            None) == 0
+        assert rel(DX.cpu().numpy(), S.conv5_backward_data(dy, wt)) <= 1e-5, (ci, co)
+        DW, DB = torch.empty((co, ci, 5, 5), device=dev), torch.empty(co, device=dev)
+        assert L.sk_conv5_backward_weights_mfma(X.data_ptr(), DY.data_ptr(), B, ci, h, w, co, DW.data_ptr(),
+                                                DB.data_ptr(), ws.data_ptr(), None) == 0
+        dw, db = S.conv5_backward_weights(x, dy)
+        assert rel(DW.cpu().numpy(), dw) <= 1e-5, (ci, co)
+        assert rel(DB.cpu().numpy(), db) <= 1e-6, (ci, co)
+
+
+def test_network_gradients_on_the_matrix_cores_equal_the_oracle():
+    import torch
+
+    model = SG.HaadfSurrogate(seed=4, channels=[2, 32, 32, 32, 1], input_n=32, output_n=24)
+    assert sorted(model.tap_major) == [0, 1, 2, 3]  # every layer takes the matrix-core path
+    rng = np.random.default_rng(16)
+    x = rng.normal(size=(2, 2, 32, 32)).astype(np.float32)
+    for k in range(4):
+        model.b[k].copy_(torch.tensor(rng.normal(size=model.b[k].numel()).astype(np.float32) * 0.1))
+    out, acts = model.forward(torch.tensor(x, device=model.dev), keep=True)
+    dout = rng.normal(size=tuple(out.shape)).astype(np.float32)
+    grads = model.backward(acts, torch.tensor(dout, device=model.dev))
+    layers = [(S.bf16_round(w.cpu().numpy()), b.cpu().numpy()) for w, b in zip(model.w32, model.b)]
+    oacts = S.network_forward(x, layers)
+    A = S.fourier_resample_matrix(24, 32)
+    assert rel(out.cpu().numpy(), S.separable_map(oacts[-1][:, 0], A)) <= 1e-5
+    ograds = S.network_backward(oacts, layers, S.separable_map(dout, A.T)[:, None])
+    for k, ((dw, db), (odw, odb)) in enumerate(zip(grads, ograds)):
+        assert rel(dw.cpu().numpy(), odw) <= 1e-4, k
+        assert rel(db.cpu().numpy(), odb) <= 1e-4, k
+    # an update refreshes the tap-major copies: the vector-unit network with the same weights agrees
+    model.w32[1].mul_(1.5)
+    model._publish()
+    plain = SG.HaadfSurrogate(seed=4, channels=[2, 32, 32, 32, 1], input_n=32, output_n=24, matrix_cores=False)
+    assert not plain.tap_major
+    for k in range(4):
+        plain.w32[k].copy_(model.w32[k])
+        plain.b[k].copy_(model.b[k])
+    plain._publish()
+    xs = torch.tensor(x, device=model.dev)
+    a, b = model.forward(xs).cpu().numpy(), plain.forward(xs).cpu().numpy()
+    assert rel(a, b.astype(np.float64)) <= 1e-5
+
+
 def test_judged_surrogate_scan_on_a_small_workload():
     # a reduced workload so that data generation and training take seconds: 256^2 waves (annular 50-100 mrad fits),
     # 48 x 48 scan, 64^2 input, thin cells; 24 training cells
