@@ -71,23 +71,22 @@ __device__ inline void dft8_z34(float2 (&u)[8], int sign) {
   u[7] = csub(e3, w3);
 }
 
-// One pixel of the final pass: intensity I at (ix, iy) into the scalar slots and the pixelated accumulators
-// (AnnularDetector, CenterOfMassDetector, PixelatedDetector; Detectors.swift:119-205).
-__device__ inline void detect_pixel(const DetPlan& d, int ix, int iy, double I, long long pos,
-                                    double (&s)[kMaxSlots]) {
-  // annular membership: a2 = ax^2 + ay^2 in float64 exactly as the host rounds it (no contraction)
-  const double a2 = d.sqx ? __dadd_rn(d.sqx[ix], d.sqy[iy]) : 0.0;
-  for (int q = 0; q < d.nslots; ++q) {
-    const int kind = d.slot_kind[q], arg = d.slot_arg[q];
-    if (kind == 0) {
-      if (a2 >= d.slot_i2[q] && a2 < d.slot_o2[q]) s[q] += I;
-    } else if (kind == 3) {
-      if (d.seg_map[arg][(long long)ix * d.ny + iy] == d.slot_bin[q]) s[q] += I;
-    } else {
-      const int mx = d.com_mapx[arg][ix], my = d.com_mapy[arg][iy];
-      if (mx >= 0 && my >= 0) s[q] += I * (kind == 1 ? d.com_wx[arg][ix] : d.com_wy[arg][iy]);
-    }
-  }
+// The final pass's detection (AnnularDetector, CenterOfMassDetector, SegmentedDetector, PixelatedDetector;
+// Detectors.swift:119-205), one scalar slot at a time so that a thread's running sum is one register: the
+// contribution of intensity I at pixel (ix, iy) to slot q, with a2 = ax^2 + ay^2 from annular_a2.
+__device__ inline double annular_a2(const DetPlan& d, int ix, int iy) {
+  // float64 exactly as the host rounds it (no contraction)
+  return d.sqx ? __dadd_rn(d.sqx[ix], d.sqy[iy]) : 0.0;
+}
+__device__ inline double slot_value(const DetPlan& d, int q, int ix, int iy, double a2, double I) {
+  const int kind = d.slot_kind[q], arg = d.slot_arg[q];
+  if (kind == 0) return (a2 >= d.slot_i2[q] && a2 < d.slot_o2[q]) ? I : 0.0;
+  if (kind == 3) return d.seg_map[arg][(long long)ix * d.ny + iy] == d.slot_bin[q] ? I : 0.0;
+  const int mx = d.com_mapx[arg][ix], my = d.com_mapy[arg][iy];
+  return (mx >= 0 && my >= 0) ? I * (kind == 1 ? d.com_wx[arg][ix] : d.com_wy[arg][iy]) : 0.0;
+}
+// The pixelated accumulators of pixel (ix, iy) of position pos
+__device__ inline void detect_pixelated(const DetPlan& d, int ix, int iy, double I, long long pos) {
   for (int p = 0; p < d.npix; ++p) {
     const int ox = d.pix_mapx[p][ix], oy = d.pix_mapy[p][iy];
     if (ox >= 0 && oy >= 0) d.pix_acc[p][pos * d.pix_stride[p] + (long long)ox * d.pix_my[p] + oy] += I;

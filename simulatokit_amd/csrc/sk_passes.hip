@@ -233,7 +233,8 @@ __device__ float2* lds_stockham_fixed(float2* a, float2* b, const float2* roots,
 // the FL lines fixed at compile time); MF = n2 > 0, the split-BF16 matrix-core stages of length p.m = 16 n2 (the
 // plan carries their tables). Returns the buffer holding the result.
 template <int MF, int FM, int FL>
-__device__ float2* lds_length_m(float2* a, float2* b, int lines, int ls, const LineFFT& p, int sign, int tid, int T) {
+__device__ __forceinline__ float2* lds_length_m(float2* a, float2* b, int lines, int ls, const LineFFT& p, int sign,
+                                                int tid, int T) {
   if constexpr (MF == 0 && FM > 0) {
     return lds_stockham_fixed<FM, FL>(a, b, p.roots, sign, tid);
   } else if constexpr (MF == 0) {
@@ -248,8 +249,8 @@ __device__ float2* lds_length_m(float2* a, float2* b, int lines, int ls, const L
 // Bluestein (1970): X_k = c_k sum_j (x_j c_j) conj(c_{k-j}), c_k = e^{sign pi i k^2 / n}, the convolution done with
 // two power-of-two transforms of length m.
 template <int MF, int FM, int FL>
-__device__ float2* lds_line_transform(float2* a, float2* b, int lines, int ls, const LineFFT& p, int sign, int tid,
-                                      int T) {
+__device__ __forceinline__ float2* lds_line_transform(float2* a, float2* b, int lines, int ls, const LineFFT& p,
+                                                      int sign, int tid, int T) {
   if (!p.blue) return lds_length_m<MF, FM, FL>(a, b, lines, ls, p, sign, tid, T);
   if constexpr (MF == 0 && FM > 0) {
     // fixed-length Bluestein: the chirp product fused into the forward transform's first reads (zero past n), the
@@ -402,19 +403,16 @@ __global__ void __launch_bounds__(kGenericThreads) row_generic_kernel(RowArgs a,
 // ---------------------------------------------------------------------------------------------------------------
 // Detector reduction of one block's pixels (final column pass). `pix(idx)` yields (ix, iy, intensity) or iy < 0.
 // ---------------------------------------------------------------------------------------------------------------
-__device__ inline void block_reduce_slots(double (&s)[kMaxSlots], int nslots, double* scratch, int tid, int T,
-                                          double* out, int out_stride) {
-  // deterministic tree reduction per slot; `scratch` holds T doubles
-  for (int q = 0; q < nslots; ++q) {
-    scratch[tid] = s[q];
-    __syncthreads();
-    for (int w = T / 2; w > 0; w >>= 1) {
-      if (tid < w) scratch[tid] += scratch[tid + w];
-      __syncthreads();
-    }
-    if (tid == 0) out[q * out_stride] = scratch[0];
+__device__ inline void block_reduce(double s, double* scratch, int tid, int T, double* out) {
+  // deterministic tree reduction; `scratch` holds T doubles
+  scratch[tid] = s;
+  __syncthreads();
+  for (int w = T / 2; w > 0; w >>= 1) {
+    if (tid < w) scratch[tid] += scratch[tid + w];
     __syncthreads();
   }
+  if (tid == 0) *out = scratch[0];
+  __syncthreads();
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -464,25 +462,36 @@ __global__ void __launch_bounds__(kGenericThreads) col_generic_kernel(ColArgs a,
     __syncthreads();
   }
   if (f & COL_DETECT) {
-    double s[kMaxSlots];
-#pragma unroll
-    for (int q = 0; q < kMaxSlots; ++q) s[q] = 0.0;
+    // one slot at a time: each thread sums its pixels in index order, then the block's tree; the pixelated
+    // accumulators after the slots
     const long long pos = first_position(a.pos0, a.pos_dev) + b;
-    for (int idx = tid; idx < cols * a.nx; idx += T) {
-      int l;
-      const int k = dcols.quot(idx, l), c = c0 + l;
-      if (c >= a.nc) continue;
-      const int iy = a.keep[c];
-      if (iy < 0) continue;
-      const float2 v = cur[at<MF>(l * ls + k)];
-      const double I = ((double)v.x * v.x + (double)v.y * v.y) * a.det.scale;
-      detect_pixel(a.det, k, iy, I, pos, s);
-    }
-    __syncthreads();
     double* scratch = reinterpret_cast<double*>(oth);
-    block_reduce_slots(s, a.det.nslots, scratch, tid, T,
-                       a.det.partial + (b * a.det.nslots) * a.det.blocks_per_wave + blockIdx.x,
-                       a.det.blocks_per_wave);
+    double* out = a.det.partial + (b * a.det.nslots) * a.det.blocks_per_wave + blockIdx.x;
+    for (int q = 0; q < a.det.nslots; ++q) {
+      double s = 0.0;
+      for (int idx = tid; idx < cols * a.nx; idx += T) {
+        int l;
+        const int k = dcols.quot(idx, l), c = c0 + l;
+        if (c >= a.nc) continue;
+        const int iy = a.keep[c];
+        if (iy < 0) continue;
+        const float2 v = cur[at<MF>(l * ls + k)];
+        const double I = ((double)v.x * v.x + (double)v.y * v.y) * a.det.scale;
+        s += slot_value(a.det, q, k, iy, annular_a2(a.det, k, iy), I);
+      }
+      block_reduce(s, scratch, tid, T, out + (long long)q * a.det.blocks_per_wave);
+    }
+    if (a.det.npix > 0) {
+      for (int idx = tid; idx < cols * a.nx; idx += T) {
+        int l;
+        const int k = dcols.quot(idx, l), c = c0 + l;
+        if (c >= a.nc) continue;
+        const int iy = a.keep[c];
+        if (iy < 0) continue;
+        const float2 v = cur[at<MF>(l * ls + k)];
+        detect_pixelated(a.det, k, iy, ((double)v.x * v.x + (double)v.y * v.y) * a.det.scale, pos);
+      }
+    }
     return;
   }
   if (f & COL_INV) cur = lds_line_transform<MF, FM, fixed_col_lines(FM)>(cur, oth, cols, ls, a.fx, +1, tid, T);
