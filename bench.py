@@ -84,10 +84,14 @@ def main():
     ap.add_argument("--cpu-runs", type=int, default=5, help="timed runs of the CPU baseline (after one warm-up)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(8, cores), the reference's default")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="haadf", choices=["haadf", "frozen_phonons", "precision_study"],
+    ap.add_argument("--workload", default="haadf",
+                    choices=["haadf", "frozen_phonons", "precision_study", "surrogate_training"],
                     help="haadf (the headline scan), frozen_phonons (16 configurations of uneven slice counts, "
-                         "pixelated 128 x 128 float32 crop + centre of mass) or precision_study (the HAADF geometry "
-                         "at grid --grid, 4096 positions in one launch)")
+                         "pixelated 128 x 128 float32 crop + centre of mass), precision_study (the HAADF geometry "
+                         "at grid --grid, 4096 positions in one launch) or surrogate_training (Adam steps of the "
+                         "HAADF surrogate, batch 16)")
+    ap.add_argument("--vector-units", action="store_true",
+                    help="surrogate_training: the convolutions on the vector units instead of the matrix cores")
     ap.add_argument("--grid", type=int, default=384, help="precision_study: grid points per axis (256, 384, 509, "
                     "512, 1024)")
     ap.add_argument("--precision", default="float32", choices=["float32", "split_bf16"],
@@ -126,6 +130,9 @@ def main():
         return
     if args.workload == "precision_study":
         print(json.dumps(precision_study_bench(args, sk, torch)), flush=True)
+        return
+    if args.workload == "surrogate_training":
+        print(json.dumps(surrogate_training_bench(args, torch)), flush=True)
         return
 
     w = HaadfScan(batch=args.batch)
@@ -362,6 +369,36 @@ def precision_study_bench(args, sk, torch):
                        "batch": npos, "precision": args.precision, "col_tile": args.col_tile or "library",
                        "detector": f"annular 50-{outer:.0f} mrad"},
             "row_ms": tm["row_ms"], "col_ms": tm["col_ms"]}
+
+
+def surrogate_training_bench(args, torch):
+    """Training of the HAADF surrogate network as specified (six 5 x 5 layers, 2 -> 32 -> ... -> 32 -> 1 channels,
+    bfloat16 weights, 128 x 128 input, 96 x 96 output): one step = one Adam step on a batch of 16 (forward, relative
+    squared error, backward, update, with the 8-symmetry augmentation), through HaadfSurrogate.train. The inputs and
+    labels are seeded random arrays of the network's shapes (the step's arithmetic does not depend on their values);
+    64 cells, so an epoch is 4 steps; `--steps` epochs are timed after `--warmup`."""
+    import numpy as np
+
+    from simulatokit_amd import surrogate as SG
+
+    rng = np.random.default_rng(7)
+    cells = 64
+    data = SG.Dataset(list(range(cells)), rng.normal(size=(cells, 2, SG.INPUT_N, SG.INPUT_N)).astype(np.float32),
+                      np.abs(rng.normal(size=(cells, SG.SCAN_N, SG.SCAN_N))).astype(np.float32) * 0.01)
+    model = SG.HaadfSurrogate(seed=0, matrix_cores=not args.vector_units)
+    model.train(data, epochs=max(1, args.warmup))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    model.train(data, epochs=args.steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    steps = args.steps * (cells // 16)
+    return {"metric": "surrogate training steps/sec (batch 16)", "value": steps / elapsed, "unit": "steps/s",
+            "n_gpus": 1, "steps": steps, "warmup": max(1, args.warmup) * (cells // 16),
+            "ms_per_step": elapsed / steps * 1e3, "higher_is_better": True, "dtype": "bf16 weights, f32 activations",
+            "data": "synthetic",
+            "config": {"workload": "haadf_surrogate_training_6x32_128sq", "channels": SG.CHANNELS, "batch": 16,
+                       "convolutions": "vector units" if args.vector_units else "matrix cores"}}
 
 
 def copy_bandwidth(torch, nbytes: int = 1 << 30, reps: int = 5) -> float:

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -2021,13 +2022,14 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
     // each tile's positions split over the streams `st` in parts of H = ceil(T / k), stream q always on the
     // workspace positions [q H, q H + H): a stream runs its parts of consecutive tiles one after another, and no two
     // streams ever share workspace, whatever the tile's size.
+    // Tiles [lo, hi) only, when given (the remaining tiles are enqueued by a later call).
     auto enqueue_tiles = [&](const sk_potential& P, int b0, const int* pos_dev, int nb, Timer& t,
-                             const std::vector<hipStream_t>& st) {
+                             const std::vector<hipStream_t>& st, int lo = 0, int hi = 1 << 30) {
       const int tiles = (nb + T - 1) / T;
       const int Tp = (nb + tiles - 1) / tiles;
       const int k = (int)st.size();
       const int H = (T + k - 1) / k;
-      for (int i = 0; i < tiles; ++i) {
+      for (int i = lo; i < std::min(tiles, hi); ++i) {
         const int first = i * Tp, count = std::min(Tp, nb - first);
         if (count <= 0) break;
         for (int q = 0; q < k && q * H < count; ++q)
@@ -2035,6 +2037,35 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
                         q * H);
         if (k == 1) t.harvest(timing);
       }
+    };
+    // One batch through its slice loop launched kernel by kernel: on the scan's stream, or forked over the split
+    // streams and joined back into it. `between` runs on the host after the first half of the batch's tiles is
+    // enqueued and before the rest, while the GPU works on the first half.
+    auto run_batch_eager = [&](const sk_potential& P, int b0, int nb, const std::function<void()>& between) {
+      const int tiles = (nb + T - 1) / T, half = (tiles + 1) / 2;
+      if (split <= 1) {
+        const std::vector<hipStream_t> one(1, s);
+        enqueue_tiles(P, b0, nullptr, nb, tm, one, 0, half);
+        between();
+        enqueue_tiles(P, b0, nullptr, nb, tm, one, half);
+        return;
+      }
+      const std::vector<hipStream_t> st = part_streams(split);
+      hipEvent_t fork;
+      SK_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+      SK_HIP(hipEventRecord(fork, s));
+      for (int q = 1; q < split; ++q) SK_HIP(hipStreamWaitEvent(st[q], fork, 0));
+      enqueue_tiles(P, b0, nullptr, nb, tm, st, 0, half);
+      between();
+      enqueue_tiles(P, b0, nullptr, nb, tm, st, half);
+      std::vector<hipEvent_t> joins(split - 1);
+      for (int q = 1; q < split; ++q) {
+        SK_HIP(hipEventCreateWithFlags(&joins[q - 1], hipEventDisableTiming));
+        SK_HIP(hipEventRecord(joins[q - 1], st[q]));
+        SK_HIP(hipStreamWaitEvent(s, joins[q - 1], 0));
+      }
+      SK_HIP(hipEventDestroy(fork));
+      for (auto e : joins) SK_HIP(hipEventDestroy(e));
     };
     if (pipe) {
       run_pipeline(pots, npots, npos, W, L, g, dp, graph, tm, timing);
@@ -2061,13 +2092,16 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
       for (int k = 0; k < npots; ++k)
         for (int b0 = 0; b0 < npos; b0 += B) enqueue_tiles(*pots[k], b0, nullptr, std::min(B, npos - b0), tm, one);
     } else {
-      // hipGraph replay: per potential, one graph per batch shape (full, remainder), captured once and replayed for
-      // every batch; the first position of a replay is read from device memory, copied in from pinned host memory
-      // before each launch.
+      // hipGraph replay: per potential, one graph per batch shape (full, remainder), replayed for the batches of
+      // that shape after the first; the first position of a replay is read from device memory, copied in from
+      // pinned host memory before each launch. The first batch of a shape runs kernel by kernel, and while the GPU
+      // works on the first half of its tiles the host records (captures) the same slice loop and instantiates it,
+      // so that the recording never leaves the GPU idle; a shape with a single batch is not captured at all.
       line_plan(nx, s);
       line_plan(ny, s);
       identity_map(ny);
       const int nbatches = (npos + B - 1) / B;
+      const int count_of[2] = {npos / B, npos % B ? 1 : 0};  // batches of the full and the remainder shape
       int* host_first = nullptr;
       SK_HIP(hipHostMalloc(&host_first, sizeof(int) * nbatches, hipHostMallocDefault));
       for (int i = 0; i < nbatches; ++i) host_first[i] = i * B;
@@ -2091,35 +2125,45 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
       try {
         for (int k = 0; k < npots; ++k) {
           hipGraphExec_t exec[2] = {nullptr, nullptr};
+          bool seen[2] = {false, false};
           for (int i = 0; i < nbatches; ++i) {
             const int nb = std::min(B, npos - i * B);
-            hipGraphExec_t& ge = exec[nb == B ? 0 : 1];
-            if (!ge) {
-              hipGraph_t gr;
-              SK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-              try {
-                if (split > 1) {
-                  // the graph forks into the streams, runs the batch's tiles on them and joins them at its end
-                  const std::vector<hipStream_t> st = part_streams(split);
-                  SK_HIP(hipEventRecord(fork_ev, s));
-                  for (int q = 1; q < split; ++q) SK_HIP(hipStreamWaitEvent(st[q], fork_ev, 0));
-                  enqueue_tiles(*pots[k], 0, w.pos_dev.p, nb, off, st);
-                  for (int q = 1; q < split; ++q) {
-                    SK_HIP(hipEventRecord(join_ev[q - 1], st[q]));
-                    SK_HIP(hipStreamWaitEvent(s, join_ev[q - 1], 0));
+            const int shape = nb == B ? 0 : 1;
+            hipGraphExec_t& ge = exec[shape];
+            if (!seen[shape]) {
+              seen[shape] = true;
+              // the capture: host-side recording of the same slice loop (on the same streams, which are between
+              // the eager halves and carry no capture-mode work of their own), then its instantiation
+              auto capture = [&]() {
+                if (count_of[shape] < 2) return;
+                hipGraph_t gr;
+                SK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                try {
+                  if (split > 1) {
+                    // the graph forks into the streams, runs the batch's tiles on them and joins them at its end
+                    const std::vector<hipStream_t> st = part_streams(split);
+                    SK_HIP(hipEventRecord(fork_ev, s));
+                    for (int q = 1; q < split; ++q) SK_HIP(hipStreamWaitEvent(st[q], fork_ev, 0));
+                    enqueue_tiles(*pots[k], 0, w.pos_dev.p, nb, off, st);
+                    for (int q = 1; q < split; ++q) {
+                      SK_HIP(hipEventRecord(join_ev[q - 1], st[q]));
+                      SK_HIP(hipStreamWaitEvent(s, join_ev[q - 1], 0));
+                    }
+                  } else {
+                    enqueue_tiles(*pots[k], 0, w.pos_dev.p, nb, off, std::vector<hipStream_t>(1, s));
                   }
-                } else {
-                  enqueue_tiles(*pots[k], 0, w.pos_dev.p, nb, off, std::vector<hipStream_t>(1, s));
+                } catch (...) {
+                  hipGraph_t dropped;
+                  (void)hipStreamEndCapture(s, &dropped);
+                  if (dropped) (void)hipGraphDestroy(dropped);
+                  throw;
                 }
-              } catch (...) {
-                hipGraph_t dropped;
-                (void)hipStreamEndCapture(s, &dropped);
-                if (dropped) (void)hipGraphDestroy(dropped);
-                throw;
-              }
-              SK_HIP(hipStreamEndCapture(s, &gr));
-              SK_HIP(hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0));
-              SK_HIP(hipGraphDestroy(gr));
+                SK_HIP(hipStreamEndCapture(s, &gr));
+                SK_HIP(hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0));
+                SK_HIP(hipGraphDestroy(gr));
+              };
+              run_batch_eager(*pots[k], i * B, nb, capture);
+              continue;
             }
             SK_HIP(hipMemcpyAsync(w.pos_dev.p, host_first + i, sizeof(int), hipMemcpyHostToDevice, s));
             SK_HIP(hipGraphLaunch(ge, s));

@@ -269,10 +269,11 @@ def test_scan_is_deterministic_and_batch_independent():
     for x, y, z in zip(a, b, c):
         assert np.array_equal(x.values, y.values)
         assert np.array_equal(x.values, z.values)
-    # the slice loop replayed from captured hipGraphs (a full-batch graph and a remainder graph) gives the same bits,
-    # on one stream (the library's choice for batches under 128) and with each batch's graph forked into parts on 2 and
-    # 3 streams
-    for batch, streams in ((64, 0), (5, 0), (64, 2), (5, 3)):
+    # the slice loop replayed from captured hipGraphs gives the same bits: the first batch of a shape runs kernel by
+    # kernel while its graph is captured, the later batches of that shape replay it (batch 5: 12 full batches and a
+    # remainder of 4, run once, uncaptured; batch 16: 4 batches; batch 64: one batch, never captured), on one stream
+    # (the library's choice for batches under 128) and with each batch's graph forked into parts on 2 and 3 streams
+    for batch, streams in ((64, 0), (5, 0), (16, 0), (64, 2), (16, 2), (5, 3)):
         g = sk.STEM.scan(probe, [prepared], scan, dets,
                          sk.ExecutionOptions(batch=batch, use_graph=True, streams=streams))
         for x, y in zip(a, g):
