@@ -73,7 +73,9 @@ typedef struct sk_detector {
 typedef struct sk_options {
   int batch;        /* positions per launch (sk_hip_scan: per host chunk, or per graph replay with use_graph; the
                        other entry points: waves resident per kernel launch); 0 = the library's choice for the grid */
-  int use_graph;    /* capture the slice loop in hipGraphs and replay them */
+  int use_graph;    /* capture the slice loop in hipGraphs and replay them: per potential, one graph per batch shape,
+                       recorded while the first batch of that shape runs kernel by kernel and replayed for the rest
+                       (a shape with one batch runs kernel by kernel only) */
   int fast_path;    /* 1 (default): register-resident FFT passes where the length has one; 0: generic LDS passes */
   int pipeline;     /* scan: 1 = lanes of positions advance one slice per launch, each through its own potential
                        and slice (potentials of different slice counts share every launch); 0 = one potential at a

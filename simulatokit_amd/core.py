@@ -207,7 +207,7 @@ class ExecutionOptions:
 
     batch: int = 0          # positions per launch (a scan's host chunk, or one graph replay with use_graph); 0 =
                             # the library's choice
-    use_graph: bool = False  # replay the slice loop from captured hipGraphs
+    use_graph: bool = False  # replay the slice loop from captured hipGraphs (the first batch of a shape is captured)
     fast_path: bool = True   # register-resident FFT passes where the grid length has one
     pipeline: int = -1       # scan: 1 lanes of positions each on its own potential and slice per launch, 0 one
                              # potential at a time, -1 the library's choice (lanes for a few positions over several
