@@ -9,10 +9,11 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
-#include <functional>
+#include <exception>
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -63,6 +64,7 @@ struct Context {
   std::string why;
   hipStream_t stream = nullptr;
   std::vector<hipStream_t> side;  // further streams of split scans, created on first use (default priority)
+  std::vector<hipStream_t> capture;  // streams a graph is recorded on while the scan's own streams run (ditto)
   bool big_llc = false;   // the device has a 256 MiB last-level cache (MI300 / MI355 class: gfx94x, gfx950)
   DevBuf<double> lobato;  // 103 x 10
   // timing
@@ -2022,32 +2024,25 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
     // each tile's positions split over the streams `st` in parts of H = ceil(T / k), stream q always on the
     // workspace positions [q H, q H + H): a stream runs its parts of consecutive tiles one after another, and no two
     // streams ever share workspace, whatever the tile's size.
-    // Tiles [lo, hi) only, when given (the remaining tiles are enqueued by a later call).
     auto enqueue_tiles = [&](const sk_potential& P, int b0, const int* pos_dev, int nb, Timer& t,
-                             const std::vector<hipStream_t>& st, int lo = 0, int hi = 1 << 30) {
+                             const std::vector<hipStream_t>& st) {
       const int tiles = (nb + T - 1) / T;
       const int Tp = (nb + tiles - 1) / tiles;
       const int k = (int)st.size();
       const int H = (T + k - 1) / k;
-      for (int i = lo; i < std::min(tiles, hi); ++i) {
+      for (int i = 0; i < tiles; ++i) {
         const int first = i * Tp, count = std::min(Tp, nb - first);
         if (count <= 0) break;
         for (int q = 0; q < k && q * H < count; ++q)
-          enqueue_batch(P, b0 + first + q * H, pos_dev, std::min(H, count - q * H), t, k > 1 ? st[q] : nullptr,
-                        q * H);
+          enqueue_batch(P, b0 + first + q * H, pos_dev, std::min(H, count - q * H), t, st[q], q * H);
         if (k == 1) t.harvest(timing);
       }
     };
     // One batch through its slice loop launched kernel by kernel: on the scan's stream, or forked over the split
-    // streams and joined back into it. `between` runs on the host after the first half of the batch's tiles is
-    // enqueued and before the rest, while the GPU works on the first half.
-    auto run_batch_eager = [&](const sk_potential& P, int b0, int nb, const std::function<void()>& between) {
-      const int tiles = (nb + T - 1) / T, half = (tiles + 1) / 2;
+    // streams and joined back into it.
+    auto run_batch_eager = [&](const sk_potential& P, int b0, int nb) {
       if (split <= 1) {
-        const std::vector<hipStream_t> one(1, s);
-        enqueue_tiles(P, b0, nullptr, nb, tm, one, 0, half);
-        between();
-        enqueue_tiles(P, b0, nullptr, nb, tm, one, half);
+        enqueue_tiles(P, b0, nullptr, nb, tm, std::vector<hipStream_t>(1, s));
         return;
       }
       const std::vector<hipStream_t> st = part_streams(split);
@@ -2055,9 +2050,7 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
       SK_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
       SK_HIP(hipEventRecord(fork, s));
       for (int q = 1; q < split; ++q) SK_HIP(hipStreamWaitEvent(st[q], fork, 0));
-      enqueue_tiles(P, b0, nullptr, nb, tm, st, 0, half);
-      between();
-      enqueue_tiles(P, b0, nullptr, nb, tm, st, half);
+      enqueue_tiles(P, b0, nullptr, nb, tm, st);
       std::vector<hipEvent_t> joins(split - 1);
       for (int q = 1; q < split; ++q) {
         SK_HIP(hipEventCreateWithFlags(&joins[q - 1], hipEventDisableTiming));
@@ -2094,9 +2087,9 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
     } else {
       // hipGraph replay: per potential, one graph per batch shape (full, remainder), replayed for the batches of
       // that shape after the first; the first position of a replay is read from device memory, copied in from
-      // pinned host memory before each launch. The first batch of a shape runs kernel by kernel, and while the GPU
-      // works on the first half of its tiles the host records (captures) the same slice loop and instantiates it,
-      // so that the recording never leaves the GPU idle; a shape with a single batch is not captured at all.
+      // pinned host memory before each launch. The first batch of a shape runs kernel by kernel while a second host
+      // thread records (captures) the same slice loop on streams of its own and instantiates it, so that the
+      // recording never leaves the GPU idle; a shape with a single batch is not captured at all.
       line_plan(nx, s);
       line_plan(ny, s);
       identity_map(ny);
@@ -2122,6 +2115,18 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
       hipEvent_t fork_ev = evs.e[0];
       hipEvent_t* join_ev = evs.e.data() + 1;
       if (split > 1) (void)part_streams(split);  // create the side streams before any capture begins
+      // the streams the graphs are recorded on: the capture runs on a second host thread while this one launches
+      // the first batch on the scan's own streams, so neither waits for the other
+      while ((int)c.capture.size() < split) {
+        hipStream_t t;
+        SK_HIP(hipStreamCreateWithFlags(&t, hipStreamNonBlocking));
+        c.capture.push_back(t);
+      }
+      const std::vector<hipStream_t> cst(c.capture.begin(), c.capture.begin() + split);
+      (void)slice_plan(nx, s);  // every plan the recording looks up exists before the two threads share them
+      (void)slice_plan(ny, s);
+      int device = 0;
+      SK_HIP(hipGetDevice(&device));
       try {
         for (int k = 0; k < npots; ++k) {
           hipGraphExec_t exec[2] = {nullptr, nullptr};
@@ -2132,37 +2137,54 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
             hipGraphExec_t& ge = exec[shape];
             if (!seen[shape]) {
               seen[shape] = true;
-              // the capture: host-side recording of the same slice loop (on the same streams, which are between
-              // the eager halves and carry no capture-mode work of their own), then its instantiation
+              // the capture: host-side recording of the same slice loop on the capture streams, then its
+              // instantiation; it launches nothing
               auto capture = [&]() {
-                if (count_of[shape] < 2) return;
+                SK_HIP(hipSetDevice(device));
                 hipGraph_t gr;
-                SK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                hipStream_t cs = cst[0];
+                SK_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
                 try {
                   if (split > 1) {
                     // the graph forks into the streams, runs the batch's tiles on them and joins them at its end
-                    const std::vector<hipStream_t> st = part_streams(split);
-                    SK_HIP(hipEventRecord(fork_ev, s));
-                    for (int q = 1; q < split; ++q) SK_HIP(hipStreamWaitEvent(st[q], fork_ev, 0));
-                    enqueue_tiles(*pots[k], 0, w.pos_dev.p, nb, off, st);
+                    SK_HIP(hipEventRecord(fork_ev, cs));
+                    for (int q = 1; q < split; ++q) SK_HIP(hipStreamWaitEvent(cst[q], fork_ev, 0));
+                    enqueue_tiles(*pots[k], 0, w.pos_dev.p, nb, off, cst);
                     for (int q = 1; q < split; ++q) {
-                      SK_HIP(hipEventRecord(join_ev[q - 1], st[q]));
-                      SK_HIP(hipStreamWaitEvent(s, join_ev[q - 1], 0));
+                      SK_HIP(hipEventRecord(join_ev[q - 1], cst[q]));
+                      SK_HIP(hipStreamWaitEvent(cs, join_ev[q - 1], 0));
                     }
                   } else {
-                    enqueue_tiles(*pots[k], 0, w.pos_dev.p, nb, off, std::vector<hipStream_t>(1, s));
+                    enqueue_tiles(*pots[k], 0, w.pos_dev.p, nb, off, cst);
                   }
                 } catch (...) {
                   hipGraph_t dropped;
-                  (void)hipStreamEndCapture(s, &dropped);
+                  (void)hipStreamEndCapture(cs, &dropped);
                   if (dropped) (void)hipGraphDestroy(dropped);
                   throw;
                 }
-                SK_HIP(hipStreamEndCapture(s, &gr));
+                SK_HIP(hipStreamEndCapture(cs, &gr));
                 SK_HIP(hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0));
                 SK_HIP(hipGraphDestroy(gr));
               };
-              run_batch_eager(*pots[k], i * B, nb, capture);
+              std::exception_ptr failed;
+              std::thread recorder;
+              if (count_of[shape] >= 2)
+                recorder = std::thread([&]() {
+                  try {
+                    capture();
+                  } catch (...) {
+                    failed = std::current_exception();
+                  }
+                });
+              try {
+                run_batch_eager(*pots[k], i * B, nb);
+              } catch (...) {
+                if (recorder.joinable()) recorder.join();
+                throw;
+              }
+              if (recorder.joinable()) recorder.join();
+              if (failed) std::rethrow_exception(failed);
               continue;
             }
             SK_HIP(hipMemcpyAsync(w.pos_dev.p, host_first + i, sizeof(int), hipMemcpyHostToDevice, s));
