@@ -625,6 +625,19 @@ ColArgs col_args(const sk_potential& p, float2* G) {
   return c;
 }
 
+// Blocks per wave of the detecting column pass a scan of p0 launches: the count of its partial sums per detector
+// slot and position.
+int detect_blocks_per_wave(const sk_potential& p0, bool fast, int pipe, hipStream_t s) {
+  const int nx = p0.grid.nx;
+  if (fast && fast_col_supported(nx, p0.nc, COL_IN_G | COL_FWD | COL_MUL_P | COL_DETECT))
+    return fast_col_blocks_per_wave(nx, p0.nc, pipe);
+  ColArgs f = col_args(p0, nullptr);
+  f.flags = COL_IN_G | COL_FWD | COL_MUL_P | COL_DETECT;
+  if (fast && reg_col_supported(f)) return reg_col_blocks_per_wave(f);
+  const int gcols = col_generic_lines(slice_plan(nx, s));
+  return (p0.nc + gcols - 1) / gcols;
+}
+
 // Runs `batch` waves whose first row pass input is ready (full real-space waves in `full`, or compact G after a
 // generated INIT); writes requested planes. planes_mask[j] = output index for plane j (or -1).
 struct Timer {
@@ -1984,10 +1997,7 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
     w.rx.ensure((size_t)T * nx);
     w.ry.ensure((size_t)T * ny);
     if (!fits) w.full.ensure((size_t)T * N);
-    const int gcols = col_generic_lines(slice_plan(nx, s));
-    int bpw = (p0.nc + gcols - 1) / gcols;
-    if (fast && fast_col_supported(nx, p0.nc, COL_IN_G | COL_FWD | COL_MUL_P | COL_DETECT))
-      bpw = fast_col_blocks_per_wave(nx, p0.nc, pipe);
+    const int bpw = detect_blocks_per_wave(p0, fast, pipe, s);
     dp.blocks_per_wave = bpw;
     w.partial.ensure((size_t)T * nslots * bpw);
     dp.partial = w.partial.p;
@@ -2127,9 +2137,9 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
       (void)slice_plan(ny, s);
       int device = 0;
       SK_HIP(hipGetDevice(&device));
+      hipGraphExec_t exec[2] = {nullptr, nullptr};  // per potential; destroyed on the way out, also on errors
       try {
         for (int k = 0; k < npots; ++k) {
-          hipGraphExec_t exec[2] = {nullptr, nullptr};
           bool seen[2] = {false, false};
           for (int i = 0; i < nbatches; ++i) {
             const int nb = std::min(B, npos - i * B);
@@ -2191,11 +2201,15 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
             SK_HIP(hipGraphLaunch(ge, s));
           }
           SK_HIP(hipStreamSynchronize(s));
-          for (auto& e : exec)
+          for (auto& e : exec) {
             if (e) SK_HIP(hipGraphExecDestroy(e));
+            e = nullptr;
+          }
         }
       } catch (...) {
         (void)hipStreamSynchronize(s);
+        for (auto& e : exec)
+          if (e) (void)hipGraphExecDestroy(e);
         (void)hipHostFree(host_first);
         throw;
       }
@@ -2308,10 +2322,7 @@ int sk_hip_scan_noalloc(const sk_potential* const* pots, int npots, const sk_pro
     }
     // ---- the scratch: fixed part, then the batches' waves ----
     const bool fast = !opt || opt->fast_path != 0;
-    const int gcols = col_generic_lines(slice_plan(nx, s));
-    int bpw = (p0.nc + gcols - 1) / gcols;
-    if (fast && fast_col_supported(nx, p0.nc, COL_IN_G | COL_FWD | COL_MUL_P | COL_DETECT))
-      bpw = fast_col_blocks_per_wave(nx, p0.nc, 0);
+    const int bpw = detect_blocks_per_wave(p0, fast, 0, s);
     dp.blocks_per_wave = bpw;
     Arena a{c.scratch_dev, kNoallocScratchBytes, 0};
     const size_t fixed = N * sizeof(float2) + (2 + (size_t)nslots) * npos * sizeof(double) +

@@ -269,6 +269,7 @@ int fast_col_blocks_per_wave(int nx, int nc, int pipeline);  // blocks of the de
 // (sk_reg.hip): the row modes of the fast passes, and the middle slices' column pass
 bool reg_row_supported(const RowArgs& a);
 bool reg_col_supported(const ColArgs& a);
+int reg_col_blocks_per_wave(const ColArgs& a);  // blocks per wave of the register column pass (its detector partials)
 void launch_row_reg(const RowArgs& a, int batch, hipStream_t s);
 void launch_col_reg(const ColArgs& a, int batch, hipStream_t s);
 // lane pipeline launches (sk_fast.hip): the row pass and the column pass of one step, the partial-sum reduction of

@@ -309,9 +309,13 @@ __device__ __forceinline__ int tile_elem(int e, int c) {
   else return rslot(e);
 }
 
-// Column pass of the middle slices (COL_IN_G | COL_FWD | COL_MUL_P | COL_INV | COL_OUT): block = TC compact columns
-// [TC blockIdx.x, + TC) of positions [b0, b0 + kRegPB); TC x T threads, the T lanes of column c = tid / T.
-template <int M, bool BLUE, int TC>
+// Column pass: block = TC compact columns [TC blockIdx.x, + TC) of positions [b0, b0 + kRegPB); TC x T threads, the T
+// lanes of column c = tid / T. MODE 0: a middle slice (COL_IN_G | COL_FWD | COL_MUL_P | COL_INV | COL_OUT); 1: the
+// first pass, whose input is the probe spectrum times the position's ramps, generated in the block phase
+// (COL_GENERATE | COL_INV | COL_OUT); 2: the last pass, which ends in the detectors (COL_IN_G | COL_FWD | COL_MUL_P |
+// COL_DETECT): each thread sums its elements' contributions to one detector slot at a time, pixels in a fixed order,
+// then a tree over the block writes the block's partial sum (partial[position][slot][blockIdx.x]).
+template <int M, bool BLUE, int TC, int MODE>
 __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColArgs a, const float2* __restrict__ roots,
                                                                        const float2* __restrict__ chirp,
                                                                        const float2* __restrict__ khat, int batch) {
@@ -322,6 +326,7 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
   __shared__ float2 tab[RegTw<M>::kSize];
   __shared__ float2 cs[BLUE ? 512 : 1];
   __shared__ float2 ks[BLUE ? 1024 : 1];
+  __shared__ double red[MODE == 2 ? (NT / 64) * kMaxSlots : 1];
   const int tid = threadIdx.x;
   const int n = BLUE ? a.nx : M;
   RegTw<M>::build(tab, roots, tid, NT);
@@ -338,8 +343,8 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
   auto slot = [wc](int e) { return tile_elem<M, TC>(e, wc); };
   const long long gw = (long long)a.nx * a.nc;
   // P of the lane's elements of column c0 + wc, for every position of the block
-  float2 pv[V];
-  {
+  float2 pv[MODE == 1 ? 1 : V];
+  if constexpr (MODE != 1) {
     const float2* Pc = a.P + c0 + wc;
 #pragma unroll
     for (int r = 0; r < V; ++r) {
@@ -347,19 +352,28 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
       pv[r] = e < n ? Pc[(long long)e * a.nc] : make_float2(0.f, 0.f);
     }
   }
-  const bool kept = a.keep[c0 + cb] >= 0;  // padding columns are never read nor written
+  const int iyb = a.keep[c0 + cb];
+  const bool kept = iyb >= 0;  // padding columns are never read nor written
   const int b0 = blockIdx.y * kRegPB;
   // The block phase's values of the next position are loaded into registers while this position transforms: the
   // loads' latency then overlaps the arithmetic (measured: 1024 points, 8 columns, 6.76 -> 3.95 ms per launch of
   // 1024 positions; 384 points 3.03 -> 2.28 ms per 4096).
   constexpr int NL = (M + RSTEP - 1) / RSTEP;
-  float2 pre[NL];
+  float2 pre[MODE == 1 ? 1 : NL];
+  // The first pass generates each position's probe where it writes the tile (prefetching the three factors of
+  // every element measured slower at 384 points and the same at 1024).
+  auto generate = [&](long long bb, int x) {
+    return kept ? cmul(cmul(a.spec[(long long)x * a.ny + iyb], a.rx[bb * a.nx + x]), a.ry[bb * a.ny + iyb])
+                : make_float2(0.f, 0.f);
+  };
   auto prefetch = [&](long long bb) {
-    const float2* gg = a.G + bb * gw + c0 + cb;
+    if constexpr (MODE != 1) {
+      const float2* gg = a.G + bb * gw + c0 + cb;
 #pragma unroll
-    for (int r = 0; r < NL; ++r) {
-      const int x = xi + RSTEP * r;
-      pre[r] = (kept && x < n) ? gg[(long long)x * a.nc] : make_float2(0.f, 0.f);
+      for (int r = 0; r < NL; ++r) {
+        const int x = xi + RSTEP * r;
+        pre[r] = (kept && x < n) ? gg[(long long)x * a.nc] : make_float2(0.f, 0.f);
+      }
     }
   };
   if (b0 < batch) prefetch(b0);
@@ -371,7 +385,10 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
 #pragma unroll
     for (int r = 0; r < NL; ++r) {
       const int x = xi + RSTEP * r;
-      if (x < n) tile[cb * CS + tile_elem<M, TC>(x, cb)] = pre[r];
+      if (x < n) {
+        if constexpr (MODE == 1) tile[cb * CS + tile_elem<M, TC>(x, cb)] = generate(b, x);
+        else tile[cb * CS + tile_elem<M, TC>(x, cb)] = pre[r];
+      }
     }
     if (p + 1 < kRegPB && b + 1 < batch) prefetch(b + 1);
     __syncthreads();
@@ -384,10 +401,65 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
       v[r] = e < n ? col[slot(e)] : make_float2(0.f, 0.f);
     }
     wave_sync();  // every lane has read its column before the exchanges overwrite it
-    line_fft<M, -1, BLUE>(v, col, il, tab, cs, ks, n, slot);
+    if constexpr (MODE != 1) {
+      line_fft<M, -1, BLUE>(v, col, il, tab, cs, ks, n, slot);
 #pragma unroll
-    for (int r = 0; r < V; ++r) v[r] = cmul(v[r], pv[r]);
-    line_fft<M, +1, BLUE>(v, col, il, tab, cs, ks, n, slot);
+      for (int r = 0; r < V; ++r) v[r] = cmul(v[r], pv[r]);
+    }
+    if constexpr (MODE == 2) {
+      // the detectors: the lane's elements x = il + T r of column c0 + wc (ky = keep); the pixelated accumulators,
+      // then each slot summed over the lane's elements, over the wavefront (shuffles, fixed order) and over the
+      // block's wavefronts in order
+      const DetPlan& d = a.det;
+      const int iy = a.keep[c0 + wc];
+      const long long pos = first_position(a.pos0, a.pos_dev) + b;
+      // The lane's values go back to its own slots of the column, and the sums read them from there in a loop that
+      // is not unrolled: held in registers through the slot loop, float64 intensities of all V values spilled.
+#pragma unroll
+      for (int r = 0; r < V; ++r) {
+        const int e = il + T * r;
+        if (e < n) col[slot(e)] = v[r];
+      }
+      auto intensity = [&](int e) {
+        const float2 u = col[slot(e)];
+        return ((double)u.x * u.x + (double)u.y * u.y) * d.scale;
+      };
+      if (iy >= 0 && d.npix > 0) {
+#pragma unroll 1
+        for (int e = il; e < n; e += T) detect_pixelated(d, e, iy, intensity(e), pos);
+      }
+      const int lane = tid & 63, wave = tid >> 6;
+      const double sy = (iy >= 0 && d.sqy) ? d.sqy[iy] : 0.0;
+      for (int q = 0; q < d.nslots; ++q) {
+        const int kind = d.slot_kind[q];
+        double val = 0.0;
+        if (iy >= 0) {
+          if (kind == 0) {
+            const double i2 = d.slot_i2[q], o2 = d.slot_o2[q];
+#pragma unroll 1
+            for (int e = il; e < n; e += T) {
+              const double a2 = __dadd_rn(d.sqx[e], sy);  // float64, as the host rounds it
+              if (a2 >= i2 && a2 < o2) val += intensity(e);
+            }
+          } else {
+#pragma unroll 1
+            for (int e = il; e < n; e += T) val += slot_value(d, q, e, iy, 0.0, intensity(e));
+          }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) val += __shfl_xor(val, off);
+        if (lane == 0) red[wave * kMaxSlots + q] = val;
+      }
+      __syncthreads();
+      if (tid < d.nslots) {
+        double t = 0.0;
+        for (int wv = 0; wv < NT / 64; ++wv) t += red[wv * kMaxSlots + tid];
+        d.partial[(b * d.nslots + tid) * d.blocks_per_wave + blockIdx.x] = t;
+      }
+      continue;  // nothing is stored; the next position's block phase waits at the loop's first barrier
+    } else {
+      line_fft<M, +1, BLUE>(v, col, il, tab, cs, ks, n, slot);
+    }
 #pragma unroll
     for (int r = 0; r < V; ++r) {
       const int e = il + T * r;
@@ -451,13 +523,22 @@ int reg_col_tile(int m, bool blue, int requested) {
                            std::to_string(m) + "-point lines (384: 8 or 16; 1024: 4 or 8)");
 }
 
-template <int M, bool BLUE, int TC>
-void col_go(const ColArgs& a, int batch, hipStream_t s) {
+constexpr int COL_FIRST = COL_GENERATE | COL_INV | COL_OUT;
+constexpr int COL_LAST = COL_IN_G | COL_FWD | COL_MUL_P | COL_DETECT;
+
+template <int M, bool BLUE, int TC, int MODE>
+void col_go_mode(const ColArgs& a, int batch, hipStream_t s) {
   constexpr int NT = TC * (M / RegPlan<M>::V);
   dim3 grid(a.nc / TC, (batch + kRegPB - 1) / kRegPB);
-  launch_timed(col_reg_kernel<M, BLUE, TC>, grid, dim3(NT), s, a.ev_start, a.ev_stop, a, a.fx.roots, a.fx.chirp,
-               a.fx.khat, batch);
+  launch_timed(col_reg_kernel<M, BLUE, TC, MODE>, grid, dim3(NT), s, a.ev_start, a.ev_stop, a, a.fx.roots,
+               a.fx.chirp, a.fx.khat, batch);
   SK_HIP(hipGetLastError());
+}
+template <int M, bool BLUE, int TC>
+void col_go(const ColArgs& a, int batch, hipStream_t s) {
+  if (a.flags == COL_FIRST) col_go_mode<M, BLUE, TC, 1>(a, batch, s);
+  else if (a.flags == COL_LAST) col_go_mode<M, BLUE, TC, 2>(a, batch, s);
+  else col_go_mode<M, BLUE, TC, 0>(a, batch, s);
 }
 
 }  // namespace
@@ -469,9 +550,14 @@ bool reg_row_supported(const RowArgs& a) {
 }
 
 bool reg_col_supported(const ColArgs& a) {
-  if (a.fx.use_mfma || a.flags != COL_MID || a.fx.n != a.nx) return false;
+  if (a.fx.use_mfma || a.fx.n != a.nx) return false;
+  if (a.flags != COL_MID && a.flags != COL_FIRST && a.flags != COL_LAST) return false;
   const int m = reg_length(a.fx);
   return m != 0 && a.nc % reg_col_tile(m, a.fx.blue, a.col_tile) == 0;
+}
+
+int reg_col_blocks_per_wave(const ColArgs& a) {
+  return a.nc / reg_col_tile(reg_length(a.fx), a.fx.blue, a.col_tile);
 }
 
 void launch_row_reg(const RowArgs& a, int batch, hipStream_t s) {
