@@ -155,6 +155,40 @@ __device__ inline void dft16(float2 (&u)[16], int sign) {
   }
 }
 
+// dft16 of inputs whose upper half u[8..15] is zero (Bluestein's zero-padded forward transform): the first
+// layer's DFT4s take two inputs, a0 +- a1 and a0 +- (sign i) a1; the twiddles and the second layer as in dft16.
+__device__ inline void dft16_half(float2 (&u)[16], int sign) {
+  const float c1 = 0.923879532511286756f, s1 = 0.382683432365089772f, r2 = 0.707106781186547524f;
+  float2 y[4][4];  // y[n2][k1]
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) {
+    const float2 a0 = u[n2], a1 = u[4 + n2], t3 = mul_si(a1, sign);
+    y[n2][0] = cadd(a0, a1);
+    y[n2][1] = cadd(a0, t3);
+    y[n2][2] = csub(a0, a1);
+    y[n2][3] = csub(a0, t3);
+  }
+  const float sg = sign < 0 ? -1.f : 1.f;
+  y[1][1] = cmul(y[1][1], make_float2(c1, sg * s1));
+  y[1][2] = cmul(y[1][2], make_float2(r2, sg * r2));
+  y[1][3] = cmul(y[1][3], make_float2(s1, sg * c1));
+  y[2][1] = cmul(y[2][1], make_float2(r2, sg * r2));
+  y[2][2] = mul_si(y[2][2], sign);
+  y[2][3] = cmul(y[2][3], make_float2(-r2, sg * r2));
+  y[3][1] = cmul(y[3][1], make_float2(s1, sg * c1));
+  y[3][2] = cmul(y[3][2], make_float2(-r2, sg * r2));
+  y[3][3] = cmul(y[3][3], make_float2(-c1, -sg * s1));
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    float2 b0 = y[0][k1], b1 = y[1][k1], b2 = y[2][k1], b3 = y[3][k1];
+    dft4(b0, b1, b2, b3, sign);
+    u[k1] = b0;
+    u[k1 + 4] = b1;
+    u[k1 + 8] = b2;
+    u[k1 + 12] = b3;
+  }
+}
+
 template <int R>
 __device__ inline void dft_pow2(float2 (&u)[R], int sign) {
   if constexpr (R == 2) {

@@ -99,7 +99,8 @@ struct RegTw {
 // One Stockham stage of radix R at sub-length L: butterfly bi = i + T q (q < V / R) reads x[bi + (M / R) j],
 // multiplies by w_{L R}^{j k} (k = bi mod L; tw = the stage's table), takes the DFT and writes y[(bi - k) R + k + L j].
 // FROM_REG: the inputs are the registers (element i + T r in v[r]); TO_REG: the outputs go to the registers.
-template <int M, int R, int L, int SIGN, bool FROM_REG, bool TO_REG, class Slot>
+// HALF_IN (a radix-16 first stage): the inputs j >= 8 are zero and are not read.
+template <int M, int R, int L, int SIGN, bool FROM_REG, bool TO_REG, class Slot, bool HALF_IN = false>
 __device__ __forceinline__ void vstage(float2 (&v)[RegPlan<M>::V], float2* __restrict__ buf, int i,
                                       const float2* __restrict__ tw, Slot slot) {
   constexpr int V = RegPlan<M>::V, T = M / V, NB = V / R, STRIDE = M / R;
@@ -110,7 +111,7 @@ __device__ __forceinline__ void vstage(float2 (&v)[RegPlan<M>::V], float2* __res
     const int bi = i + T * q;
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-      if constexpr (FROM_REG) u[q][j] = v[q + NB * j];
+      if constexpr (FROM_REG) u[q][j] = (HALF_IN && j >= R / 2) ? make_float2(0.f, 0.f) : v[q + NB * j];
       else u[q][j] = buf[slot(bi + STRIDE * j)];
     }
   }
@@ -126,7 +127,12 @@ __device__ __forceinline__ void vstage(float2 (&v)[RegPlan<M>::V], float2* __res
         u[q][j] = SIGN < 0 ? cmul(u[q][j], w) : cmulc(u[q][j], w);
       }
     }
-    dft_r<R>(u[q], SIGN);
+    if constexpr (HALF_IN) {
+      static_assert(R == 16 && L == 1 && FROM_REG, "the zero upper half: a radix-16 first stage");
+      dft16_half(u[q], SIGN);
+    } else {
+      dft_r<R>(u[q], SIGN);
+    }
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       if constexpr (TO_REG) v[q + NB * j] = u[q][j];
@@ -136,11 +142,12 @@ __device__ __forceinline__ void vstage(float2 (&v)[RegPlan<M>::V], float2* __res
   if constexpr (!TO_REG) wave_sync();
 }
 
-// Unscaled transform of the line in the registers (SIGN -1 forward, +1 inverse).
-template <int M, int SIGN, class Slot>
+// Unscaled transform of the line in the registers (SIGN -1 forward, +1 inverse). HALF_IN: the elements of the upper
+// half of the line (r >= V / 2 in every lane) are zero.
+template <int M, int SIGN, class Slot, bool HALF_IN = false>
 __device__ __forceinline__ void vfft(float2 (&v)[RegPlan<M>::V], float2* buf, int i, const float2* tab, Slot slot) {
   using P = RegPlan<M>;
-  vstage<M, P::R1, 1, SIGN, true, false>(v, buf, i, nullptr, slot);
+  vstage<M, P::R1, 1, SIGN, true, false, Slot, HALF_IN>(v, buf, i, nullptr, slot);
   vstage<M, P::R2, P::R1, SIGN, false, false>(v, buf, i, tab, slot);
   vstage<M, P::R3, P::R1 * P::R2, SIGN, false, true>(v, buf, i, tab + RegTw<M>::S2, slot);
 }
@@ -165,17 +172,22 @@ __device__ __forceinline__ void line_fft(float2 (&v)[RegPlan<M>::V], float2* buf
         v[r] = make_float2(0.f, 0.f);
       }
     }
-    vfft<M, -1>(v, buf, i, tab, slot);
+    // the zero-padded input: elements i + T r with r >= V / 2 lie at or past 512 >= n
+    vfft<M, -1, Slot, true>(v, buf, i, tab, slot);
 #pragma unroll
     for (int r = 0; r < V; ++r) {
       const float2 kh = ks[i + T * r];
       v[r] = SIGN < 0 ? cmul(v[r], kh) : cmulc(v[r], kh);
     }
     vfft<M, +1>(v, buf, i, tab, slot);
+    // only the first n <= 512 outputs are kept: the upper half is dropped here, so that the last stage's work for it
+    // is dead code
 #pragma unroll
     for (int r = 0; r < V; ++r) {
       const int e = i + T * r;
-      if (e < n) {
+      if (r >= V / 2) {
+        v[r] = make_float2(0.f, 0.f);
+      } else if (e < n) {
         const float2 c = cs[e];
         v[r] = SIGN < 0 ? cmul(v[r], c) : cmulc(v[r], c);
       }

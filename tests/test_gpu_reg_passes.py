@@ -35,7 +35,10 @@ def test_register_passes_scan_as_the_generic_passes_and_the_oracle(nx, ny, tiles
     positions = sk.GridScan((0, 0), (SRTIO3_A, SRTIO3_A), 3, 3).positions[[0, 4, 8]]
     prepared = pot.prepare(probe.beam)
     outer = 100.0  # inside the largest simulated angle of every grid here (about 150 mrad at 384 x 384 over 31 A)
-    dets = [sk.AnnularDetector(40, outer), sk.PixelatedDetector("cutoff"), sk.CenterOfMassDetector("cutoff", "mrad")]
+    # every kind of the detecting pass: an annular slot, pixelated accumulators, two centre-of-mass slots and the
+    # bins of a segmented detector
+    dets = [sk.AnnularDetector(40, outer), sk.PixelatedDetector("cutoff"), sk.CenterOfMassDetector("cutoff", "mrad"),
+            sk.SegmentedDetector(2, 4, 10, 40, 0.3)]
     reg = sk.STEM.scan(probe, [prepared], positions, dets, sk.ExecutionOptions())
     gen = sk.STEM.scan(probe, [prepared], positions, dets, sk.ExecutionOptions(fast_path=False))
     widths = (8, 16) if nx == 384 else ((4, 8) if nx in (509, 1024) else ())
@@ -45,8 +48,9 @@ def test_register_passes_scan_as_the_generic_passes_and_the_oracle(nx, ny, tiles
             assert rel_max(other[k].values, reg[k].values) <= 2e-5, (tc, k)
     V, _ = O.potential(nx, ny, st.numbers, st.positions, st.box, pot.slice_thicknesses)
     want = O.stem_scan([V], [pot.slice_thicknesses], (st.box[0], st.box[1]), 200e3, 20.0, positions,
-                       [O.det_annular(40, outer), O.det_pixelated("cutoff"), O.det_com("cutoff", "mrad")])
-    for k, name in enumerate(("annular", "pattern", "COM")):
+                       [O.det_annular(40, outer), O.det_pixelated("cutoff"), O.det_com("cutoff", "mrad"),
+                        O.det_segmented(2, 4, 10, 40, 0.3)])
+    for k, name in enumerate(("annular", "pattern", "COM", "segmented")):
         e_rg = rel_max(reg[k].values, gen[k].values)
         e_ro = rel_l2(reg[k].values, np.asarray(want[k]).reshape(-1))
         e_go = rel_l2(gen[k].values, np.asarray(want[k]).reshape(-1))

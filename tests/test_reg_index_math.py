@@ -147,3 +147,37 @@ def test_column_tiles_are_free_of_bank_conflicts(M, TC, CS):
                         if s < len(radices) - 1:
                             assert worst_conflict(writes) == 1
             L *= R
+
+
+def _dft16_half(u, sign):
+    """dft16_half (sk_dft.h): the 4 x 4 DFT16 of inputs whose upper half is zero, in the kernel's order: first-layer
+    DFT4s of (a0, a1, 0, 0), the w16 twiddles, the second-layer DFT4s."""
+    si = 1j * sign
+
+    def dft4(a0, a1, a2, a3):
+        t0, t1, t2, t3 = a0 + a2, a0 - a2, a1 + a3, si * (a1 - a3)
+        return t0 + t2, t1 + t3, t0 - t2, t1 - t3
+
+    y = np.zeros((4, 4), complex)
+    for n2 in range(4):
+        a0, a1 = u[n2], u[4 + n2]
+        y[n2] = [a0 + a1, a0 + si * a1, a0 - a1, a0 - si * a1]
+    w16 = np.exp(sign * 2j * np.pi / 16)
+    for n2 in range(4):
+        for k1 in range(4):
+            y[n2][k1] *= w16 ** (n2 * k1)
+    out = np.zeros(16, complex)
+    for k1 in range(4):
+        b = dft4(y[0][k1], y[1][k1], y[2][k1], y[3][k1])
+        for k2 in range(4):
+            out[k1 + 4 * k2] = b[k2]
+    return out
+
+
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_dft16_of_a_zero_upper_half_equals_numpy(sign):
+    rng = np.random.default_rng(21)
+    u = np.zeros(16, complex)
+    u[:8] = rng.normal(size=8) + 1j * rng.normal(size=8)
+    want = np.fft.fft(u) if sign < 0 else np.fft.ifft(u) * 16
+    assert np.abs(_dft16_half(u, sign) - want).max() <= 1e-12 * np.abs(want).max()
