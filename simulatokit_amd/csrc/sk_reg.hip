@@ -195,6 +195,18 @@ __device__ __forceinline__ void line_fft(float2 (&v)[RegPlan<M>::V], float2* buf
   }
 }
 
+// Whether the elements i + T r of an M-point line (n = M, not Bluestein's) lie, for every lane i, in the band that
+// the antialias aperture zeroes on every grid: |k| > floor(M / 3), since the kept columns and the propagator's rows
+// satisfy |k| <= 0.3333333 n. Their values are known zeros on the way in (compact inputs, products with P) and are
+// never stored on the way out, so the code for them folds away (1024: r = 6..9; 384: r = 9..15).
+template <int M>
+__host__ __device__ constexpr bool in_band(int r) {
+  constexpr int T = M / RegPlan<M>::V, K = M / 3;
+  return r * T > K && r * T + T - 1 < M - K;
+}
+static_assert(!in_band<1024>(5) && in_band<1024>(6) && in_band<1024>(9) && !in_band<1024>(10), "1024-point band");
+static_assert(!in_band<384>(8) && in_band<384>(9) && in_band<384>(15) && !in_band<384>(16), "384-point band");
+
 constexpr int kRegPB = 8;      // positions per block (t or P loaded once per 8 positions)
 constexpr int kRowWaves = 2;   // wavefronts per row-pass block
 
@@ -257,7 +269,8 @@ __global__ void __launch_bounds__(64 * kRowWaves) row_reg_kernel(RowArgs a, cons
 #pragma unroll
       for (int r = 0; r < V; ++r) {
         const int c = cidx(il, r);
-        v[r] = c >= 0 ? g[c] : make_float2(0.f, 0.f);
+        if (!BLUE && in_band<M>(r)) v[r] = make_float2(0.f, 0.f);
+        else v[r] = c >= 0 ? g[c] : make_float2(0.f, 0.f);
       }
     } else {
       const float2* g = a.in_full + b * a.in_full_stride + xs * n;
@@ -292,6 +305,7 @@ __global__ void __launch_bounds__(64 * kRowWaves) row_reg_kernel(RowArgs a, cons
         float2* g = a.G + b * gw + xs * a.nc;
 #pragma unroll
         for (int r = 0; r < V; ++r) {
+          if (!BLUE && in_band<M>(r)) continue;  // never kept: the forward transform's outputs there are dead
           const int c = cidx(il, r);
           if (c >= 0) g[c] = v[r];
         }
@@ -361,7 +375,8 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
 #pragma unroll
     for (int r = 0; r < V; ++r) {
       const int e = i + T * r;
-      pv[r] = e < n ? Pc[(long long)e * a.nc] : make_float2(0.f, 0.f);
+      if (!BLUE && in_band<M>(r)) pv[r] = make_float2(0.f, 0.f);
+      else pv[r] = e < n ? Pc[(long long)e * a.nc] : make_float2(0.f, 0.f);
     }
   }
   const int iyb = a.keep[c0 + cb];
@@ -416,7 +431,10 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
     if constexpr (MODE != 1) {
       line_fft<M, -1, BLUE>(v, col, il, tab, cs, ks, n, slot);
 #pragma unroll
-      for (int r = 0; r < V; ++r) v[r] = cmul(v[r], pv[r]);
+      for (int r = 0; r < V; ++r) {
+        if (!BLUE && in_band<M>(r)) v[r] = make_float2(0.f, 0.f);  // P is zero there
+        else v[r] = cmul(v[r], pv[r]);
+      }
     }
     if constexpr (MODE == 2) {
       // the detectors: the lane's elements x = il + T r of column c0 + wc (ky = keep); the pixelated accumulators,

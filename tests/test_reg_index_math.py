@@ -181,3 +181,31 @@ def test_dft16_of_a_zero_upper_half_equals_numpy(sign):
     u[:8] = rng.normal(size=8) + 1j * rng.normal(size=8)
     want = np.fft.fft(u) if sign < 0 else np.fft.ifft(u) * 16
     assert np.abs(_dft16_half(u, sign) - want).max() <= 1e-12 * np.abs(want).max()
+
+
+def _in_band(M, r):
+    """in_band<M> (sk_reg.hip): r T > floor(M / 3) and r T + T - 1 < M - floor(M / 3)."""
+    T = M // PLANS[M][0]
+    K = M // 3
+    return r * T > K and r * T + T - 1 < M - K
+
+
+@pytest.mark.parametrize("M", [384, 1024])
+def test_the_folded_band_holds_no_kept_frequency_on_any_grid(M):
+    # the antialias aperture keeps |k| <= 0.6666666 / max(sx, sy) / 2 (Antialias.swift:18-39, radial, so its rows and
+    # the kept columns both lie in that range on each axis); the elements the register passes treat as known zeros
+    # (i + T r for r in the band, every lane i) must hold no kept frequency, for square and unequal samplings
+    V = PLANS[M][0]
+    T = M // V
+    band = [i + T * r for r in range(V) if _in_band(M, r) for i in range(T)]
+    assert band, M
+    for ex, ey in ((30.0, 30.0), (31.2, 31.2), (30.0, 45.0), (45.0, 30.0), (12.0, 80.0), (100.0, 7.5)):
+        for n, extent in ((M, ex), (M, ey)):
+            s = extent / n
+            kc = 0.6666666 / max(ex / M, ey / M) / 2
+            k = np.fft.fftfreq(n, s)
+            kept = set(np.nonzero(np.abs(k) <= kc)[0].tolist())
+            assert kept.isdisjoint(band), (M, ex, ey)
+    # and the band is as wide as that argument allows: the element just outside it can be kept
+    r0 = min(r for r in range(V) if _in_band(M, r))
+    assert not _in_band(M, r0 - 1) and (r0 - 1) * T + T - 1 >= M // 3
