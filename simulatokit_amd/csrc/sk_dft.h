@@ -189,6 +189,108 @@ __device__ inline void dft16_half(float2 (&u)[16], int sign) {
   }
 }
 
+// Complex sums whose operands may be known zeros (compile-time flags ZA, ZB): a known zero costs nothing. A result
+// is a known zero when both operands are.
+template <bool ZA, bool ZB>
+__device__ __forceinline__ float2 zadd(float2 a, float2 b) {
+  if constexpr (ZA && ZB) return make_float2(0.f, 0.f);
+  else if constexpr (ZA) return b;
+  else if constexpr (ZB) return a;
+  else return cadd(a, b);
+}
+template <bool ZA, bool ZB>
+__device__ __forceinline__ float2 zsub(float2 a, float2 b) {
+  if constexpr (ZA && ZB) return make_float2(0.f, 0.f);
+  else if constexpr (ZA) return make_float2(-b.x, -b.y);
+  else if constexpr (ZB) return a;
+  else return csub(a, b);
+}
+
+// dft4 whose inputs j with bit j of Z set are known zeros (not read); with Z = 0 the operations of dft4
+template <int Z>
+__device__ __forceinline__ void dft4_z(float2& a0, float2& a1, float2& a2, float2& a3, int sign) {
+  constexpr bool z0 = Z & 1, z1 = (Z >> 1) & 1, z2 = (Z >> 2) & 1, z3 = (Z >> 3) & 1;
+  constexpr bool zt0 = z0 && z2, zt2 = z1 && z3;  // t1 is zero with t0, t3 with t2
+  const float2 t0 = zadd<z0, z2>(a0, a2), t1 = zsub<z0, z2>(a0, a2), t2 = zadd<z1, z3>(a1, a3);
+  const float2 t3 = zt2 ? make_float2(0.f, 0.f) : mul_si(zsub<z1, z3>(a1, a3), sign);
+  a0 = zadd<zt0, zt2>(t0, t2);
+  a2 = zsub<zt0, zt2>(t0, t2);
+  a1 = zadd<zt0, zt2>(t1, t3);
+  a3 = zsub<zt0, zt2>(t1, t3);
+}
+
+// dft8 whose inputs with bit j of Z set are known zeros; with Z = 0 the operations of dft8
+template <int Z>
+__device__ __forceinline__ void dft8_z(float2 (&u)[8], int sign) {
+  constexpr int ZE = (Z & 1) | ((Z >> 1) & 2) | ((Z >> 2) & 4) | ((Z >> 3) & 8);         // samples 0, 2, 4, 6
+  constexpr int ZO = ((Z >> 1) & 1) | ((Z >> 2) & 2) | ((Z >> 3) & 4) | ((Z >> 4) & 8);  // samples 1, 3, 5, 7
+  constexpr bool ze = ZE == 15, zo = ZO == 15;
+  float2 e0 = u[0], e1 = u[2], e2 = u[4], e3 = u[6];
+  float2 o0 = u[1], o1 = u[3], o2 = u[5], o3 = u[7];
+  dft4_z<ZE>(e0, e1, e2, e3, sign);
+  dft4_z<ZO>(o0, o1, o2, o3, sign);
+  const float h = 0.70710678118654752f;
+  const float s = float(sign);
+  const float2 w1 = make_float2((o1.x - s * o1.y) * h, (o1.y + s * o1.x) * h);
+  const float2 w2 = mul_si(o2, sign);
+  const float2 w3 = make_float2((-o3.x - s * o3.y) * h, (-o3.y + s * o3.x) * h);
+  u[0] = zadd<ze, zo>(e0, o0);
+  u[4] = zsub<ze, zo>(e0, o0);
+  u[1] = zadd<ze, zo>(e1, w1);
+  u[5] = zsub<ze, zo>(e1, w1);
+  u[2] = zadd<ze, zo>(e2, w2);
+  u[6] = zsub<ze, zo>(e2, w2);
+  u[3] = zadd<ze, zo>(e3, w3);
+  u[7] = zsub<ze, zo>(e3, w3);
+}
+
+// dft16 (4 x 4, as dft16) whose inputs with bit j of Z set are known zeros; with Z = 0 the operations of dft16
+template <int Z>
+__device__ __forceinline__ void dft16_z(float2 (&u)[16], int sign) {
+  const float c1 = 0.923879532511286756f, s1 = 0.382683432365089772f, r2 = 0.707106781186547524f;
+  // first layer over n1 for each n2: inputs n2, 4 + n2, 8 + n2, 12 + n2
+  constexpr int Z0 = (Z & 1) | ((Z >> 3) & 2) | ((Z >> 6) & 4) | ((Z >> 9) & 8);
+  constexpr int Z1 = ((Z >> 1) & 1) | ((Z >> 4) & 2) | ((Z >> 7) & 4) | ((Z >> 10) & 8);
+  constexpr int Z2 = ((Z >> 2) & 1) | ((Z >> 5) & 2) | ((Z >> 8) & 4) | ((Z >> 11) & 8);
+  constexpr int Z3 = ((Z >> 3) & 1) | ((Z >> 6) & 2) | ((Z >> 9) & 4) | ((Z >> 12) & 8);
+  float2 y[4][4];  // y[n2][k1]
+  y[0][0] = u[0], y[0][1] = u[4], y[0][2] = u[8], y[0][3] = u[12];
+  y[1][0] = u[1], y[1][1] = u[5], y[1][2] = u[9], y[1][3] = u[13];
+  y[2][0] = u[2], y[2][1] = u[6], y[2][2] = u[10], y[2][3] = u[14];
+  y[3][0] = u[3], y[3][1] = u[7], y[3][2] = u[11], y[3][3] = u[15];
+  dft4_z<Z0>(y[0][0], y[0][1], y[0][2], y[0][3], sign);
+  dft4_z<Z1>(y[1][0], y[1][1], y[1][2], y[1][3], sign);
+  dft4_z<Z2>(y[2][0], y[2][1], y[2][2], y[2][3], sign);
+  dft4_z<Z3>(y[3][0], y[3][1], y[3][2], y[3][3], sign);
+  constexpr bool zy1 = Z1 == 15, zy2 = Z2 == 15, zy3 = Z3 == 15, zy0 = Z0 == 15;
+  const float sg = sign < 0 ? -1.f : 1.f;
+  if constexpr (!zy1) {
+    y[1][1] = cmul(y[1][1], make_float2(c1, sg * s1));
+    y[1][2] = cmul(y[1][2], make_float2(r2, sg * r2));
+    y[1][3] = cmul(y[1][3], make_float2(s1, sg * c1));
+  }
+  if constexpr (!zy2) {
+    y[2][1] = cmul(y[2][1], make_float2(r2, sg * r2));
+    y[2][2] = mul_si(y[2][2], sign);
+    y[2][3] = cmul(y[2][3], make_float2(-r2, sg * r2));
+  }
+  if constexpr (!zy3) {
+    y[3][1] = cmul(y[3][1], make_float2(s1, sg * c1));
+    y[3][2] = cmul(y[3][2], make_float2(-r2, sg * r2));
+    y[3][3] = cmul(y[3][3], make_float2(-c1, -sg * s1));
+  }
+  constexpr int ZK = (zy0 ? 1 : 0) | (zy1 ? 2 : 0) | (zy2 ? 4 : 0) | (zy3 ? 8 : 0);
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    float2 b0 = y[0][k1], b1 = y[1][k1], b2 = y[2][k1], b3 = y[3][k1];
+    dft4_z<ZK>(b0, b1, b2, b3, sign);
+    u[k1] = b0;
+    u[k1 + 4] = b1;
+    u[k1 + 8] = b2;
+    u[k1 + 12] = b3;
+  }
+}
+
 template <int R>
 __device__ inline void dft_pow2(float2 (&u)[R], int sign) {
   if constexpr (R == 2) {

@@ -26,6 +26,8 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
 
+#include <utility>
+
 #include "sk_dft.h"
 #include "sk_internal.h"
 
@@ -96,11 +98,51 @@ struct RegTw {
   }
 };
 
+// Whether the elements i + T r of an M-point line (n = M, not Bluestein's) lie, for every lane i, in the band that
+// the antialias aperture zeroes on every grid: |k| > floor(M / 3), since the kept columns and the propagator's rows
+// satisfy |k| <= 0.3333333 n. Their values are known zeros on the way in (compact inputs, products with P) and are
+// never stored on the way out, so the code for them folds away (1024: r = 6..9; 384: r = 9..15).
+template <int M>
+__host__ __device__ constexpr bool in_band(int r) {
+  constexpr int T = M / RegPlan<M>::V, K = M / 3;
+  return r * T > K && r * T + T - 1 < M - K;
+}
+static_assert(!in_band<1024>(5) && in_band<1024>(6) && in_band<1024>(9) && !in_band<1024>(10), "1024-point band");
+static_assert(!in_band<384>(8) && in_band<384>(9) && in_band<384>(15) && !in_band<384>(16), "384-point band");
+
+// The first stage's known zeros in that band: bit j of band_mask<M, R>(q) when butterfly q's input j (element
+// i + T (q + NB j)) lies in it.
+template <int M, int R>
+__host__ __device__ constexpr int band_mask(int q) {
+  constexpr int NB = RegPlan<M>::V / R;
+  int z = 0;
+  for (int j = 0; j < R; ++j)
+    if (in_band<M>(q + NB * j)) z |= 1 << j;
+  return z;
+}
+static_assert(band_mask<1024, 16>(0) == 0x3c0, "1024: inputs 6..9");
+static_assert(band_mask<384, 8>(0) == 0x38 && band_mask<384, 8>(1) == 0x18 && band_mask<384, 8>(2) == 0x18,
+              "384: inputs 3..5 of the first butterfly, 3..4 of the others");
+
+template <int M, int R, int Q>
+__device__ __forceinline__ void band_dft(float2 (&u)[R], int sign) {
+  constexpr int Z = band_mask<M, R>(Q);
+  if constexpr (R == 16) dft16_z<Z>(u, sign);
+  else if constexpr (R == 8) dft8_z<Z>(u, sign);
+  else static_assert(R == 8 || R == 16, "a radix-8 or radix-16 first stage");
+}
+template <int M, int R, int NB, int... Q>
+__device__ __forceinline__ void band_dfts(float2 (&u)[NB][R], int sign, std::integer_sequence<int, Q...>) {
+  (band_dft<M, R, Q>(u[Q], sign), ...);
+}
+
 // One Stockham stage of radix R at sub-length L: butterfly bi = i + T q (q < V / R) reads x[bi + (M / R) j],
 // multiplies by w_{L R}^{j k} (k = bi mod L; tw = the stage's table), takes the DFT and writes y[(bi - k) R + k + L j].
 // FROM_REG: the inputs are the registers (element i + T r in v[r]); TO_REG: the outputs go to the registers.
-// HALF_IN (a radix-16 first stage): the inputs j >= 8 are zero and are not read.
-template <int M, int R, int L, int SIGN, bool FROM_REG, bool TO_REG, class Slot, bool HALF_IN = false>
+// HALF_IN (a radix-16 first stage): the inputs j >= 8 are zero and are not read. BAND (a first stage): the inputs in
+// the antialias band (in_band) are zero and are not read.
+template <int M, int R, int L, int SIGN, bool FROM_REG, bool TO_REG, class Slot, bool HALF_IN = false,
+          bool BAND = false>
 __device__ __forceinline__ void vstage(float2 (&v)[RegPlan<M>::V], float2* __restrict__ buf, int i,
                                       const float2* __restrict__ tw, Slot slot) {
   constexpr int V = RegPlan<M>::V, T = M / V, NB = V / R, STRIDE = M / R;
@@ -111,11 +153,17 @@ __device__ __forceinline__ void vstage(float2 (&v)[RegPlan<M>::V], float2* __res
     const int bi = i + T * q;
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-      if constexpr (FROM_REG) u[q][j] = (HALF_IN && j >= R / 2) ? make_float2(0.f, 0.f) : v[q + NB * j];
+      if constexpr (FROM_REG)
+        u[q][j] = ((HALF_IN && j >= R / 2) || (BAND && in_band<M>(q + NB * j))) ? make_float2(0.f, 0.f)
+                                                                                : v[q + NB * j];
       else u[q][j] = buf[slot(bi + STRIDE * j)];
     }
   }
   if constexpr (!FROM_REG) wave_sync();  // every lane has read the line before any lane overwrites it
+  if constexpr (BAND) {
+    static_assert(FROM_REG && L == 1 && !HALF_IN, "the band's zeros: a first stage from the registers");
+    band_dfts<M, R, NB>(u, SIGN, std::make_integer_sequence<int, NB>{});
+  }
 #pragma unroll
   for (int q = 0; q < NB; ++q) {
     const int bi = i + T * q;
@@ -130,7 +178,7 @@ __device__ __forceinline__ void vstage(float2 (&v)[RegPlan<M>::V], float2* __res
     if constexpr (HALF_IN) {
       static_assert(R == 16 && L == 1 && FROM_REG, "the zero upper half: a radix-16 first stage");
       dft16_half(u[q], SIGN);
-    } else {
+    } else if constexpr (!BAND) {  // BAND: transformed above
       dft_r<R>(u[q], SIGN);
     }
 #pragma unroll
@@ -143,11 +191,11 @@ __device__ __forceinline__ void vstage(float2 (&v)[RegPlan<M>::V], float2* __res
 }
 
 // Unscaled transform of the line in the registers (SIGN -1 forward, +1 inverse). HALF_IN: the elements of the upper
-// half of the line (r >= V / 2 in every lane) are zero.
-template <int M, int SIGN, class Slot, bool HALF_IN = false>
+// half of the line (r >= V / 2 in every lane) are zero. BAND: the elements in the antialias band (in_band) are zero.
+template <int M, int SIGN, class Slot, bool HALF_IN = false, bool BAND = false>
 __device__ __forceinline__ void vfft(float2 (&v)[RegPlan<M>::V], float2* buf, int i, const float2* tab, Slot slot) {
   using P = RegPlan<M>;
-  vstage<M, P::R1, 1, SIGN, true, false, Slot, HALF_IN>(v, buf, i, nullptr, slot);
+  vstage<M, P::R1, 1, SIGN, true, false, Slot, HALF_IN, BAND>(v, buf, i, nullptr, slot);
   vstage<M, P::R2, P::R1, SIGN, false, false>(v, buf, i, tab, slot);
   vstage<M, P::R3, P::R1 * P::R2, SIGN, false, true>(v, buf, i, tab + RegTw<M>::S2, slot);
 }
@@ -155,11 +203,13 @@ __device__ __forceinline__ void vfft(float2 (&v)[RegPlan<M>::V], float2* buf, in
 // The line transform of logical length n: the M-point transform itself, or (BLUE, M = 1024) Bluestein's for n <= 512
 // with the chirp cs[n] and the kernel transform ks[1024] (both in LDS; conjugated for the inverse). Values at
 // elements e >= n are ignored on entry and undefined on return.
-template <int M, int SIGN, bool BLUE, class Slot>
+// BAND (not Bluestein's): the input is zero in the antialias band.
+template <int M, int SIGN, bool BLUE, class Slot, bool BAND = false>
 __device__ __forceinline__ void line_fft(float2 (&v)[RegPlan<M>::V], float2* buf, int i, const float2* tab,
                                          const float2* cs, const float2* ks, int n, Slot slot) {
+  static_assert(!(BAND && BLUE), "the band's zeros are known at compile time for M-point lines only");
   if constexpr (!BLUE) {
-    vfft<M, SIGN>(v, buf, i, tab, slot);
+    vfft<M, SIGN, Slot, false, BAND>(v, buf, i, tab, slot);
   } else {
     constexpr int V = RegPlan<M>::V, T = M / V;
 #pragma unroll
@@ -194,18 +244,6 @@ __device__ __forceinline__ void line_fft(float2 (&v)[RegPlan<M>::V], float2* buf
     }
   }
 }
-
-// Whether the elements i + T r of an M-point line (n = M, not Bluestein's) lie, for every lane i, in the band that
-// the antialias aperture zeroes on every grid: |k| > floor(M / 3), since the kept columns and the propagator's rows
-// satisfy |k| <= 0.3333333 n. Their values are known zeros on the way in (compact inputs, products with P) and are
-// never stored on the way out, so the code for them folds away (1024: r = 6..9; 384: r = 9..15).
-template <int M>
-__host__ __device__ constexpr bool in_band(int r) {
-  constexpr int T = M / RegPlan<M>::V, K = M / 3;
-  return r * T > K && r * T + T - 1 < M - K;
-}
-static_assert(!in_band<1024>(5) && in_band<1024>(6) && in_band<1024>(9) && !in_band<1024>(10), "1024-point band");
-static_assert(!in_band<384>(8) && in_band<384>(9) && in_band<384>(15) && !in_band<384>(16), "384-point band");
 
 constexpr int kRegPB = 8;      // positions per block (t or P loaded once per 8 positions)
 constexpr int kRowWaves = 2;   // wavefronts per row-pass block
@@ -280,7 +318,11 @@ __global__ void __launch_bounds__(64 * kRowWaves) row_reg_kernel(RowArgs a, cons
         v[r] = e < n ? g[e] : make_float2(0.f, 0.f);
       }
     }
-    if constexpr ((FLAGS & ROW_INV) != 0) line_fft<M, +1, BLUE>(v, buf, il, tab, cs, ks, n, slot);
+    if constexpr ((FLAGS & ROW_INV) != 0) {
+      // from the compact wave, the band's elements are zero
+      constexpr bool band = !BLUE && (FLAGS & ROW_IN_COMPACT) != 0;
+      line_fft<M, +1, BLUE, decltype(slot), band>(v, buf, il, tab, cs, ks, n, slot);
+    }
     if constexpr ((FLAGS & ROW_WRITE_PLANE) != 0) {
       if (live) {
         float2* pl = a.plane + b * a.plane_stride + xs * n;
@@ -488,7 +530,9 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
       }
       continue;  // nothing is stored; the next position's block phase waits at the loop's first barrier
     } else {
-      line_fft<M, +1, BLUE>(v, col, il, tab, cs, ks, n, slot);
+      // after the product with P the band's elements are zero (not in the first pass, whose generated probe
+      // spectrum need not be)
+      line_fft<M, +1, BLUE, decltype(slot), !BLUE && MODE == 0>(v, col, il, tab, cs, ks, n, slot);
     }
 #pragma unroll
     for (int r = 0; r < V; ++r) {

@@ -209,3 +209,55 @@ def test_the_folded_band_holds_no_kept_frequency_on_any_grid(M):
     # and the band is as wide as that argument allows: the element just outside it can be kept
     r0 = min(r for r in range(V) if _in_band(M, r))
     assert not _in_band(M, r0 - 1) and (r0 - 1) * T + T - 1 >= M // 3
+
+
+def _dft4_z(a, Z, sign):
+    """dft4_z (sk_dft.h): known-zero inputs (bit j of Z) are not read."""
+    z = [(Z >> j) & 1 for j in range(4)]
+    a = [0j if z[j] else a[j] for j in range(4)]
+    t0, t1, t2, t3 = a[0] + a[2], a[0] - a[2], a[1] + a[3], 1j * sign * (a[1] - a[3])
+    return [t0 + t2, t1 + t3, t0 - t2, t1 - t3]
+
+
+def _dft8_z(u, Z, sign):
+    ZE = (Z & 1) | ((Z >> 1) & 2) | ((Z >> 2) & 4) | ((Z >> 3) & 8)
+    ZO = ((Z >> 1) & 1) | ((Z >> 2) & 2) | ((Z >> 3) & 4) | ((Z >> 4) & 8)
+    e = _dft4_z([u[0], u[2], u[4], u[6]], ZE, sign)
+    o = _dft4_z([u[1], u[3], u[5], u[7]], ZO, sign)
+    w = [o[k] * np.exp(sign * 1j * np.pi * k / 4) for k in range(4)]
+    return [e[k] + w[k] for k in range(4)] + [e[k] - w[k] for k in range(4)]
+
+
+def _dft16_z(u, Z, sign):
+    masks = [sum(((Z >> (n2 + 4 * j)) & 1) << j for j in range(4)) for n2 in range(4)]
+    y = [_dft4_z([u[n2], u[4 + n2], u[8 + n2], u[12 + n2]], masks[n2], sign) for n2 in range(4)]
+    w16 = np.exp(sign * 2j * np.pi / 16)
+    y = [[y[n2][k1] * w16 ** (n2 * k1) for k1 in range(4)] for n2 in range(4)]
+    out = np.zeros(16, complex)
+    for k1 in range(4):
+        b = _dft4_z([y[0][k1], y[1][k1], y[2][k1], y[3][k1]], 0, sign)
+        for k2 in range(4):
+            out[k1 + 4 * k2] = b[k2]
+    return out
+
+
+@pytest.mark.parametrize("R,Z", [(8, 0x38), (8, 0x18), (8, 0), (8, 0xff), (16, 0x3c0), (16, 0xff00), (16, 0)])
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_dfts_with_known_zero_inputs_equal_numpy(R, Z, sign):
+    # the first-stage masks of the band (384: inputs 3..5 of butterfly 0, 3..4 of the others; 1024: inputs 6..9),
+    # no zeros and all zeros; the inputs flagged zero hold garbage, which the masked transforms must not read
+    rng = np.random.default_rng(R + Z)
+    u = rng.normal(size=R) + 1j * rng.normal(size=R)
+    clean = np.array([0j if (Z >> j) & 1 else u[j] for j in range(R)])
+    want = np.fft.fft(clean) if sign < 0 else np.fft.ifft(clean) * R
+    got = np.array(_dft8_z(u, Z, sign) if R == 8 else _dft16_z(u, Z, sign))
+    assert np.abs(got - want).max() <= 1e-12 * max(1.0, np.abs(want).max())
+
+
+def test_band_masks_match_the_band():
+    for M, R in ((384, 8), (1024, 16)):
+        V = PLANS[M][0]
+        NB = V // R
+        for q in range(NB):
+            z = sum(1 << j for j in range(R) if _in_band(M, q + NB * j))
+            assert z == {(384, 0): 0x38, (384, 1): 0x18, (384, 2): 0x18, (1024, 0): 0x3c0}[(M, q)]
