@@ -823,8 +823,9 @@ void row_launch(const RowArgs& a, int batch, hipStream_t s, const float2* tw) {
 // keeps both blocks of every span on one XCD.
 template <int F, bool LANES = false>
 int span_count(long long units) {
-  static int resident = 0;  // blocks of this kernel the device holds at once
-  if (resident == 0) {
+  // blocks of this kernel the device holds at once; a static initialised once, under the language's guarantee, since
+  // a graph-replayed scan launches its first batch on one host thread while a second records the same launches
+  static const int resident = [] {
     int dev = 0, cus = 0, occ = 0;
     SK_HIP(hipGetDevice(&dev));
     SK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -832,8 +833,8 @@ int span_count(long long units) {
       SK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, col512_lanes_kernel<F>, 8 * 64, 0));
     else
       SK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, col512_span_kernel<F>, 8 * 64, 0));
-    resident = std::max(2, cus * std::max(1, occ));
-  }
+    return std::max(2, cus * std::max(1, occ));
+  }();
   long long r = std::min<long long>(units, resident / 2);
   if (r >= 8) r -= r % 8;
   return (int)std::max<long long>(1, r);
