@@ -377,12 +377,24 @@ __device__ __forceinline__ int tile_elem(int e, int c) {
   else return rslot(e);
 }
 
-// Column pass: block = TC compact columns [TC blockIdx.x, + TC) of positions [b0, b0 + kRegPB); TC x T threads, the T
-// lanes of column c = tid / T. MODE 0: a middle slice (COL_IN_G | COL_FWD | COL_MUL_P | COL_INV | COL_OUT); 1: the
+// The (tile, position group) unit of a column-pass block. The device deals work-groups out round-robin over its 8
+// XCDs, each with its own L2, in dispatch order L = blockIdx.x + gridDim.x blockIdx.y. Taken in that order, block L
+// runs unit (L mod 8) U / 8 + L / 8 of the U = gridDim.x gridDim.y units (tile fastest) when 8 divides U: each XCD
+// walks consecutive tiles of its own position groups, so tiles whose row segments (TC values, 32 or 64 bytes) make
+// up the same 128-byte lines are read through one L2 rather than fetched by two or four XCDs.
+__device__ __forceinline__ int2 xcd_unit() {
+  const int gx = gridDim.x, units = gx * gridDim.y;
+  const int L = blockIdx.x + gx * blockIdx.y;
+  const int u = units % 8 == 0 ? (L % 8) * (units / 8) + L / 8 : L;
+  return make_int2(u % gx, u / gx);
+}
+
+// Column pass: block = TC compact columns [TC tile, + TC) of positions [b0, b0 + kRegPB) of its unit (xcd_unit);
+// TC x T threads, the T lanes of column c = tid / T. MODE 0: a middle slice (COL_IN_G | COL_FWD | COL_MUL_P | COL_INV | COL_OUT); 1: the
 // first pass, whose input is the probe spectrum times the position's ramps, generated in the block phase
 // (COL_GENERATE | COL_INV | COL_OUT); 2: the last pass, which ends in the detectors (COL_IN_G | COL_FWD | COL_MUL_P |
 // COL_DETECT): each thread sums its elements' contributions to one detector slot at a time, pixels in a fixed order,
-// then a tree over the block writes the block's partial sum (partial[position][slot][blockIdx.x]).
+// then a tree over the block writes the block's partial sum (partial[position][slot][tile]).
 template <int M, bool BLUE, int TC, int MODE>
 __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColArgs a, const float2* __restrict__ roots,
                                                                        const float2* __restrict__ chirp,
@@ -402,7 +414,8 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
     for (int e = tid; e < n; e += NT) cs[e] = chirp[e];
     for (int e = tid; e < 1024; e += NT) ks[e] = khat[e];
   }
-  const int c0 = blockIdx.x * TC;
+  const int2 unit = xcd_unit();
+  const int c0 = unit.x * TC;
   // block phases: thread -> column cb of the tile, rows xi + RSTEP r
   const int cb = tid % TC, xi = tid / TC;
   // column lanes: column wc, lane i
@@ -423,7 +436,7 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
   }
   const int iyb = a.keep[c0 + cb];
   const bool kept = iyb >= 0;  // padding columns are never read nor written
-  const int b0 = blockIdx.y * kRegPB;
+  const int b0 = unit.y * kRegPB;
   // The block phase's values of the next position are loaded into registers while this position transforms: the
   // loads' latency then overlaps the arithmetic (measured: 1024 points, 8 columns, 6.76 -> 3.95 ms per launch of
   // 1024 positions; 384 points 3.03 -> 2.28 ms per 4096).
@@ -526,7 +539,7 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
       if (tid < d.nslots) {
         double t = 0.0;
         for (int wv = 0; wv < NT / 64; ++wv) t += red[wv * kMaxSlots + tid];
-        d.partial[(b * d.nslots + tid) * d.blocks_per_wave + blockIdx.x] = t;
+        d.partial[(b * d.nslots + tid) * d.blocks_per_wave + unit.x] = t;
       }
       continue;  // nothing is stored; the next position's block phase waits at the loop's first barrier
     } else {

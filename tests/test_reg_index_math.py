@@ -261,3 +261,27 @@ def test_band_masks_match_the_band():
         for q in range(NB):
             z = sum(1 << j for j in range(R) if _in_band(M, q + NB * j))
             assert z == {(384, 0): 0x38, (384, 1): 0x18, (384, 2): 0x18, (1024, 0): 0x3c0}[(M, q)]
+
+
+def _xcd_unit(bx, by, gx, gy):
+    """The column pass's (tile, position group) of block (bx, by) in a gx x gy grid (xcd_unit in sk_reg.hip)."""
+    units = gx * gy
+    L = bx + gx * by
+    u = (L % 8) * (units // 8) + L // 8 if units % 8 == 0 else L
+    return u % gx, u // gx
+
+
+@pytest.mark.parametrize("gx,gy", [(34, 512), (86, 512), (88, 512), (43, 128), (22, 1), (17, 3), (5, 8), (1, 1)])
+def test_column_units_cover_every_tile_and_position_group_once(gx, gy):
+    seen = {}
+    for by in range(gy):
+        for bx in range(gx):
+            seen.setdefault(_xcd_unit(bx, by, gx, gy), []).append((bx, by))
+    assert set(seen) == {(x, y) for y in range(gy) for x in range(gx)}
+    assert all(len(v) == 1 for v in seen.values())
+    if (gx * gy) % 8 == 0:
+        # the blocks that one XCD receives (dispatch order L = 8 k + xcd) run consecutive units, tile fastest
+        for xcd in range(8):
+            order = [_xcd_unit(L % gx, L // gx, gx, gy) for L in range(xcd, gx * gy, 8)]
+            flat = [x + gx * y for x, y in order]
+            assert flat == list(range(flat[0], flat[0] + len(flat)))
