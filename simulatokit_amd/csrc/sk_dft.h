@@ -1,4 +1,5 @@
-// sk_dft.h — small unscaled DFTs (sign -1 forward, +1 inverse) shared by the generic and the fast passes.
+// sk_dft.h — small unscaled DFTs (sign -1 forward, +1 inverse) and the column passes' block order, shared by the
+// generic, fast and register passes.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -6,6 +7,19 @@
 #include "sk_internal.h"
 
 namespace sk {
+
+// The (tile, position) unit of a column-pass block in a 2-D grid of tiles x positions (or position groups). The
+// device deals work-groups out round-robin over its 8 XCDs, each with its own L2, in dispatch order L = blockIdx.x +
+// gridDim.x blockIdx.y. Taken in that order, block L runs unit (L mod 8) U / 8 + L / 8 of the U = gridDim.x gridDim.y
+// units (tile fastest) when 8 divides U: each XCD walks consecutive tiles of its own positions, so tiles whose row
+// segments (a few columns, 16 to 64 bytes) make up the same 128-byte lines are read through one L2 rather than
+// fetched by two or more XCDs.
+__device__ __forceinline__ int2 xcd_unit() {
+  const int gx = gridDim.x, units = gx * gridDim.y;
+  const int L = blockIdx.x + gx * blockIdx.y;
+  const int u = units % 8 == 0 ? (L % 8) * (units / 8) + L / 8 : L;
+  return make_int2(u % gx, u / gx);
+}
 
 __device__ inline float2 mul_si(float2 a, int sign) {  // a * (sign * i)
   return sign < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);

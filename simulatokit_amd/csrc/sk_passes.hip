@@ -425,8 +425,9 @@ __global__ void __launch_bounds__(kGenericThreads) col_generic_kernel(ColArgs a,
   float2* A = lds;
   float2* B = lds + lds_entries(cols * ls, MF == 0);
   const int tid = threadIdx.x, T = blockDim.x;
-  const int c0 = blockIdx.x * cols;
-  const long long b = blockIdx.y;
+  const int2 unit = xcd_unit();  // tiles of one position on one XCD: their row segments share 128-byte lines
+  const int c0 = unit.x * cols;
+  const long long b = unit.y;
   const int f = a.flags;
   const long long gw = (long long)a.nx * a.nc;
   const Div dcols(cols);
@@ -466,7 +467,7 @@ __global__ void __launch_bounds__(kGenericThreads) col_generic_kernel(ColArgs a,
     // accumulators after the slots
     const long long pos = first_position(a.pos0, a.pos_dev) + b;
     double* scratch = reinterpret_cast<double*>(oth);
-    double* out = a.det.partial + (b * a.det.nslots) * a.det.blocks_per_wave + blockIdx.x;
+    double* out = a.det.partial + (b * a.det.nslots) * a.det.blocks_per_wave + unit.x;
     for (int q = 0; q < a.det.nslots; ++q) {
       double s = 0.0;
       for (int idx = tid; idx < cols * a.nx; idx += T) {

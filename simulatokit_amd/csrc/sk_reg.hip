@@ -377,18 +377,6 @@ __device__ __forceinline__ int tile_elem(int e, int c) {
   else return rslot(e);
 }
 
-// The (tile, position group) unit of a column-pass block. The device deals work-groups out round-robin over its 8
-// XCDs, each with its own L2, in dispatch order L = blockIdx.x + gridDim.x blockIdx.y. Taken in that order, block L
-// runs unit (L mod 8) U / 8 + L / 8 of the U = gridDim.x gridDim.y units (tile fastest) when 8 divides U: each XCD
-// walks consecutive tiles of its own position groups, so tiles whose row segments (TC values, 32 or 64 bytes) make
-// up the same 128-byte lines are read through one L2 rather than fetched by two or four XCDs.
-__device__ __forceinline__ int2 xcd_unit() {
-  const int gx = gridDim.x, units = gx * gridDim.y;
-  const int L = blockIdx.x + gx * blockIdx.y;
-  const int u = units % 8 == 0 ? (L % 8) * (units / 8) + L / 8 : L;
-  return make_int2(u % gx, u / gx);
-}
-
 // Column pass: block = TC compact columns [TC tile, + TC) of positions [b0, b0 + kRegPB) of its unit (xcd_unit);
 // TC x T threads, the T lanes of column c = tid / T. MODE 0: a middle slice (COL_IN_G | COL_FWD | COL_MUL_P | COL_INV | COL_OUT); 1: the
 // first pass, whose input is the probe spectrum times the position's ramps, generated in the block phase
