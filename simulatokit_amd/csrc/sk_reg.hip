@@ -57,7 +57,9 @@ struct RegPlan<384> {
 //   against 1.51 ms), and helps only the Bluestein column pass (10.30 against 13.30 ms).
 // - The column tile (TC columns per block), with the next position's loads in flight during the transforms: 384,
 //   8 columns (2.28 against 2.32 ms for 16); 1024, 8 columns (3.95 against 5.36 ms for 4); Bluestein, 4 columns
-//   (9.27 against 10.61 ms for 8). Blocks of 1, 2 or 4 row wavefronts differ by under 2 %.
+//   (9.27 against 10.61 ms for 8). With the blocks in XCD order (xcd_unit), per 4096 positions at every length:
+//   384, 2.02-2.07 against 2.22-2.25 ms for 16; 1024, 13.69-13.71 against 14.91 ms for 4; Bluestein 509,
+//   8.21-8.23 against 8.85-8.86 ms for 8. Blocks of 1, 2 or 4 row wavefronts differ by under 2 %.
 template <int M, bool BLUE>
 struct RegTuning {
   static constexpr bool T_REGS = M == 1024, ROW_OPAQUE = false, COL_OPAQUE = BLUE;
