@@ -18,10 +18,11 @@
 //   row pass:    blocks of 2 wavefronts; each line is one row x of one position, IFFT_y -> * t(x, y) -> FFT_y (and
 //                the other row modes of the fast passes), the block's rows for 8 positions in turn (RegTuning: t in
 //                registers or read per position)
-//   column pass: the middle slices (FFT_x -> * P -> IFFT_x); a block moves TC compact columns of one position between
-//                HBM and LDS in row segments of TC values, each column's lanes transform it inside the block's LDS,
-//                P in registers for the block's 8 positions
-// The probe-generating first and the detecting last column pass of a scan run the generic passes.
+//   column pass: FFT_x -> * P -> IFFT_x (middle slices), the probe-generating first pass and the detecting last one;
+//                a block moves TC compact columns of one position between HBM and LDS in row segments of TC values,
+//                each column's lanes transform it inside the block's LDS, P in registers for the block's 8 positions.
+//                At 384 and 1024 points the block phases also run the first stage of the first transform (from the
+//                values they load) and the last stage of the inverse (into the values they store).
 
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
@@ -54,7 +55,10 @@ struct RegPlan<384> {
 //   Bluestein 509: 10.29 against 11.42 ms holding.
 // - OPAQUE: the lane index is made opaque to the compiler per position, so the lane's LDS slot addresses are
 //   recomputed in each position instead of held in registers across the loop. It costs the row passes (384: 2.53
-//   against 1.51 ms), and helps only the Bluestein column pass (10.30 against 13.30 ms).
+//   against 1.51 ms). In the column passes it helped the Bluestein pass (10.30 against 13.30 ms); since the block
+//   phases run the first and last transform stages (whose lane index is opaque too), it keeps the 384-point middle
+//   pass at 255 VGPRs (two wavefronts per SIMD, against one with 256 VGPRs and 52 AGPRs) and the 1024-point one at
+//   176 VGPRs (202 without): 384, 1.57 ms against 2.09-2.11 ms before the fused stages; 1024, 13.69 against 13.7.
 // - The column tile (TC columns per block), with the next position's loads in flight during the transforms: 384,
 //   8 columns (2.28 against 2.32 ms for 16); 1024, 8 columns (3.95 against 5.36 ms for 4); Bluestein, 4 columns
 //   (9.27 against 10.61 ms for 8). With the blocks in XCD order (xcd_unit), per 4096 positions at every length:
@@ -62,7 +66,7 @@ struct RegPlan<384> {
 //   8.21-8.23 against 8.85-8.86 ms for 8. Blocks of 1, 2 or 4 row wavefronts differ by under 2 %.
 template <int M, bool BLUE>
 struct RegTuning {
-  static constexpr bool T_REGS = M == 1024, ROW_OPAQUE = false, COL_OPAQUE = BLUE;
+  static constexpr bool T_REGS = M == 1024, ROW_OPAQUE = false, COL_OPAQUE = true;
   static constexpr int COL_TILE = BLUE ? 4 : 8;
 };
 
@@ -190,6 +194,29 @@ __device__ __forceinline__ void vstage(float2 (&v)[RegPlan<M>::V], float2* __res
     }
   }
   if constexpr (!TO_REG) wave_sync();
+}
+
+// The last stage of the inverse transform (radix R3 at sub-length M / R3, whose outputs land where its inputs were)
+// for lane i of a line in LDS, one butterfly at a time, each butterfly's outputs (elements bi + (M / R3) j) stored
+// straight to out[element * stride]: the values of one butterfly are live at a time, not the lane's V.
+template <int M, class Slot>
+__device__ __forceinline__ void last_stage_store(const float2* __restrict__ buf, int i, const float2* __restrict__ tw,
+                                                 Slot slot, float2* out, long long stride) {
+  using P = RegPlan<M>;
+  constexpr int V = P::V, T = M / V, R = P::R3, L = P::R1 * P::R2, NB = V / R, STRIDE = M / R;
+  static_assert(L * R == M, "the last stage");
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int bi = i + T * q;  // k = bi mod L = bi
+    float2 u[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) u[j] = buf[slot(bi + STRIDE * j)];
+#pragma unroll
+    for (int j = 1; j < R; ++j) u[j] = cmulc(u[j], tw[(j - 1) * L + bi]);
+    dft_r<R>(u, +1);
+#pragma unroll
+    for (int j = 0; j < R; ++j) out[(long long)(bi + STRIDE * j) * stride] = u[j];
+  }
 }
 
 // Unscaled transform of the line in the registers (SIGN -1 forward, +1 inverse). HALF_IN: the elements of the upper
@@ -380,9 +407,9 @@ __device__ __forceinline__ int tile_elem(int e, int c) {
 }
 
 // Column pass: block = TC compact columns [TC tile, + TC) of positions [b0, b0 + kRegPB) of its unit (xcd_unit);
-// TC x T threads, the T lanes of column c = tid / T. MODE 0: a middle slice (COL_IN_G | COL_FWD | COL_MUL_P | COL_INV | COL_OUT); 1: the
-// first pass, whose input is the probe spectrum times the position's ramps, generated in the block phase
-// (COL_GENERATE | COL_INV | COL_OUT); 2: the last pass, which ends in the detectors (COL_IN_G | COL_FWD | COL_MUL_P |
+// TC x T threads, the T lanes of column c = tid / T. MODE 0: a middle slice (COL_IN_G | COL_FWD | COL_MUL_P | COL_INV |
+// COL_OUT); 1: the first pass, whose input is the probe spectrum times the position's ramps, generated in the block
+// phase (COL_GENERATE | COL_INV | COL_OUT); 2: the last pass, which ends in the detectors (COL_IN_G | COL_FWD | COL_MUL_P |
 // COL_DETECT): each thread sums its elements' contributions to one detector slot at a time, pixels in a fixed order,
 // then a tree over the block writes the block's partial sum (partial[position][slot][tile]).
 template <int M, bool BLUE, int TC, int MODE>
@@ -448,18 +475,41 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
       }
     }
   };
+  // FUSE (not Bluestein's): a block-phase thread (column cb, rows xi + T r) holds what lane xi of column cb holds
+  // for the line's first transform stage (RSTEP = T), so the block phase runs that stage from its loaded values into
+  // the tile, and the last stage of the inverse from the tile into the values it stores: the columns' own lanes run
+  // the stages between, and two of the tile's passes through LDS per position (the raw values in and out) are gone.
+  constexpr bool FUSE = !BLUE;
+  static_assert(!FUSE || (RSTEP == T && NL == V), "the block phase's rows are a lane's elements");
+  float2* bcol = tile + cb * CS;
+  auto bslot = [cb](int e) { return tile_elem<M, TC>(e, cb); };
   if (b0 < batch) prefetch(b0);
   for (int p = 0; p < kRegPB; ++p) {
     const long long b = b0 + p;
     if (b >= batch) break;  // uniform over the block
     float2* g = a.G + b * gw + c0 + cb;
     __syncthreads();  // the tile's previous position is stored (and, first, the tables are built)
+    // the block phase's lane index, opaque to the compiler per position: its LDS slot addresses are recomputed here
+    // rather than held in registers across the loop
+    int xl = xi;
+    asm volatile("" : "+v"(xl));
+    if constexpr (FUSE) {
+      if constexpr (MODE == 1) {
+        float2 gv[V];
 #pragma unroll
-    for (int r = 0; r < NL; ++r) {
-      const int x = xi + RSTEP * r;
-      if (x < n) {
-        if constexpr (MODE == 1) tile[cb * CS + tile_elem<M, TC>(x, cb)] = generate(b, x);
-        else tile[cb * CS + tile_elem<M, TC>(x, cb)] = pre[r];
+        for (int r = 0; r < V; ++r) gv[r] = generate(b, xl + T * r);
+        vstage<M, P::R1, 1, +1, true, false>(gv, bcol, xl, nullptr, bslot);
+      } else {
+        vstage<M, P::R1, 1, -1, true, false>(pre, bcol, xl, nullptr, bslot);
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < NL; ++r) {
+        const int x = xi + RSTEP * r;
+        if (x < n) {
+          if constexpr (MODE == 1) tile[cb * CS + tile_elem<M, TC>(x, cb)] = generate(b, x);
+          else tile[cb * CS + tile_elem<M, TC>(x, cb)] = pre[r];
+        }
       }
     }
     if (p + 1 < kRegPB && b + 1 < batch) prefetch(b + 1);
@@ -467,14 +517,22 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
     int il = i;  // COL_OPAQUE: as ROW_OPAQUE in the row pass
     if constexpr (RegTuning<M, BLUE>::COL_OPAQUE) asm volatile("" : "+v"(il));
     float2 v[V];
+    if constexpr (FUSE) {
+      if constexpr (MODE != 1) {
+        // the forward transform's second and last stages
+        vstage<M, P::R2, P::R1, -1, false, false>(v, col, il, tab, slot);
+        vstage<M, P::R3, P::R1 * P::R2, -1, false, true>(v, col, il, tab + RegTw<M>::S2, slot);
+      }
+    } else {
 #pragma unroll
-    for (int r = 0; r < V; ++r) {
-      const int e = il + T * r;
-      v[r] = e < n ? col[slot(e)] : make_float2(0.f, 0.f);
+      for (int r = 0; r < V; ++r) {
+        const int e = il + T * r;
+        v[r] = e < n ? col[slot(e)] : make_float2(0.f, 0.f);
+      }
+      wave_sync();  // every lane has read its column before the exchanges overwrite it
+      if constexpr (MODE != 1) line_fft<M, -1, BLUE>(v, col, il, tab, cs, ks, n, slot);
     }
-    wave_sync();  // every lane has read its column before the exchanges overwrite it
     if constexpr (MODE != 1) {
-      line_fft<M, -1, BLUE>(v, col, il, tab, cs, ks, n, slot);
 #pragma unroll
       for (int r = 0; r < V; ++r) {
         if (!BLUE && in_band<M>(r)) v[r] = make_float2(0.f, 0.f);  // P is zero there
@@ -532,19 +590,29 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
         d.partial[(b * d.nslots + tid) * d.blocks_per_wave + unit.x] = t;
       }
       continue;  // nothing is stored; the next position's block phase waits at the loop's first barrier
+    } else if constexpr (FUSE) {
+      // the inverse transform's stages before the last: after the product with P the band's elements are zero (not
+      // in the first pass, whose generated probe spectrum need not be, and whose first stage ran in the block phase)
+      if constexpr (MODE == 0)
+        vstage<M, P::R1, 1, +1, true, false, decltype(slot), false, true>(v, col, il, nullptr, slot);
+      vstage<M, P::R2, P::R1, +1, false, false>(v, col, il, tab, slot);
+      __syncthreads();
+      // the last stage in the block phase: thread (cb, xi) runs lane xi's butterflies of column cb and stores their
+      // outputs
+      int xs = xi;
+      asm volatile("" : "+v"(xs));
+      if (kept) last_stage_store<M>(bcol, xs, tab + RegTw<M>::S2, bslot, g, a.nc);
     } else {
-      // after the product with P the band's elements are zero (not in the first pass, whose generated probe
-      // spectrum need not be)
       line_fft<M, +1, BLUE, decltype(slot), !BLUE && MODE == 0>(v, col, il, tab, cs, ks, n, slot);
-    }
 #pragma unroll
-    for (int r = 0; r < V; ++r) {
-      const int e = il + T * r;
-      if (e < n) col[slot(e)] = v[r];
+      for (int r = 0; r < V; ++r) {
+        const int e = il + T * r;
+        if (e < n) col[slot(e)] = v[r];
+      }
+      __syncthreads();
+      if (kept)
+        for (int x = xi; x < n; x += RSTEP) g[(long long)x * a.nc] = tile[cb * CS + tile_elem<M, TC>(x, cb)];
     }
-    __syncthreads();
-    if (kept)
-      for (int x = xi; x < n; x += RSTEP) g[(long long)x * a.nc] = tile[cb * CS + tile_elem<M, TC>(x, cb)];
   }
 }
 
