@@ -63,7 +63,9 @@ struct RegPlan<384> {
 //   8 columns (2.28 against 2.32 ms for 16); 1024, 8 columns (3.95 against 5.36 ms for 4); Bluestein, 4 columns
 //   (9.27 against 10.61 ms for 8). With the blocks in XCD order (xcd_unit), per 4096 positions at every length:
 //   384, 2.02-2.07 against 2.22-2.25 ms for 16; 1024, 13.69-13.71 against 14.91 ms for 4; Bluestein 509,
-//   8.21-8.23 against 8.85-8.86 ms for 8. Blocks of 1, 2 or 4 row wavefronts differ by under 2 %.
+//   8.21-8.23 against 8.85-8.86 ms for 8; with the fused first and last stages, 1024 at 8 columns 13.67-13.77 ms
+//   against 15.24 for 4, and 17.64 for 4 held to three wavefronts per SIMD (168 VGPRs, 48 bytes spilled). Blocks of
+//   1, 2 or 4 row wavefronts differ by under 2 %.
 template <int M, bool BLUE>
 struct RegTuning {
   static constexpr bool T_REGS = M == 1024, ROW_OPAQUE = false, COL_OPAQUE = true;
