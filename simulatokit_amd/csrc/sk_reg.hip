@@ -56,9 +56,11 @@ struct RegPlan<384> {
 // - OPAQUE: the lane index is made opaque to the compiler per position, so the lane's LDS slot addresses are
 //   recomputed in each position instead of held in registers across the loop. It costs the row passes (384: 2.53
 //   against 1.51 ms). In the column passes it helped the Bluestein pass (10.30 against 13.30 ms); since the block
-//   phases run the first and last transform stages (whose lane index is opaque too), it keeps the 384-point middle
-//   pass at 255 VGPRs (two wavefronts per SIMD, against one with 256 VGPRs and 52 AGPRs) and the 1024-point one at
-//   176 VGPRs (202 without): 384, 1.57 ms against 2.09-2.11 ms before the fused stages; 1024, 13.69 against 13.7.
+//   phases run the first and last transform stages (whose lane index is opaque in every pass), it keeps the 384-point
+//   middle pass at 255 VGPRs (two wavefronts per SIMD, against one with 256 VGPRs and 52 AGPRs): 1.57 ms against
+//   2.09-2.11 ms before the fused stages. At 1024 points it would save registers that buy no occupancy (176 against
+//   202 VGPRs, two wavefronts per SIMD either way) for 30 % more vector instructions: 13.03-13.07 ms without it,
+//   13.72-13.78 with it.
 // - The column tile (TC columns per block), with the next position's loads in flight during the transforms: 384,
 //   8 columns (2.28 against 2.32 ms for 16); 1024, 8 columns (3.95 against 5.36 ms for 4); Bluestein, 4 columns
 //   (9.27 against 10.61 ms for 8). With the blocks in XCD order (xcd_unit), per 4096 positions at every length:
@@ -68,7 +70,7 @@ struct RegPlan<384> {
 //   1, 2 or 4 row wavefronts differ by under 2 %.
 template <int M, bool BLUE>
 struct RegTuning {
-  static constexpr bool T_REGS = M == 1024, ROW_OPAQUE = false, COL_OPAQUE = true;
+  static constexpr bool T_REGS = M == 1024, ROW_OPAQUE = false, COL_OPAQUE = M == 384 || BLUE;
   static constexpr int COL_TILE = BLUE ? 4 : 8;
 };
 
