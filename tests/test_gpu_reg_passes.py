@@ -23,8 +23,10 @@ def device():
         pytest.fail("no HIP device: " + sk._native.last_error())
 
 
-# (nx, ny, tiles, tz): 384 = 8 8 6 on both axes; 1024; 509 by Bluestein on 1024 points; mixed axes
-GRIDS = [(384, 384, 8, 5), (1024, 1024, 8, 3), (509, 509, 8, 5), (509, 384, 8, 4), (300, 1024, 4, 3)]
+# (nx, ny, tiles, tz): 384 = 8 8 6 on both axes; 1024; 509 by Bluestein on 1024 points; mixed axes (the column
+# pass's tiles then span the other length's kept columns)
+GRIDS = [(384, 384, 8, 5), (1024, 1024, 8, 3), (509, 509, 8, 5), (509, 384, 8, 4), (300, 1024, 4, 3),
+         (1024, 384, 8, 3), (384, 1024, 8, 3)]
 
 
 @pytest.mark.parametrize("nx,ny,tiles,tz", GRIDS)
