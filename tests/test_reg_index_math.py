@@ -5,7 +5,9 @@
   k = bi mod L, writes y[(bi - k) R + k + L j]; first stage from the registers, last into them), equal numpy's FFT;
 - Bluestein's transform of n points on 1024 (chirp, forward transform, kernel product, inverse, chirp) equals it;
 - the LDS slot map e -> e ^ (((e >> 3) ^ (e >> 7)) & 31) is a permutation of each line, and a bank model (64 banks of
-  4 bytes, a float2 access served in groups of 32 lanes) finds every exchange of both lengths free of conflicts.
+  4 bytes, a float2 access served in groups of 32 lanes) finds every exchange of both lengths free of conflicts;
+- the column pass with its first and last stages in the block phases, run through the tile's slot map, equals
+  IFFT(FFT(g) P) per column, and the XCD-ordered block numbering runs every (tile, position group) unit once.
 """
 import numpy as np
 import pytest
@@ -285,3 +287,78 @@ def test_column_units_cover_every_tile_and_position_group_once(gx, gy):
             order = [_xcd_unit(L % gx, L // gx, gx, gy) for L in range(xcd, gx * gy, 8)]
             flat = [x + gx * y for x, y in order]
             assert flat == list(range(flat[0], flat[0] + len(flat)))
+
+
+def _stage(M, V, R, L, sign, src, dst, lanes):
+    """One Stockham stage (radix R, sub-length L) of the kernel's vstage for the lanes `lanes`: (i, load, store)
+    with load(e) -> the element e of the line's input (registers or LDS slot) and store(e, value) -> its output;
+    returns nothing, the lanes' reads all happen before their writes (the wavefront barrier between them)."""
+    T, NB, STRIDE = M // V, V // R, M // R
+    roots = np.exp(-2j * np.pi * np.arange(M) / M)
+    outs = []
+    for i, load, store in lanes:
+        for q in range(NB):
+            bi = i + T * q
+            k = 0 if L == 1 else bi % L
+            u = np.array([load(i, q + NB * j, bi + STRIDE * j) for j in range(R)], dtype=complex)
+            for j in range(1, R):
+                if L > 1:
+                    w = roots[j * k * (M // (L * R))]
+                    u[j] *= w if sign < 0 else np.conj(w)
+            u = dft(u, sign)
+            outs += [(store, i, q + NB * j, (bi - k) * R + k + j * L, bi + STRIDE * j, u[j]) for j in range(R)]
+    for store, i, reg, slot_e, nat_e, val in outs:
+        store(i, reg, slot_e, nat_e, val)
+
+
+@pytest.mark.parametrize("M,TC,CS", [(384, 8, 400), (384, 16, 400), (1024, 8, 1028)])
+def test_fused_column_pass_through_the_tile_equals_numpy(M, TC, CS):
+    """The register column pass with the first and last stages in the block phases (col_reg_kernel, not
+    Bluestein's): block-phase thread t = (column t mod TC, lane t / TC) runs the forward transform's first stage from
+    its loaded rows into the tile; each column's lanes run the second and last stages, the product with P (zero in
+    the antialias band), the inverse's first (band inputs zero) and second stages; the block phase runs the inverse's
+    last stage from the tile (last_stage_store) and stores rows bi + (M / R3) j. Through the tile's slot map (rslot,
+    XORed per column at 384 points) the result is IFFT(FFT(g) P) for every column."""
+    V, radices, _ = PLANS[M]
+    R1, R2, R3 = radices
+    T = M // V
+    rng = np.random.default_rng(M + TC)
+    g = rng.normal(size=(M, TC)) + 1j * rng.normal(size=(M, TC))
+    band = np.array([_in_band(M, r) for r in range(V)])
+    P = rng.normal(size=(M, TC)) + 1j * rng.normal(size=(M, TC))
+    for e in range(M):
+        if band[e // T]:
+            P[e] = 0
+
+    def elem(e, c):
+        return rslot(e) ^ ((4 * c if TC == 8 else c) & 31) if M == 384 else rslot(e)
+
+    tile = np.full(TC * CS, np.nan, dtype=complex)
+    block = [(t % TC, t // TC) for t in range(TC * T)]
+    # block phase: the forward transform's first stage, thread (cb, xi) as lane xi of column cb
+    for cb in range(TC):
+        lanes = [(xi, lambda i, r, e, cb=cb: g[i + T * r, cb],
+                  lambda i, r, s, n, val, cb=cb: tile.__setitem__(cb * CS + elem(s, cb), val))
+                 for (c, xi) in block if c == cb]
+        _stage(M, V, R1, 1, -1, None, None, lanes)
+    out = np.zeros((M, TC), complex)
+    for wc in range(TC):
+        v = np.zeros((T, V), complex)
+        rd = lambda i, r, e, wc=wc: tile[wc * CS + elem(e, wc)]
+        wr = lambda i, r, s, n, val, wc=wc: tile.__setitem__(wc * CS + elem(s, wc), val)
+        to_reg = lambda i, r, s, n, val: v.__setitem__((i, r), val)
+        _stage(M, V, R2, R1, -1, None, None, [(i, rd, wr) for i in range(T)])
+        _stage(M, V, R3, R1 * R2, -1, None, None, [(i, rd, to_reg) for i in range(T)])
+        for i in range(T):
+            for r in range(V):
+                v[i, r] = 0 if band[r] else v[i, r] * P[i + T * r, wc]
+        from_reg = lambda i, r, e: 0 if band[r] else v[i, r]
+        _stage(M, V, R1, 1, +1, None, None, [(i, from_reg, wr) for i in range(T)])
+        _stage(M, V, R2, R1, +1, None, None, [(i, rd, wr) for i in range(T)])
+    # block phase: the inverse's last stage, outputs to their natural rows
+    for cb in range(TC):
+        rd = lambda i, r, e, cb=cb: tile[cb * CS + elem(e, cb)]
+        st = lambda i, r, s, n, val, cb=cb: out.__setitem__((n, cb), val)
+        _stage(M, V, R3, R1 * R2, +1, None, None, [(xi, rd, st) for (c, xi) in block if c == cb])
+    want = np.fft.ifft(np.fft.fft(g, axis=0) * P, axis=0) * M
+    assert np.abs(out - want).max() <= 1e-12 * np.abs(want).max()
