@@ -106,6 +106,22 @@ def test_radial_profile_over_frozen_phonons_equals_the_oracle():
     assert np.all(want.sum(axis=1) <= 1.0 + 1e-9) and np.all(got.values.reshape(4, 8).sum(axis=1) > 0.5)
 
 
+@pytest.mark.parametrize("n", [256, 384, 509, 512, 1024])
+def test_radial_profile_through_the_register_and_fast_passes_equals_the_oracle(n):
+    # the intensity scan's slice loop runs the fast passes (256, 512) or the register passes (384, 1024, and 509 by
+    # Bluestein on 1024 points) for its first and middle column passes, and the library's other passes for the last
+    st = perovskite((2, 2, 1))
+    pot = sk.Potential(st, n, n, slice_thickness=2.0)
+    probe = sk.Probe(sk.ElectronBeam(200e3), pot.grid, sk.Aperture(20.0))
+    positions = np.array([[0.0, 0.0], [1.2, 3.1]])
+    det = RadialProfile()
+    got = sk.STEM.scan(probe, [pot], positions, [det])[0]
+    want = oracle_custom([st], n, 2.0, positions, det)
+    err = rel_max(got.values.reshape(2, 8), want)
+    print(f"{n}x{n} radial profile, 2 positions: rel max vs float64 oracle {err:.2e}", flush=True)
+    assert err <= 2e-4
+
+
 def test_a_python_annular_detector_equals_the_builtin_one():
     st = perovskite()
     pot = sk.Potential(st, 128, 128, slice_thickness=2.0)
