@@ -237,7 +237,12 @@ __device__ __forceinline__ void vfft(float2 (&v)[RegPlan<M>::V], float2* buf, in
 // with the chirp cs[n] and the kernel transform ks[1024] (both in LDS; conjugated for the inverse). Values at
 // elements e >= n are ignored on entry and undefined on return.
 // BAND (not Bluestein's): the input is zero in the antialias band.
-template <int M, int SIGN, bool BLUE, class Slot, bool BAND = false>
+// CHIRP_IN / CHIRP_OUT (Bluestein's): false leaves out the opening / closing chirp product. An inverse transform
+// without its closing product followed, after a pointwise product, by a forward one without its opening product
+// (or a forward one then an inverse one) computes the same: the two chirps are conj(c_e) and c_e (or c_e and
+// conj(c_e)), whose product is 1; an intensity |X_k|^2 does not see the closing product either. Without CHIRP_IN
+// the elements e >= n are set to zero here as the opening product does.
+template <int M, int SIGN, bool BLUE, class Slot, bool BAND = false, bool CHIRP_IN = true, bool CHIRP_OUT = true>
 __device__ __forceinline__ void line_fft(float2 (&v)[RegPlan<M>::V], float2* buf, int i, const float2* tab,
                                          const float2* cs, const float2* ks, int n, Slot slot) {
   static_assert(!(BAND && BLUE), "the band's zeros are known at compile time for M-point lines only");
@@ -249,8 +254,10 @@ __device__ __forceinline__ void line_fft(float2 (&v)[RegPlan<M>::V], float2* buf
     for (int r = 0; r < V; ++r) {
       const int e = i + T * r;
       if (e < n) {
-        const float2 c = cs[e];
-        v[r] = SIGN < 0 ? cmul(v[r], c) : cmulc(v[r], c);
+        if constexpr (CHIRP_IN) {
+          const float2 c = cs[e];
+          v[r] = SIGN < 0 ? cmul(v[r], c) : cmulc(v[r], c);
+        }
       } else {
         v[r] = make_float2(0.f, 0.f);
       }
@@ -270,7 +277,7 @@ __device__ __forceinline__ void line_fft(float2 (&v)[RegPlan<M>::V], float2* buf
       const int e = i + T * r;
       if (r >= V / 2) {
         v[r] = make_float2(0.f, 0.f);
-      } else if (e < n) {
+      } else if (CHIRP_OUT && e < n) {
         const float2 c = cs[e];
         v[r] = SIGN < 0 ? cmul(v[r], c) : cmulc(v[r], c);
       }
@@ -327,6 +334,10 @@ __global__ void __launch_bounds__(64 * kRowWaves) row_reg_kernel(RowArgs a, cons
       treg[r] = e < n ? tr[e] : make_float2(0.f, 0.f);
     }
   }
+  // Bluestein's inverse, the product with t and Bluestein's forward in a row (no exit plane between them): the
+  // inverse's closing chirp and the forward's opening one cancel, and neither is applied (t is zero past n)
+  constexpr bool CHIRP_PAIR = BLUE && (FLAGS & ROW_INV) != 0 && (FLAGS & ROW_MUL_T) != 0 && (FLAGS & ROW_FWD) != 0 &&
+                              (FLAGS & ROW_WRITE_PLANE) == 0;
   for (int p = 0; p < kRegPB; ++p) {
     const long long b = b0 + p;
     if (b >= batch) break;  // uniform over the block
@@ -354,7 +365,7 @@ __global__ void __launch_bounds__(64 * kRowWaves) row_reg_kernel(RowArgs a, cons
     if constexpr ((FLAGS & ROW_INV) != 0) {
       // from the compact wave, the band's elements are zero
       constexpr bool band = !BLUE && (FLAGS & ROW_IN_COMPACT) != 0;
-      line_fft<M, +1, BLUE, decltype(slot), band>(v, buf, il, tab, cs, ks, n, slot);
+      line_fft<M, +1, BLUE, decltype(slot), band, true, !CHIRP_PAIR>(v, buf, il, tab, cs, ks, n, slot);
     }
     if constexpr ((FLAGS & ROW_WRITE_PLANE) != 0) {
       if (live) {
@@ -374,7 +385,8 @@ __global__ void __launch_bounds__(64 * kRowWaves) row_reg_kernel(RowArgs a, cons
         else v[r] = e < n ? cmul(v[r], tr[e]) : make_float2(0.f, 0.f);
       }
     }
-    if constexpr ((FLAGS & ROW_FWD) != 0) line_fft<M, -1, BLUE>(v, buf, il, tab, cs, ks, n, slot);
+    if constexpr ((FLAGS & ROW_FWD) != 0)
+      line_fft<M, -1, BLUE, decltype(slot), false, !CHIRP_PAIR>(v, buf, il, tab, cs, ks, n, slot);
     if constexpr ((FLAGS & ROW_OUT_COMPACT) != 0) {
       if (live) {
         float2* g = a.G + b * gw + xs * a.nc;
@@ -413,9 +425,9 @@ __device__ __forceinline__ int tile_elem(int e, int c) {
 // Column pass: block = TC compact columns [TC tile, + TC) of positions [b0, b0 + kRegPB) of its unit (xcd_unit);
 // TC x T threads, the T lanes of column c = tid / T. MODE 0: a middle slice (COL_IN_G | COL_FWD | COL_MUL_P | COL_INV |
 // COL_OUT); 1: the first pass, whose input is the probe spectrum times the position's ramps, generated in the block
-// phase (COL_GENERATE | COL_INV | COL_OUT); 2: the last pass, which ends in the detectors (COL_IN_G | COL_FWD | COL_MUL_P |
-// COL_DETECT): each thread sums its elements' contributions to one detector slot at a time, pixels in a fixed order,
-// then a tree over the block writes the block's partial sum (partial[position][slot][tile]).
+// phase (COL_GENERATE | COL_INV | COL_OUT); 2: the last pass, which ends in the detectors (COL_IN_G | COL_FWD |
+// COL_MUL_P | COL_DETECT): each thread sums its elements' contributions to one detector slot at a time, pixels in a
+// fixed order, then a tree over the block writes the block's partial sum (partial[position][slot][tile]).
 template <int M, bool BLUE, int TC, int MODE>
 __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColArgs a, const float2* __restrict__ roots,
                                                                        const float2* __restrict__ chirp,
@@ -534,7 +546,10 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
         v[r] = e < n ? col[slot(e)] : make_float2(0.f, 0.f);
       }
       wave_sync();  // every lane has read its column before the exchanges overwrite it
-      if constexpr (MODE != 1) line_fft<M, -1, BLUE>(v, col, il, tab, cs, ks, n, slot);
+      // Bluestein's forward without its closing chirp: the inverse after the product with P (zero past n) leaves
+      // out its opening one (mode 0), and the detectors sum intensities (mode 2)
+      if constexpr (MODE != 1)
+        line_fft<M, -1, BLUE, decltype(slot), false, true, false>(v, col, il, tab, cs, ks, n, slot);
     }
     if constexpr (MODE != 1) {
 #pragma unroll
@@ -607,7 +622,7 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
       asm volatile("" : "+v"(xs));
       if (kept) last_stage_store<M>(bcol, xs, tab + RegTw<M>::S2, bslot, g, a.nc);
     } else {
-      line_fft<M, +1, BLUE, decltype(slot), !BLUE && MODE == 0>(v, col, il, tab, cs, ks, n, slot);
+      line_fft<M, +1, BLUE, decltype(slot), !BLUE && MODE == 0, MODE != 0>(v, col, il, tab, cs, ks, n, slot);
 #pragma unroll
       for (int r = 0; r < V; ++r) {
         const int e = il + T * r;

@@ -362,3 +362,39 @@ def test_fused_column_pass_through_the_tile_equals_numpy(M, TC, CS):
         _stage(M, V, R3, R1 * R2, +1, None, None, [(xi, rd, st) for (c, xi) in block if c == cb])
     want = np.fft.ifft(np.fft.fft(g, axis=0) * P, axis=0) * M
     assert np.abs(out - want).max() <= 1e-12 * np.abs(want).max()
+
+
+def _bluestein(x, n, sign, chirp_in=True, chirp_out=True):
+    """line_fft's Bluestein transform of n points on 1024 (sk_reg.hip), with or without its opening and closing chirp
+    products; elements past n are zero on the way in and dropped on the way out."""
+    M = 1024
+    k = np.arange(n)
+    chirp = np.exp(-1j * np.pi * ((k * k) % (2 * n)) / n)
+    b = np.zeros(M, complex)
+    b[0] = np.conj(chirp[0])
+    for q in range(1, n):
+        b[q] = b[M - q] = np.conj(chirp[q])
+    khat = np.fft.fft(b) / M
+    c, kh = (chirp, khat) if sign < 0 else (np.conj(chirp), np.conj(khat))
+    u = np.zeros(M, complex)
+    u[:n] = x[:n] * c if chirp_in else x[:n]
+    w = vfft(vfft(u, M, 16, (16, 8, 8), -1) * kh, M, 16, (16, 8, 8), +1)[:n]
+    return w * c if chirp_out else w
+
+
+@pytest.mark.parametrize("n", [257, 509])
+def test_adjacent_bluestein_chirps_cancel(n):
+    """The row pass's inverse transform without its closing chirp, the product with t, and the forward transform
+    without its opening chirp give FFT(t IFFT(x)) n; the column pass's forward without its closing chirp, the product
+    with P and the inverse without its opening chirp give IFFT(FFT(x) P) n; the forward without its closing chirp
+    gives the same intensities |FFT(x)|^2."""
+    rng = np.random.default_rng(n)
+    x, t, P = (rng.normal(size=(n, 2)) @ np.array([1, 1j]) for _ in range(3))
+    row = _bluestein(_bluestein(x, n, +1, chirp_out=False) * t, n, -1, chirp_in=False)
+    want = np.fft.fft(t * np.fft.ifft(x) * n)
+    assert np.abs(row - want).max() <= 1e-12 * np.abs(want).max()
+    col = _bluestein(_bluestein(x, n, -1, chirp_out=False) * P, n, +1, chirp_in=False)
+    want = np.fft.ifft(np.fft.fft(x) * P) * n
+    assert np.abs(col - want).max() <= 1e-12 * np.abs(want).max()
+    inten = np.abs(_bluestein(x, n, -1, chirp_out=False)) ** 2
+    assert np.abs(inten - np.abs(np.fft.fft(x)) ** 2).max() <= 1e-12 * (np.abs(np.fft.fft(x)) ** 2).max()
