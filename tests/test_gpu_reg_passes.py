@@ -86,3 +86,22 @@ def test_column_widths_the_length_has_not_are_refused():
     beam = sk.ElectronBeam(200e3)
     with pytest.raises(sk.SimulatoError):
         sk.Multislice.run(sk.Waves.plane_wave(beam, pot.grid), pot, options=sk.ExecutionOptions(col_tile=4))
+
+
+@pytest.mark.parametrize("n", [384, 509, 1024])
+def test_register_passes_replayed_from_graphs_give_the_same_bits(n):
+    # 13 positions in batches of 4: the first batch of each shape runs kernel by kernel while its graph is captured,
+    # the next two replay it (the batch's first position read from device memory), the remainder of 1 runs once
+    st = srtio3(4, 4, 2)
+    pot = sk.Potential(st, n, n, slice_thickness=2.0)
+    probe = sk.Probe(sk.ElectronBeam(200e3), pot.grid, sk.Aperture(20.0))
+    positions = sk.GridScan((0, 0), (SRTIO3_A, SRTIO3_A), 4, 4).positions[:13]
+    prepared = pot.prepare(probe.beam)
+    dets = [sk.AnnularDetector(40, 100), sk.CenterOfMassDetector("cutoff", "mrad"), sk.PixelatedDetector("cutoff")]
+    eager = sk.STEM.scan(probe, [prepared], positions, dets, sk.ExecutionOptions(batch=4))
+    for streams in (1, 2):
+        graph = sk.STEM.scan(probe, [prepared], positions, dets,
+                             sk.ExecutionOptions(batch=4, use_graph=True, streams=streams))
+        for k, (x, y) in enumerate(zip(eager, graph)):
+            assert np.array_equal(x.values, y.values), (n, streams, k)
+    prepared.release()
