@@ -94,6 +94,27 @@ def test_noalloc_scan_equals_the_scan_twice_in_a_row(lib):
         p.release()
 
 
+@pytest.mark.parametrize("n", [384, 509, 1024])
+def test_noalloc_scan_on_the_register_passes(lib, n):
+    # the register passes (384 and 1024 points, 509 by Bluestein on 1024): the detecting pass's partial sums per
+    # column tile live in the scratch array as they do in the library's own workspace
+    L = lib
+    beam = sk.ElectronBeam(200e3)
+    pot = sk.Potential(srtio3(4, 4, 2), n, n, slice_thickness=2.0)
+    prepared = [pot.prepare(beam)]
+    probe = sk.Probe(beam, pot.grid, sk.Aperture(20.0))
+    pos = np.ascontiguousarray(sk.GridScan((0.0, 0.0), (SRTIO3_A, SRTIO3_A), 5, 5).positions)
+    dets = [sk.AnnularDetector(0, 20), sk.AnnularDetector(40, 100)]
+    print(f"{n} x {n}, 1 potential, 25 positions", flush=True)
+    got, T = _call(L, L.sk_hip_scan_noalloc, prepared, probe, pos, dets, _opt())
+    assert 1 <= T <= 25
+    want, Tw = _call(L, L.sk_hip_scan, prepared, probe, pos, dets, _opt(batch=T, streams=1))
+    assert Tw == T
+    for k in range(len(dets)):
+        _compare(f"{n} x {n} detector {k} (T = {T})", got[k], want[k])
+    prepared[0].release()
+
+
 def test_noalloc_scan_refuses_what_its_scratch_cannot_hold(lib):
     L = lib
     beam = sk.ElectronBeam(200e3)
