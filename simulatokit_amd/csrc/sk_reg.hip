@@ -286,6 +286,7 @@ __device__ __forceinline__ void line_fft(float2 (&v)[RegPlan<M>::V], float2* buf
 }
 
 constexpr int kRegPB = 8;      // positions per block (t or P loaded once per 8 positions)
+constexpr int kWaitVmcnt0 = 0x0F70;  // s_waitcnt vmcnt(0) expcnt(7) lgkmcnt(15): vector memory only (gfx9 fields)
 constexpr int kRowWaves = 2;   // wavefronts per row-pass block
 
 __device__ __forceinline__ int compact_col(const RowArgs& a, int iy) {
@@ -468,7 +469,7 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
     }
   }
   const int iyb = a.keep[c0 + cb];
-  const bool kept = iyb >= 0;  // padding columns are never read nor written
+  const bool kept = iyb >= 0;  // padding columns are never read (and written only by the FUSE block phase's stores)
   const int b0 = unit.y * kRegPB;
   // The block phase's values of the next position are loaded into registers while this position transforms: the
   // loads' latency then overlaps the arithmetic (measured: 1024 points, 8 columns, 6.76 -> 3.95 ms per launch of
@@ -500,6 +501,13 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
   float2* bcol = tile + cb * CS;
   auto bslot = [cb](int e) { return tile_elem<M, TC>(e, cb); };
   if (b0 < batch) prefetch(b0);
+  // The first position's loads complete before the loop. At the loop's head the compiler waits for the loads of
+  // the position about to run; the paths into the head are the first entry (those loads outstanding, nothing after
+  // them) and the back edge (those loads, then the previous position's stores): merged, they would make it wait for
+  // every outstanding access, the stores just issued included, once per position. With the first entry's loads
+  // done, the head waits only for the loads (s_waitcnt vmcnt(V), the V stores still in flight); the FUSE stores are
+  // unconditional for the same reason (below).
+  if constexpr (FUSE) __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
   for (int p = 0; p < kRegPB; ++p) {
     const long long b = b0 + p;
     if (b >= batch) break;  // uniform over the block
@@ -620,7 +628,10 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
       // outputs
       int xs = xi;
       asm volatile("" : "+v"(xs));
-      if (kept) last_stage_store<M>(bcol, xs, tab + RegTw<M>::S2, bslot, g, a.nc);
+      // every column is stored, padding columns too (nothing reads them, and their values are zeros from zero
+      // inputs): a store that only kept columns take would put a branch in the loop and bring back the head's wait
+      // for all outstanding accesses
+      last_stage_store<M>(bcol, xs, tab + RegTw<M>::S2, bslot, g, a.nc);
     } else {
       line_fft<M, +1, BLUE, decltype(slot), !BLUE && MODE == 0, MODE != 0>(v, col, il, tab, cs, ks, n, slot);
 #pragma unroll
