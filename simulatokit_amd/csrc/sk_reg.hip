@@ -469,12 +469,14 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
     }
   }
   const int iyb = a.keep[c0 + cb];
-  const bool kept = iyb >= 0;  // padding columns are never read (and written only by the FUSE block phase's stores)
+  const bool kept = iyb >= 0;  // padding columns: their values are never used (the block phases store zeros there)
   const int b0 = unit.y * kRegPB;
   // The block phase's values of the next position are loaded into registers while this position transforms: the
   // loads' latency then overlaps the arithmetic (measured: 1024 points, 8 columns, 6.76 -> 3.95 ms per launch of
   // 1024 positions; 384 points 3.03 -> 2.28 ms per 4096).
-  constexpr int NL = (M + RSTEP - 1) / RSTEP;
+  // the rows a block-phase thread moves: every row of the M-point line, or (Bluestein's) the rows below 512 >= n
+  constexpr int NL = (BLUE ? 512 : M) / RSTEP;
+  static_assert((BLUE ? 512 : M) % RSTEP == 0, "whole rounds of rows");
   float2 pre[MODE == 1 ? 1 : NL];
   // The first pass generates each position's probe where it writes the tile (prefetching the three factors of
   // every element measured slower at 384 points and the same at 1024).
@@ -482,13 +484,24 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
     return kept ? cmul(cmul(a.spec[(long long)x * a.ny + iyb], a.rx[bb * a.nx + x]), a.ry[bb * a.ny + iyb])
                 : make_float2(0.f, 0.f);
   };
+  auto masked = [&](int r) {  // Bluestein's prefetched value of row xi + RSTEP r, or zero
+    return (kept && xi + RSTEP * r < n) ? pre[r] : make_float2(0.f, 0.f);
+  };
   auto prefetch = [&](long long bb) {
     if constexpr (MODE != 1) {
       const float2* gg = a.G + bb * gw + c0 + cb;
 #pragma unroll
       for (int r = 0; r < NL; ++r) {
+        // branch-free loads of every column (padding columns are read too, and their values replaced by zeros)
         const int x = xi + RSTEP * r;
-        pre[r] = (kept && x < n) ? gg[(long long)x * a.nc] : make_float2(0.f, 0.f);
+        if constexpr (BLUE) {
+          // rows past n (Bluestein's rows n..511) read row n - 1; the zeros are put in where the values are used
+          // (masked), since a select here made the compiler wait for these loads at once
+          pre[r] = gg[(long long)(x < n ? x : n - 1) * a.nc];
+        } else {
+          const float2 t = gg[(long long)x * a.nc];
+          pre[r] = kept ? t : make_float2(0.f, 0.f);
+        }
       }
     }
   };
@@ -505,9 +518,9 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
   // the position about to run; the paths into the head are the first entry (those loads outstanding, nothing after
   // them) and the back edge (those loads, then the previous position's stores): merged, they would make it wait for
   // every outstanding access, the stores just issued included, once per position. With the first entry's loads
-  // done, the head waits only for the loads (s_waitcnt vmcnt(V), the V stores still in flight); the FUSE stores are
-  // unconditional for the same reason (below).
-  if constexpr (FUSE) __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+  // done, the head waits only for the loads (s_waitcnt vmcnt(NL), the NL stores still in flight); the stores and
+  // the loads take no branch for the same reason (below).
+  __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
   for (int p = 0; p < kRegPB; ++p) {
     const long long b = b0 + p;
     if (b >= batch) break;  // uniform over the block
@@ -529,10 +542,11 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
     } else {
 #pragma unroll
       for (int r = 0; r < NL; ++r) {
-        const int x = xi + RSTEP * r;
-        if (x < n) {
-          if constexpr (MODE == 1) tile[cb * CS + tile_elem<M, TC>(x, cb)] = generate(b, x);
-          else tile[cb * CS + tile_elem<M, TC>(x, cb)] = pre[r];
+        const int x = xi + RSTEP * r;  // below 512: a slot of the tile; the transform ignores the elements past n
+        if constexpr (MODE == 1) {
+          if (x < n) tile[cb * CS + tile_elem<M, TC>(x, cb)] = generate(b, x);
+        } else {
+          tile[cb * CS + tile_elem<M, TC>(x, cb)] = masked(r);
         }
       }
     }
@@ -640,8 +654,14 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
         if (e < n) col[slot(e)] = v[r];
       }
       __syncthreads();
-      if (kept)
-        for (int x = xi; x < n; x += RSTEP) g[(long long)x * a.nc] = tile[cb * CS + tile_elem<M, TC>(x, cb)];
+      // every column and NL rows a thread, as the FUSE stores (padding columns get zeros): a row past n stores the
+      // lane's last row below n again, the same value to the same address (n >= 257 > RSTEP: every lane has one)
+      const int xlast = xi + RSTEP * ((n - 1 - xi) / RSTEP);
+#pragma unroll
+      for (int r = 0; r < NL; ++r) {
+        const int x0 = xi + RSTEP * r, x = x0 < n ? x0 : xlast;
+        g[(long long)x * a.nc] = tile[cb * CS + tile_elem<M, TC>(x, cb)];
+      }
     }
   }
 }
