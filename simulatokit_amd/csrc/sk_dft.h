@@ -21,6 +21,11 @@ __device__ __forceinline__ int2 xcd_unit() {
   return make_int2(u % gx, u / gx);
 }
 
+// The immediate of s_waitcnt vmcnt(0) expcnt(7) lgkmcnt(15) (gfx9 fields): a wait for vector memory accesses only,
+// for __builtin_amdgcn_s_waitcnt. A column pass completes its first unit's prefetch with it before its loop, so that
+// the compiler's wait at the loop's head covers the prefetched loads and not the stores issued after them.
+constexpr int kWaitVmcnt0 = 0x0F70;
+
 __device__ inline float2 mul_si(float2 a, int sign) {  // a * (sign * i)
   return sign < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
 }

@@ -456,14 +456,18 @@ __device__ __forceinline__ void col512_run(const ColArgs& a, int u0, int u1, Whe
   auto elem = [loff, rstep](const float2* t0, int r) {
     return reinterpret_cast<const float2*>(reinterpret_cast<const char*>(t0) + r * rstep + loff);
   };
-  // the block-phase load of unit u: column cb of its tile, rows xi + 64 r
+  // the block-phase load of unit u: column cb of its tile, rows xi + 64 r. From G every column is loaded, padding
+  // columns too (gk false: their values are replaced by zeros where they are used), without a branch that would
+  // make the compiler's waits for these loads cover every access before them
+  bool gk = false;
   auto gload = [&](int u, float2 (&g)[8]) {
     const ColUnit U = where(u);
     const int c = U.tx * TC + cb, iy = a.keep[c];
     if constexpr ((FLAGS & COL_IN_G) != 0) {
       const float2* t0 = a.G + U.b * gw + U.tx * TC;
+      gk = iy >= 0;
 #pragma unroll
-      for (int r = 0; r < 8; ++r) g[r] = iy >= 0 ? *elem(t0, r) : make_float2(0.f, 0.f);
+      for (int r = 0; r < 8; ++r) g[r] = *elem(t0, r);
     } else {
 #pragma unroll
       for (int r = 0; r < 8; ++r) {
@@ -497,6 +501,7 @@ __device__ __forceinline__ void col512_run(const ColArgs& a, int u0, int u1, Whe
       pload(tcur, U.P);
     }
   }
+  __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);  // the first unit's loads, before the loop
   for (int u = u0; u < u1; ++u) {
     if constexpr (SKIPS) {
       if (!where(u).on) continue;  // uniform over the block
@@ -522,6 +527,10 @@ __device__ __forceinline__ void col512_run(const ColArgs& a, int u0, int u1, Whe
     auto f2 = [&w](int q, int j) { return w.s2(q, j); };
     auto f3 = [&w](int q, int j) { return w.s3(q, j); };
     auto bidx = [cb, tile_slot](int j) { return tile_slot(j, cb); };
+    if constexpr ((FLAGS & COL_IN_G) != 0) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) g[r] = gk ? g[r] : make_float2(0.f, 0.f);
+    }
     if constexpr (!FUSE) {
 #pragma unroll
       for (int r = 0; r < 8; ++r) tile[tile_slot(xi + T * r, cb)] = g[r];
@@ -581,10 +590,10 @@ __device__ __forceinline__ void col512_run(const ColArgs& a, int u0, int u1, Whe
       stage<N, 8, 64, +1, false, true, false>(o, tile, xi, f3b, bidx);
       const ColUnit U = where(u);
       const float2* t0 = a.G + U.b * gw + U.tx * TC;
-      if (a.keep[U.tx * TC + cb] >= 0) {  // padding columns stay as the row pass leaves them: never read or written
+      // every column is stored: padding columns get zeros (zero inputs, and P is zero there), which no pass reads;
+      // a store only kept columns take would put a branch here and bring back a wait for these stores at the head
 #pragma unroll
-        for (int r = 0; r < 8; ++r) *const_cast<float2*>(elem(t0, r)) = o[r];
-      }
+      for (int r = 0; r < 8; ++r) *const_cast<float2*>(elem(t0, r)) = o[r];
     }
   }
 }

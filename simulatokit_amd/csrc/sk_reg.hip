@@ -286,7 +286,6 @@ __device__ __forceinline__ void line_fft(float2 (&v)[RegPlan<M>::V], float2* buf
 }
 
 constexpr int kRegPB = 8;      // positions per block (t or P loaded once per 8 positions)
-constexpr int kWaitVmcnt0 = 0x0F70;  // s_waitcnt vmcnt(0) expcnt(7) lgkmcnt(15): vector memory only (gfx9 fields)
 constexpr int kRowWaves = 2;   // wavefronts per row-pass block
 
 __device__ __forceinline__ int compact_col(const RowArgs& a, int iy) {
