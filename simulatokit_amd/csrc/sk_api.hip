@@ -2034,8 +2034,9 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
     // each tile's positions split over the streams `st` in parts of H = ceil(T / k), stream q always on the
     // workspace positions [q H, q H + H): a stream runs its parts of consecutive tiles one after another, and no two
     // streams ever share workspace, whatever the tile's size.
+    // `only` >= 0: only stream `only`'s parts (a graph of that stream's share).
     auto enqueue_tiles = [&](const sk_potential& P, int b0, const int* pos_dev, int nb, Timer& t,
-                             const std::vector<hipStream_t>& st) {
+                             const std::vector<hipStream_t>& st, int only = -1) {
       const int tiles = (nb + T - 1) / T;
       const int Tp = (nb + tiles - 1) / tiles;
       const int k = (int)st.size();
@@ -2044,7 +2045,8 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
         const int first = i * Tp, count = std::min(Tp, nb - first);
         if (count <= 0) break;
         for (int q = 0; q < k && q * H < count; ++q)
-          enqueue_batch(P, b0 + first + q * H, pos_dev, std::min(H, count - q * H), t, st[q], q * H);
+          if (only < 0 || q == only)
+            enqueue_batch(P, b0 + first + q * H, pos_dev, std::min(H, count - q * H), t, st[q], q * H);
         if (k == 1) t.harvest(timing);
       }
     };
@@ -2095,11 +2097,14 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
       for (int k = 0; k < npots; ++k)
         for (int b0 = 0; b0 < npos; b0 += B) enqueue_tiles(*pots[k], b0, nullptr, std::min(B, npos - b0), tm, one);
     } else {
-      // hipGraph replay: per potential, one graph per batch shape (full, remainder), replayed for the batches of
-      // that shape after the first; the first position of a replay is read from device memory, copied in from
-      // pinned host memory before each launch. The first batch of a shape runs kernel by kernel while a second host
-      // thread records (captures) the same slice loop on streams of its own and instantiates it, so that the
-      // recording never leaves the GPU idle; a shape with a single batch is not captured at all.
+      // hipGraph replay: per potential, one set of graphs per batch shape (full, remainder), replayed for the
+      // batches of that shape after the first; the first position of a replay is read from device memory, copied in
+      // from pinned host memory before each launch. The first batch of a shape runs kernel by kernel while a second
+      // host thread records (captures) the same slice loop on streams of its own and instantiates it, so that the
+      // recording never leaves the GPU idle; a shape with a single batch is not captured at all. With the batch
+      // split over streams, each stream's share is a graph of its own (a chain of launches on that stream),
+      // launched on its stream between a fork and a join recorded outside the graphs: one graph holding both
+      // shares as forked branches replayed them with less overlap than the two streams give kernel by kernel.
       line_plan(nx, s);
       line_plan(ny, s);
       identity_map(ny);
@@ -2110,7 +2115,7 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
       for (int i = 0; i < nbatches; ++i) host_first[i] = i * B;
       w.pos_dev.ensure(1);
       Timer off;
-      // the fork and join events of split graphs (the capture records them as dependencies between the streams)
+      // the fork and join events of split replays
       struct Events {
         std::vector<hipEvent_t> e;
         ~Events() {
@@ -2124,7 +2129,8 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
       }
       hipEvent_t fork_ev = evs.e[0];
       hipEvent_t* join_ev = evs.e.data() + 1;
-      if (split > 1) (void)part_streams(split);  // create the side streams before any capture begins
+      // the streams the replays run on: the scan's own, then the side streams (created before any capture begins)
+      const std::vector<hipStream_t> rst = split > 1 ? part_streams(split) : std::vector<hipStream_t>(1, s);
       // the streams the graphs are recorded on: the capture runs on a second host thread while this one launches
       // the first batch on the scan's own streams, so neither waits for the other
       while ((int)c.capture.size() < split) {
@@ -2137,45 +2143,44 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
       (void)slice_plan(ny, s);
       int device = 0;
       SK_HIP(hipGetDevice(&device));
-      hipGraphExec_t exec[2] = {nullptr, nullptr};  // per potential; destroyed on the way out, also on errors
+      // [shape][stream]: per potential; destroyed on the way out, also on errors
+      std::vector<hipGraphExec_t> exec(2 * (size_t)split, nullptr);
+      auto destroy_execs = [&exec]() {
+        for (auto& e : exec) {
+          if (e) (void)hipGraphExecDestroy(e);
+          e = nullptr;
+        }
+      };
       try {
         for (int k = 0; k < npots; ++k) {
           bool seen[2] = {false, false};
           for (int i = 0; i < nbatches; ++i) {
             const int nb = std::min(B, npos - i * B);
             const int shape = nb == B ? 0 : 1;
-            hipGraphExec_t& ge = exec[shape];
+            hipGraphExec_t* ge = exec.data() + (size_t)shape * split;
             if (!seen[shape]) {
               seen[shape] = true;
-              // the capture: host-side recording of the same slice loop on the capture streams, then its
-              // instantiation; it launches nothing
+              // the capture: host-side recording of each stream's share of the slice loop on its capture stream,
+              // then its instantiation; it launches nothing
               auto capture = [&]() {
                 SK_HIP(hipSetDevice(device));
-                hipGraph_t gr;
-                hipStream_t cs = cst[0];
-                SK_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-                try {
-                  if (split > 1) {
-                    // the graph forks into the streams, runs the batch's tiles on them and joins them at its end
-                    SK_HIP(hipEventRecord(fork_ev, cs));
-                    for (int q = 1; q < split; ++q) SK_HIP(hipStreamWaitEvent(cst[q], fork_ev, 0));
-                    enqueue_tiles(*pots[k], 0, w.pos_dev.p, nb, off, cst);
-                    for (int q = 1; q < split; ++q) {
-                      SK_HIP(hipEventRecord(join_ev[q - 1], cst[q]));
-                      SK_HIP(hipStreamWaitEvent(cs, join_ev[q - 1], 0));
-                    }
-                  } else {
-                    enqueue_tiles(*pots[k], 0, w.pos_dev.p, nb, off, cst);
+                for (int q = 0; q < split; ++q) {
+                  hipGraph_t gr;
+                  hipStream_t cs = cst[q];
+                  SK_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+                  try {
+                    enqueue_tiles(*pots[k], 0, w.pos_dev.p, nb, off, cst, split > 1 ? q : -1);
+                  } catch (...) {
+                    hipGraph_t dropped;
+                    (void)hipStreamEndCapture(cs, &dropped);
+                    if (dropped) (void)hipGraphDestroy(dropped);
+                    throw;
                   }
-                } catch (...) {
-                  hipGraph_t dropped;
-                  (void)hipStreamEndCapture(cs, &dropped);
-                  if (dropped) (void)hipGraphDestroy(dropped);
-                  throw;
+                  SK_HIP(hipStreamEndCapture(cs, &gr));
+                  const hipError_t inst = hipGraphInstantiate(&ge[q], gr, nullptr, nullptr, 0);
+                  (void)hipGraphDestroy(gr);
+                  SK_HIP(inst);
                 }
-                SK_HIP(hipStreamEndCapture(cs, &gr));
-                SK_HIP(hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0));
-                SK_HIP(hipGraphDestroy(gr));
               };
               std::exception_ptr failed;
               std::thread recorder;
@@ -2198,18 +2203,24 @@ int sk_hip_scan(const sk_potential* const* pots, int npots, const sk_probe* prob
               continue;
             }
             SK_HIP(hipMemcpyAsync(w.pos_dev.p, host_first + i, sizeof(int), hipMemcpyHostToDevice, s));
-            SK_HIP(hipGraphLaunch(ge, s));
+            if (split > 1) {
+              SK_HIP(hipEventRecord(fork_ev, s));
+              for (int q = 1; q < split; ++q) SK_HIP(hipStreamWaitEvent(rst[q], fork_ev, 0));
+            }
+            for (int q = 0; q < split; ++q) SK_HIP(hipGraphLaunch(ge[q], rst[q]));
+            // joined before the next batch's copy of its first position, which the kernels above read
+            for (int q = 1; q < split; ++q) {
+              SK_HIP(hipEventRecord(join_ev[q - 1], rst[q]));
+              SK_HIP(hipStreamWaitEvent(s, join_ev[q - 1], 0));
+            }
           }
           SK_HIP(hipStreamSynchronize(s));
-          for (auto& e : exec) {
-            if (e) SK_HIP(hipGraphExecDestroy(e));
-            e = nullptr;
-          }
+          destroy_execs();
         }
       } catch (...) {
         (void)hipStreamSynchronize(s);
-        for (auto& e : exec)
-          if (e) (void)hipGraphExecDestroy(e);
+        for (int q = 1; q < (int)rst.size(); ++q) (void)hipStreamSynchronize(rst[q]);
+        destroy_execs();
         (void)hipHostFree(host_first);
         throw;
       }
