@@ -73,9 +73,9 @@ typedef struct sk_detector {
 typedef struct sk_options {
   int batch;        /* positions per launch (sk_hip_scan: per host chunk, or per graph replay with use_graph; the
                        other entry points: waves resident per kernel launch); 0 = the library's choice for the grid */
-  int use_graph;    /* capture the slice loop in hipGraphs and replay them: per potential, one graph per batch shape,
-                       recorded while the first batch of that shape runs kernel by kernel and replayed for the rest
-                       (a shape with one batch runs kernel by kernel only) */
+  int use_graph;    /* capture the slice loop in hipGraphs and replay them: per potential, one graph per batch shape
+                       and stream, recorded while the first batch of that shape runs kernel by kernel and replayed
+                       for the rest (a shape with one batch runs kernel by kernel only) */
   int fast_path;    /* 1 (default): register-resident FFT passes where the length has one; 0: generic LDS passes */
   int pipeline;     /* scan: 1 = lanes of positions advance one slice per launch, each through its own potential
                        and slice (potentials of different slice counts share every launch); 0 = one potential at a
@@ -89,8 +89,8 @@ typedef struct sk_options {
                        384, 512 and 1024, and the Bluestein lengths 65-512 that have a prime factor
                        above 13) */
   int streams;      /* scan, one potential at a time, fast passes, probe inside the bandlimit: the parts of each
-                       batch run their slice loops on this many streams (1-4; with use_graph, each batch's graph
-                       forks into the parts); 0 = the library's choice (2 for batches of at least 128 positions,
+                       batch run their slice loops on this many streams (1-4; with use_graph, each stream's share
+                       of a batch is a graph of its own); 0 = the library's choice (2 for batches of at least 128 positions,
                        with 3/4 of the cache-resident batch at 512 x 512, else 1); the lane pipeline and other
                        scans use one stream */
   int tile;         /* sk_hip_scan, one potential at a time: positions each kernel launch carries, a part of the
