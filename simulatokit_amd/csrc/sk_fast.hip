@@ -571,10 +571,9 @@ __device__ __forceinline__ void col512_run(const ColArgs& a, int u0, int u1, Whe
       __syncthreads();
       const ColUnit U = where(u);
       const float2* t0 = a.G + U.b * gw + U.tx * TC;
-      if (a.keep[U.tx * TC + cb] >= 0) {  // padding columns are not stored by this (1024-thread) variant
+      // every column is stored, as in the FUSE variant below (padding columns get zeros)
 #pragma unroll
-        for (int r = 0; r < 8; ++r) *const_cast<float2*>(elem(t0, r)) = tile[tile_slot(xi + T * r, cb)];
-      }
+      for (int r = 0; r < 8; ++r) *const_cast<float2*>(elem(t0, r)) = tile[tile_slot(xi + T * r, cb)];
       __syncthreads();
       continue;
     }
