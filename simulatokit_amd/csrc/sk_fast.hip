@@ -285,8 +285,15 @@ __global__ void __launch_bounds__(64) row_fast_kernel(RowArgs a, const float2* _
       float2* g = a.G + b * gw + (long long)x * a.nc;
       g[i] = v[0];
       g[i + T] = v[1];
-      if (cidx[2] >= 0) g[cidx[2]] = v[2];
-      if (cidx[5] >= 0) g[cidx[5]] = v[5];
+      // A lane whose element 2 or 5 is not kept stores a zero into one of the row's padding columns instead (lanes
+      // sharing one store the same zero), so that the row is written in whole lines, padding included; without
+      // padding columns it stores its element 0 or 1 again (the same value to the same address).
+      const int nkept = a.ny - a.kept_shift, npad = a.nc - nkept;
+      const int cpad = npad > 0 ? nkept + i % npad : -1;
+      g[cidx[2] >= 0 ? cidx[2] : (npad > 0 ? cpad : i)] =
+          cidx[2] >= 0 ? v[2] : (npad > 0 ? make_float2(0.f, 0.f) : v[0]);
+      g[cidx[5] >= 0 ? cidx[5] : (npad > 0 ? cpad : i + T)] =
+          cidx[5] >= 0 ? v[5] : (npad > 0 ? make_float2(0.f, 0.f) : v[1]);
       g[cidx[6]] = v[6];
       g[cidx[7]] = v[7];
     } else if constexpr ((FLAGS & ROW_OUT_COMPACT) != 0) {
