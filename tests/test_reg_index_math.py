@@ -398,3 +398,23 @@ def test_adjacent_bluestein_chirps_cancel(n):
     assert np.abs(col - want).max() <= 1e-12 * np.abs(want).max()
     inten = np.abs(_bluestein(x, n, -1, chirp_out=False)) ** 2
     assert np.abs(inten - np.abs(np.fft.fft(x)) ** 2).max() <= 1e-12 * (np.abs(np.fft.fft(x)) ** 2).max()
+
+
+@pytest.mark.parametrize("n", [256, 512])
+def test_band_row_pass_writes_every_padding_column(n):
+    """The band-pruned row pass (row_fast_kernel, sk_fast.hip) writes whole rows: a lane i whose element 2 or 5
+    (i + 2 T, i + 5 T; T = n / 8) is not kept stores a zero into padding column nkept + i mod npad. On the HAADF
+    grids every padding column gets one."""
+    T = n // 8
+    ky = np.fft.fftfreq(n, 1.0 / n)  # integer frequencies; the kept set is |k| <= floor(n / 3), as on square grids
+    kept = np.abs(ky) <= n // 3
+    nkept = int(kept.sum())
+    nc = -(-nkept // 16) * 16
+    npad = nc - nkept
+    assert npad > 0
+    written = set()
+    for i in range(T):
+        for r in (2, 5):
+            if not kept[i + r * T]:
+                written.add(nkept + i % npad)
+    assert written == set(range(nkept, nc))
