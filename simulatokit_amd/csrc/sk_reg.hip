@@ -16,11 +16,11 @@
 // closing chirp product.
 //
 //   row pass:    blocks of 2 wavefronts; each line is one row x of one position, IFFT_y -> * t(x, y) -> FFT_y (and
-//                the other row modes of the fast passes), the block's rows for 8 positions in turn (RegTuning: t in
+//                the other row modes of the fast passes), the block's rows for PB positions in turn (RegTuning: t in
 //                registers or read per position)
 //   column pass: FFT_x -> * P -> IFFT_x (middle slices), the probe-generating first pass and the detecting last one;
 //                a block moves TC compact columns of one position between HBM and LDS in row segments of TC values,
-//                each column's lanes transform it inside the block's LDS, P in registers for the block's 8 positions.
+//                each column's lanes transform it inside the block's LDS, P in registers for the block's PB positions.
 //                At 384 and 1024 points the block phases also run the first stage of the first transform (from the
 //                values they load) and the last stage of the inverse (into the values they store).
 
@@ -68,10 +68,14 @@ struct RegPlan<384> {
 //   8.21-8.23 against 8.85-8.86 ms for 8; with the fused first and last stages, 1024 at 8 columns 13.67-13.77 ms
 //   against 15.24 for 4, and 17.64 for 4 held to three wavefronts per SIMD (168 VGPRs, 48 bytes spilled). Blocks of
 //   1, 2 or 4 row wavefronts differ by under 2 %.
+// - PB: positions per block of both passes (t or P loaded, and the block's tables built, once per PB positions):
+//   16 at 1024 points (1024^2 174,400 -> 184,100, Bluestein 509 244,800 -> 251,300 position-slices/s against 8),
+//   8 at 384 (1.287 M against 1.252 M for 16).
 template <int M, bool BLUE>
 struct RegTuning {
   static constexpr bool T_REGS = M == 1024, ROW_OPAQUE = false, COL_OPAQUE = M == 384 || BLUE;
   static constexpr int COL_TILE = BLUE ? 4 : 8;
+  static constexpr int PB = M == 1024 ? 16 : 8;
 };
 
 __device__ __forceinline__ int rslot(int e) { return e ^ (((e >> 3) ^ (e >> 7)) & 31); }
@@ -285,14 +289,13 @@ __device__ __forceinline__ void line_fft(float2 (&v)[RegPlan<M>::V], float2* buf
   }
 }
 
-constexpr int kRegPB = 8;      // positions per block (t or P loaded once per 8 positions)
 constexpr int kRowWaves = 2;   // wavefronts per row-pass block
 
 __device__ __forceinline__ int compact_col(const RowArgs& a, int iy) {
   return iy <= a.kept_hi ? iy : (iy >= a.kept_lo ? iy - a.kept_shift : -1);
 }
 
-// Row pass: LINES = (64 / T) x kRowWaves rows x per block, positions [b0, b0 + kRegPB). Rows past nx (a grid length
+// Row pass: LINES = (64 / T) x kRowWaves rows x per block, positions [b0, b0 + PB). Rows past nx (a grid length
 // that is not a multiple of LINES) compute on the last row and store nothing.
 template <int M, int FLAGS, bool BLUE>
 __global__ void __launch_bounds__(64 * kRowWaves) row_reg_kernel(RowArgs a, const float2* __restrict__ roots,
@@ -315,7 +318,8 @@ __global__ void __launch_bounds__(64 * kRowWaves) row_reg_kernel(RowArgs a, cons
   const int x = blockIdx.x * LINES + line;
   const bool live = x < a.nx;
   const long long xs = live ? x : a.nx - 1;
-  const int b0 = blockIdx.y * kRegPB;
+  constexpr int PB = RegTuning<M, BLUE>::PB;
+  const int b0 = blockIdx.y * PB;
   float2* buf = lds + line * LS;
   auto slot = [](int e) { return rslot(e); };
   const long long gw = (long long)a.nx * a.nc;
@@ -338,7 +342,7 @@ __global__ void __launch_bounds__(64 * kRowWaves) row_reg_kernel(RowArgs a, cons
   // inverse's closing chirp and the forward's opening one cancel, and neither is applied (t is zero past n)
   constexpr bool CHIRP_PAIR = BLUE && (FLAGS & ROW_INV) != 0 && (FLAGS & ROW_MUL_T) != 0 && (FLAGS & ROW_FWD) != 0 &&
                               (FLAGS & ROW_WRITE_PLANE) == 0;
-  for (int p = 0; p < kRegPB; ++p) {
+  for (int p = 0; p < PB; ++p) {
     const long long b = b0 + p;
     if (b >= batch) break;  // uniform over the block
     // ROW_OPAQUE: the lane index as a value the compiler cannot follow across positions, so the lane's LDS slot
@@ -422,7 +426,7 @@ __device__ __forceinline__ int tile_elem(int e, int c) {
   else return rslot(e);
 }
 
-// Column pass: block = TC compact columns [TC tile, + TC) of positions [b0, b0 + kRegPB) of its unit (xcd_unit);
+// Column pass: block = TC compact columns [TC tile, + TC) of positions [b0, b0 + PB) of its unit (xcd_unit);
 // TC x T threads, the T lanes of column c = tid / T. MODE 0: a middle slice (COL_IN_G | COL_FWD | COL_MUL_P | COL_INV |
 // COL_OUT); 1: the first pass, whose input is the probe spectrum times the position's ramps, generated in the block
 // phase (COL_GENERATE | COL_INV | COL_OUT); 2: the last pass, which ends in the detectors (COL_IN_G | COL_FWD |
@@ -469,7 +473,8 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
   }
   const int iyb = a.keep[c0 + cb];
   const bool kept = iyb >= 0;  // padding columns: their values are never used (the block phases store zeros there)
-  const int b0 = unit.y * kRegPB;
+  constexpr int PB = RegTuning<M, BLUE>::PB;
+  const int b0 = unit.y * PB;
   // The block phase's values of the next position are loaded into registers while this position transforms: the
   // loads' latency then overlaps the arithmetic (measured: 1024 points, 8 columns, 6.76 -> 3.95 ms per launch of
   // 1024 positions; 384 points 3.03 -> 2.28 ms per 4096).
@@ -520,7 +525,7 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
   // done, the head waits only for the loads (s_waitcnt vmcnt(NL), the NL stores still in flight); the stores and
   // the loads take no branch for the same reason (below).
   __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-  for (int p = 0; p < kRegPB; ++p) {
+  for (int p = 0; p < PB; ++p) {
     const long long b = b0 + p;
     if (b >= batch) break;  // uniform over the block
     float2* g = a.G + b * gw + c0 + cb;
@@ -549,7 +554,7 @@ __global__ void __launch_bounds__(TC * (M / RegPlan<M>::V)) col_reg_kernel(ColAr
         }
       }
     }
-    if (p + 1 < kRegPB && b + 1 < batch) prefetch(b + 1);
+    if (p + 1 < PB && b + 1 < batch) prefetch(b + 1);
     __syncthreads();
     int il = i;  // COL_OPAQUE: as ROW_OPAQUE in the row pass
     if constexpr (RegTuning<M, BLUE>::COL_OPAQUE) asm volatile("" : "+v"(il));
@@ -689,7 +694,8 @@ int reg_length(const LineFFT& f) {
 template <int M, int F, bool BLUE>
 void row_go(const RowArgs& a, int batch, hipStream_t s) {
   constexpr int LINES = (64 / (M / RegPlan<M>::V)) * kRowWaves;
-  dim3 grid((a.nx + LINES - 1) / LINES, (batch + kRegPB - 1) / kRegPB);
+  constexpr int PB = RegTuning<M, BLUE>::PB;
+  dim3 grid((a.nx + LINES - 1) / LINES, (batch + PB - 1) / PB);
   launch_timed(row_reg_kernel<M, F, BLUE>, grid, dim3(64 * kRowWaves), s, a.ev_start, a.ev_stop, a, a.fy.roots,
                a.fy.chirp, a.fy.khat, batch);
 }
@@ -723,7 +729,8 @@ constexpr int COL_LAST = COL_IN_G | COL_FWD | COL_MUL_P | COL_DETECT;
 template <int M, bool BLUE, int TC, int MODE>
 void col_go_mode(const ColArgs& a, int batch, hipStream_t s) {
   constexpr int NT = TC * (M / RegPlan<M>::V);
-  dim3 grid(a.nc / TC, (batch + kRegPB - 1) / kRegPB);
+  constexpr int PB = RegTuning<M, BLUE>::PB;
+  dim3 grid(a.nc / TC, (batch + PB - 1) / PB);
   launch_timed(col_reg_kernel<M, BLUE, TC, MODE>, grid, dim3(NT), s, a.ev_start, a.ev_stop, a, a.fx.roots,
                a.fx.chirp, a.fx.khat, batch);
   SK_HIP(hipGetLastError());
