@@ -418,3 +418,22 @@ def test_band_row_pass_writes_every_padding_column(n):
             if not kept[i + r * T]:
                 written.add(nkept + i % npad)
     assert written == set(range(nkept, nc))
+
+
+def test_bluestein_block_phase_rows():
+    """Bluestein's column pass (col_reg_kernel, BLUE): a block-phase thread xi moves rows xi + 64 r for r < 8; a row
+    past n stores the lane's last row below n, xlast = xi + 64 floor((n - 1 - xi) / 64), again. For every n the pass
+    takes (257..512) and every lane that row exists, lies below n, is the lane's own and is its last; every row below
+    n is stored by exactly one (lane, r) as itself."""
+    for n in range(257, 513):
+        stored = []
+        for xi in range(64):
+            xlast = xi + 64 * ((n - 1 - xi) // 64)
+            assert xlast < n and xlast % 64 == xi and xlast + 64 >= n
+            for r in range(8):
+                x0 = xi + 64 * r
+                if x0 < n:
+                    stored.append(x0)
+                else:
+                    assert xlast in stored  # a repeat of a row the lane already stores
+        assert sorted(stored) == list(range(n))
